@@ -1,0 +1,205 @@
+"""Benchmark of the DDSCAT Green-tensor MATVEC (and Krylov solve) on one MI355X.
+
+Workload (default): rectangular block of 128 x 128 x 128 dipoles (2,097,152
+dipoles, padded FFT grid 256^3 x 3 components, complex64), m = 1.5+0.1i,
+x = k a_eff = 10 (kd = 0.126), incidence +x, E || y, LDR polarizabilities.
+Synthetic, seeded inputs; everything resident in HBM before timing starts.
+
+A "step" is one MATVEC y = A x on the compact solver vectors (5 FFT passes).
+Prints one JSON line: MATVECs/s, per-pass roofline of the dominant kernel,
+and the CPU baseline (the oracle's complex64 FFT MATVEC on host cores).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+for _p in (ROOT, ROOT / "ddscat-cuda_amd"):
+    if str(_p) not in sys.path:
+        sys.path.insert(0, str(_p))
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "matvecs/s + time-to-Q_ext at 1e-5 resid: batched mixed vs plain FP32, 1 GPU"
+
+
+def pass_bytes(nat: int, nat0: int, n: tuple[int, int, int]):
+    """Algorithmic HBM bytes per MATVEC pass (complex64, 3 components, pruned
+    5-pass minimum; green octant read once in pass 3)."""
+    nx, ny, nz = n
+    green = 48 * (nx + 1) * (ny + 1) * (nz + 1)
+    return [9 * nat * 8, 18 * nat * 8, 24 * nat * 8 + green, 18 * nat * 8, 18 * nat * 8]
+
+
+def setup_workload(lib, edge=128, m=1.5 + 0.1j, xsize=10.0, seed=20261202):
+    import numpy as np
+    import torch
+    from ddscat_hip import targets as T
+    tg = T.block(edge)
+    kd = T.kd_for_size_parameter(tg, xsize)
+    khat = np.array([1.0, 0.0, 0.0])
+    e0 = np.array([0, 1, 0], complex)
+    adia, aoff = T.operators(tg, m, kd, khat, e0)
+    lib.prepare_green(tg.nx, tg.ny, tg.nz, tg.iocc, khat * kd)
+    occ = tg.occ
+    comp = lambda v: np.ascontiguousarray(v.reshape(3, -1)[:, occ].reshape(-1))
+    rng = np.random.default_rng(seed)
+    n0 = 3 * tg.nat0
+    x = (rng.normal(size=n0) + 1j * rng.normal(size=n0)).astype(np.complex64)
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a).view(np.float32)).cuda()
+    bufs = dict(x=dev(x), y=torch.empty(2 * n0, dtype=torch.float32, device="cuda"),
+                adia=dev(comp(adia)), aoff=dev(comp(aoff)))
+    torch.cuda.synchronize()
+    return tg, kd, bufs, (x, adia, aoff)
+
+
+def cpu_baseline(tg, kd, host, budget_s=20.0):
+    """Oracle FP32 (complex64) FFT MATVEC on all host cores (scipy.fft workers)."""
+    import numpy as np
+    from oracle import ddscat_oracle as O
+    x, adia, aoff = host
+    xf = np.zeros(3 * tg.nat, np.complex64)
+    xf.reshape(3, -1)[:, tg.occ] = x.reshape(3, -1)
+    goct = _green_octant_fast(tg.nx, tg.ny, tg.nz, float(np.float32(kd)))
+    cores = os.cpu_count() or 1
+    mv = O.CpuMatvecFP32(goct, tg.nx, tg.ny, tg.nz, workers=cores)
+    mv.apply("N", xf, adia, aoff)
+    t0 = time.perf_counter()
+    n = 0
+    while True:
+        mv.apply("N", xf, adia, aoff)
+        n += 1
+        dt = time.perf_counter() - t0
+        if dt > budget_s or n >= 50:
+            break
+    return dict(value=round(n / dt, 4), unit="matvecs/s", cores=cores, kind="port",
+                sample=f"{n} complex64 FFT MATVECs of the {tg.nx}^3 block (oracle CpuMatvecFP32: scipy.fft "
+                       f"workers={cores} on the full padded grid, Green expanded once) in {dt:.1f} s")
+
+
+def _green_octant_fast(nx, ny, nz, akd):
+    """complex128 spectral octant via separable even/odd transforms (same
+    result as oracle.green_octant, (N+1)^3 closed-form evaluations)."""
+    import numpy as np
+    from oracle import ddscat_oracle as O
+    ls = [np.arange(n + 1) for n in (nx, ny, nz)]
+    Z, Y, X = np.meshgrid(ls[2], ls[1], ls[0], indexing="ij")
+    g = O.green_elements(X * 1.0, Y * 1.0, Z * 1.0, akd)
+    g[:, nz] = 0; g[:, :, ny] = 0; g[:, :, :, nx] = 0
+
+    def tmat(N, odd):
+        k = np.arange(N + 1)[:, None]; l = np.arange(N + 1)[None, :]
+        M = 2 * (np.sin if odd else np.cos)(np.pi * k * l / N)
+        M[:, 0] = 0 if odd else 1
+        M[:, N] = 0
+        return M
+    out = np.empty_like(g)
+    for m, par in enumerate(O.COMP_PARITY):
+        a = np.einsum("kl,zyl->zyk", tmat(nx, par[0]), g[m], optimize=True)
+        a = np.einsum("kl,zlx->zkx", tmat(ny, par[1]), a, optimize=True)
+        a = np.einsum("kl,lyx->kyx", tmat(nz, par[2]), a, optimize=True)
+        out[m] = (-1 if m in (1, 2, 4) else 1) * a
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--edge", type=int, default=128)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--backend", choices=["fft3d", "slices"], default="fft3d")
+    ap.add_argument("--pmc-calib", action="store_true", help="run one torch copy of known size (rocprof scale)")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build measures one GPU only (single-GPU box); --gpus must be 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from ddscat_hip import Library, BACKEND_SLICES, BACKEND_FFT3D
+    assert torch.cuda.is_available(), "bench.py needs an MI355X"
+    lib = Library()
+    lib.set_backend(BACKEND_SLICES if args.backend == "slices" else BACKEND_FFT3D)
+    tg, kd, bufs, host = setup_workload(lib, edge=args.edge)
+    info = lib.info()
+    st = torch.cuda.ExternalStream(lib.stream())
+    xp, yp, ap_, op = (bufs[k].data_ptr() for k in ("x", "y", "adia", "aoff"))
+
+    if args.pmc_calib:
+        src = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device="cuda")  # 256 MiB
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+
+    for _ in range(args.warmup):
+        lib.apply_device("N", xp, yp, ap_, op, layout=1)
+    lib.sync()
+    torch.cuda.synchronize()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    with torch.cuda.stream(st):
+        ev0.record(st)
+        for _ in range(args.steps):
+            lib.apply_device("N", xp, yp, ap_, op, layout=1)
+        ev1.record(st)
+    ev1.synchronize()
+    lib.sync()
+    torch.cuda.synchronize()
+    ms_total = ev0.elapsed_time(ev1)
+    ms_step = ms_total / args.steps
+    value = 1000.0 / ms_step
+
+    passes = lib.time_matvec("N", xp, yp, ap_, op, layout=1, reps=max(10, min(args.steps, 100)))
+    pb = pass_bytes(tg.nat, tg.nat0, (tg.nx, tg.ny, tg.nz))
+    per_pass = []
+    for i in range(5):
+        gbs = pb[i] / (passes[i] * 1e-3) / 1e9 if passes[i] > 0 else float("nan")
+        per_pass.append(dict(kernel=["pass1_xfwd", "pass2_yfwd", "pass3_z_green", "pass4_yinv",
+                                     "pass5_xinv_finish"][i], ms=round(passes[i], 5), bytes=pb[i],
+                             gbs=round(gbs, 1)))
+    dom = max(range(5), key=lambda i: passes[i])
+    total_bytes = sum(pb) + 0  # compact layout: no IOCC read
+    roofline = dict(bound="hbm", achieved=round(pb[dom] / (passes[dom] * 1e-3) / 1e9, 1), peak=HBM_PEAK_GBS,
+                    unit="GB/s", frac=round(pb[dom] / (passes[dom] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                    traffic=None, kernel=per_pass[dom]["kernel"])
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(tg, kd, host, budget_s=args.cpu_budget)
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "matvecs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c64",
+        "data": "synthetic",
+        "config": {"workload": f"block {tg.nx}x{tg.ny}x{tg.nz} dipoles ({tg.nat0} dipoles), padded grid "
+                               f"{2*tg.nx}x{2*tg.ny}x{2*tg.nz}x3 complex64, m=1.5+0.1i, x=10 (kd={kd:.4f}), "
+                               f"{args.backend.upper()} MATVEC on compact vectors",
+                   "backend": args.backend, "nat0": tg.nat0},
+        "roofline": roofline,
+        "matvec_gbs": round(total_bytes / (ms_step * 1e-3) / 1e9, 1),
+        "matvec_bytes": total_bytes,
+        "passes": per_pass,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+    lib.release()
+
+
+if __name__ == "__main__":
+    main()

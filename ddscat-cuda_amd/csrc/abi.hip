@@ -1,0 +1,224 @@
+// Reference-compatible C ABI: prepare / apply / release / last_error and the
+// ddscat_hip_* extensions.  Argument checks, return codes and lifetime follow
+// ddscat_cuda_unified/cuda/ddscat_matvec_cuda.cu:913-916, 1005-1029.
+#include "solver_common.hpp"
+#include "../../include/ddscat_matvec_hip.h"
+#include <cmath>
+#include <vector>
+
+using namespace ddhip;
+
+#ifndef DDSCAT_HIP_DEFAULT_BACKEND
+#define DDSCAT_HIP_DEFAULT_BACKEND 0
+#endif
+
+namespace {
+struct DefaultBackend {
+  DefaultBackend() { ctx().backend = DDSCAT_HIP_DEFAULT_BACKEND; }
+} g_default_backend;
+
+bool grid_args_ok(int nx, int ny, int nz, int nat, int nat0, int ipbc) {
+  if (nx <= 0 || ny <= 0 || nz <= 0 || nat <= 0 || nat0 <= 0 || nat0 > nat) return false;
+  if (ipbc != 0 && ipbc != 1) return false;
+  if ((long long)nx * ny * nz != (long long)nat) return false;
+  if ((long long)3 * nat > 0x7fffffffLL) return false;
+  return true;
+}
+}  // namespace
+
+extern "C" DDSCAT_HIP_API int ddscat_cuda_release(void) {
+  release_context(ctx());
+  clear_error();
+  return 0;
+}
+
+extern "C" DDSCAT_HIP_API const char* ddscat_cuda_last_error(void) { return error_text(); }
+
+extern "C" DDSCAT_HIP_API int ddscat_cuda_prepare_green_f32(int nx, int ny, int nz, int ipbc, int nat, int nat0,
+                                                            int idipint, float gamma, float pyd, float pzd, float akx,
+                                                            float aky, float akz, float dx1, float dx2, float dx3,
+                                                            const int16_t* iocc) {
+  clear_error();
+  (void)pyd; (void)pzd;
+  if (!grid_args_ok(nx, ny, nz, nat, nat0, ipbc) || (idipint != 0 && idipint != 1) || !iocc) {
+    set_error("prepare_green", "invalid argument");
+    return 201;
+  }
+  if (ipbc && gamma <= 0.f) {
+    set_error("prepare_green", "GAMMA must be >0 for PBC");
+    return 202;
+  }
+  if (ipbc) {
+    set_error("prepare_green", "IPBC=1 (periodic targets) is not built in this library");
+    return 209;
+  }
+  Context& c = ctx();
+  release_context(c);
+  if (int rc = ensure_stream(c)) return rc;
+  c.nx = nx; c.ny = ny; c.nz = nz; c.ipbc = ipbc; c.nat = nat; c.nat0 = nat0;
+  c.green_count = (size_t)6 * (nx + 1) * (ny + 1) * (nz + 1);
+  if (hipMalloc((void**)&c.green, c.green_count * sizeof(float2)) != hipSuccess) {
+    set_error("prepare_green", "malloc resident Green failed");
+    release_context(c);
+    return 204;
+  }
+  const double kx = akx, ky = aky, kz = akz;
+  const double akd = std::sqrt(kx * kx + ky * ky + kz * kz);
+  const double dx[3] = {dx1, dx2, dx3};
+  int grc = green_build(c, idipint, akd, dx);
+  if (grc) { release_context(c); return 220 + grc; }
+  int rc = finish_prepare(c, iocc, 204, 208);
+  if (rc) { release_context(c); return rc; }
+  c.ready = true;
+  return 0;
+}
+
+extern "C" DDSCAT_HIP_API int ddscat_cuda_prepare_f32(int nx, int ny, int nz, int ipbc, int nat, int nat0,
+                                                      const void* green, const int16_t* iocc) {
+  clear_error();
+  if (!grid_args_ok(nx, ny, nz, nat, nat0, ipbc) || !green || !iocc) {
+    set_error("prepare", "invalid argument");
+    return 1;
+  }
+  if (ipbc) {
+    set_error("prepare", "IPBC=1 (periodic targets) is not built in this library");
+    return 2;
+  }
+  Context& c = ctx();
+  release_context(c);
+  if (ensure_stream(c)) return 2;
+  c.nx = nx; c.ny = ny; c.nz = nz; c.ipbc = ipbc; c.nat = nat; c.nat0 = nat0;
+  c.green_count = (size_t)6 * (nx + 1) * (ny + 1) * (nz + 1);
+  if (hipMalloc((void**)&c.green, c.green_count * sizeof(float2)) != hipSuccess) {
+    set_error("prepare", "malloc green failed");
+    release_context(c);
+    return 2;
+  }
+  int rc = finish_prepare(c, iocc, 2, 6);
+  if (rc) { release_context(c); return rc == 6 ? 6 : (rc >= 200 ? rc : 2); }
+  if (hipMemcpy(c.green, green, c.green_count * sizeof(float2), hipMemcpyHostToDevice) != hipSuccess) {
+    set_error("prepare", "green H2D failed");
+    release_context(c);
+    return 4;
+  }
+  c.ready = true;
+  return 0;
+}
+
+extern "C" DDSCAT_HIP_API int ddscat_cuda_apply_f32(int cwhat, const void* x, void* y, const void* adia,
+                                                    const void* aoff) {
+  Context& c = ctx();
+  if (!c.ready) { set_error("apply", "not prepared"); return 10; }
+  if ((cwhat != 'N' && cwhat != 'C') || !x || !y || !adia || !aoff) { set_error("apply", "invalid argument"); return 11; }
+  clear_error();
+  const size_t b = (size_t)3 * c.nat * sizeof(float2);
+  DDH_TRY(hipMemcpyAsync(c.xfull, x, b, hipMemcpyHostToDevice, c.stream), "x H2D", 12);
+  DDH_TRY(hipMemcpyAsync(c.adia, adia, b, hipMemcpyHostToDevice, c.stream), "adia H2D", 13);
+  DDH_TRY(hipMemcpyAsync(c.aoff, aoff, b, hipMemcpyHostToDevice, c.stream), "aoff H2D", 14);
+  int rc = (c.backend == BACKEND_SLICES) ? slices_matvec(c, cwhat, c.xfull, c.yfull, c.adia, c.aoff, 0, c.stream)
+                                         : matvec_launch(c, cwhat, c.xfull, c.yfull, c.adia, c.aoff, 0, c.stream);
+  if (rc) return rc;
+  DDH_TRY(hipMemcpyAsync(y, c.yfull, b, hipMemcpyDeviceToHost, c.stream), "y D2H", 15);
+  DDH_TRY(hipStreamSynchronize(c.stream), "y D2H", 15);
+  return 0;
+}
+
+// ------------------------------------------------------------- extensions --
+extern "C" DDSCAT_HIP_API int ddscat_hip_set_backend(int backend) {
+  if (backend != BACKEND_FFT3D && backend != BACKEND_SLICES) { set_error("set_backend", "unknown backend"); return 1; }
+  ctx().backend = backend;
+  return 0;
+}
+
+extern "C" DDSCAT_HIP_API int ddscat_hip_set_precision(int precision) {
+  if (precision != PREC_MIXED && precision != PREC_FP32) { set_error("set_precision", "unknown precision"); return 1; }
+  ctx().precision = precision;
+  return 0;
+}
+
+extern "C" DDSCAT_HIP_API int ddscat_hip_apply_device(int cwhat, const void* x, void* y, const void* adia,
+                                                      const void* aoff, int layout, void* stream) {
+  Context& c = ctx();
+  if (!c.ready) { set_error("apply_device", "not prepared"); return 10; }
+  if ((cwhat != 'N' && cwhat != 'C') || !x || !y || !adia || !aoff || (layout != 0 && layout != 1)) {
+    set_error("apply_device", "invalid argument");
+    return 11;
+  }
+  hipStream_t st = stream ? (hipStream_t)stream : c.stream;
+  return (c.backend == BACKEND_SLICES)
+             ? slices_matvec(c, cwhat, (const float2*)x, (float2*)y, (const float2*)adia, (const float2*)aoff, layout, st)
+             : matvec_launch(c, cwhat, (const float2*)x, (float2*)y, (const float2*)adia, (const float2*)aoff, layout, st);
+}
+
+extern "C" DDSCAT_HIP_API int ddscat_hip_get_green(void* out) {
+  Context& c = ctx();
+  if (!c.ready || !out) { set_error("get_green", "not prepared"); return 10; }
+  DDH_TRY(hipStreamSynchronize(c.stream), "get_green sync", 12);
+  DDH_TRY(hipMemcpy(out, c.green, c.green_count * sizeof(float2), hipMemcpyDeviceToHost), "get_green D2H", 12);
+  return 0;
+}
+
+extern "C" DDSCAT_HIP_API int ddscat_hip_info(long long* out, int n) {
+  Context& c = ctx();
+  if (!out) return 1;
+  const long long v[] = {c.nx, c.ny, c.nz, c.nat, c.nat0, (long long)c.green_count, (long long)c.matvec_count,
+                         c.backend, c.precision, c.mv.gx, c.mv.gy, c.mv.gz, c.mv.rx_fwd, c.mv.cy, c.mv.cz,
+                         c.mv.rx_inv, (long long)(c.green_build_ms * 1000.0)};
+  const int m = (int)(sizeof(v) / sizeof(v[0]));
+  for (int i = 0; i < n && i < m; ++i) out[i] = v[i];
+  return 0;
+}
+
+extern "C" DDSCAT_HIP_API void* ddscat_hip_stream(void) {
+  Context& c = ctx();
+  if (ensure_stream(c)) return nullptr;
+  return (void*)c.stream;
+}
+
+extern "C" DDSCAT_HIP_API int ddscat_hip_sync(void) {
+  Context& c = ctx();
+  if (!c.stream) return 0;
+  DDH_TRY(hipStreamSynchronize(c.stream), "sync", 1);
+  return 0;
+}
+
+// Runs `reps` MATVECs on device pointers with HIP events between the five
+// FFT3D passes; ms_out[0..4] = mean per-pass time, ms_out[5] = mean MATVEC
+// time (first to last event).  For the SLICES backend only ms_out[5] is set.
+extern "C" DDSCAT_HIP_API int ddscat_hip_time_matvec(int cwhat, const void* x, void* y, const void* adia,
+                                                     const void* aoff, int layout, int reps, double* ms_out) {
+  Context& c = ctx();
+  if (!c.ready) { set_error("time_matvec", "not prepared"); return 10; }
+  if (!x || !y || !adia || !aoff || !ms_out || reps <= 0) { set_error("time_matvec", "invalid argument"); return 11; }
+  hipEvent_t ev[6];
+  for (int i = 0; i < 6; ++i) DDH_TRY(hipEventCreate(&ev[i]), "event create", 12);
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  int rc = 0;
+  for (int r = 0; r < reps && !rc; ++r) {
+    if (c.backend == BACKEND_SLICES) {
+      hipEventRecord(ev[0], c.stream);
+      rc = slices_matvec(c, cwhat, (const float2*)x, (float2*)y, (const float2*)adia, (const float2*)aoff, layout,
+                         c.stream);
+      hipEventRecord(ev[5], c.stream);
+    } else {
+      rc = matvec_launch(c, cwhat, (const float2*)x, (float2*)y, (const float2*)adia, (const float2*)aoff, layout,
+                         c.stream, ev);
+    }
+    if (rc) break;
+    if (hipEventSynchronize(ev[5]) != hipSuccess) { set_error("time_matvec", "event sync"); rc = 12; break; }
+    float t = 0.f;
+    if (c.backend != BACKEND_SLICES) {
+      for (int i = 0; i < 5; ++i) {
+        hipEventElapsedTime(&t, ev[i], ev[i + 1]);
+        acc[i] += t;
+      }
+    }
+    hipEventElapsedTime(&t, ev[0], ev[5]);
+    acc[5] += t;
+  }
+  for (int i = 0; i < 6; ++i) {
+    ms_out[i] = acc[i] / reps;
+    hipEventDestroy(ev[i]);
+  }
+  return rc;
+}

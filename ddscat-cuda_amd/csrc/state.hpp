@@ -1,0 +1,70 @@
+// Device-resident state of one prepared target (Green tensor, FFT work
+// buffers, occupancy maps, solver workspace).  The reference C ABI is
+// stateful and single-instance (ddscat_cuda_unified/cuda/ddscat_matvec_cuda.cu:20-56);
+// the library keeps one default Context for it.
+#pragma once
+#include "common.hpp"
+
+namespace ddhip {
+
+enum Backend { BACKEND_FFT3D = 0, BACKEND_SLICES = 1 };
+enum Precision { PREC_MIXED = 0, PREC_FP32 = 1 };
+
+struct MatvecPlan {
+  int nx = 0, ny = 0, nz = 0, nat = 0, nat0 = 0;
+  int gx = 0, gy = 0, gz = 0;
+  size_t ngrid = 0;
+  FftPlan px{}, py{}, pz{};
+  float2 *twx = nullptr, *twy = nullptr, *twz = nullptr;  // device twiddle tables
+  // tile parameters
+  int rx_fwd = 1, rx_inv = 1, cy = 1, cz = 1;
+};
+
+struct Context {
+  bool ready = false;
+  int nx = 0, ny = 0, nz = 0, ipbc = 0, nat = 0, nat0 = 0;
+  int backend = BACKEND_FFT3D;
+  int precision = PREC_MIXED;
+  MatvecPlan mv;
+  hipStream_t stream = nullptr;
+  // Green tensor (trimmed octant for IPBC=0): [6][nz+1][ny+1][nx+1] complex64
+  float2* green = nullptr;
+  size_t green_count = 0;
+  // FFT3D work: a1 [3][nz][ny][gx], a2 [3][nz][gy][gx]
+  float2 *a1 = nullptr, *a2 = nullptr;
+  // occupancy
+  int16_t* iocc = nullptr;     // [nat]
+  int* occ = nullptr;          // [nat0] occupied site j, increasing
+  int* rowptr = nullptr;       // [ny*nz+1] compact range of each (iy,iz) row
+  // full-layout operator and vector staging
+  float2 *adia = nullptr, *aoff = nullptr;      // [3*nat]
+  float2 *adia_c = nullptr, *aoff_c = nullptr;  // [3*nat0]
+  float2 *xfull = nullptr, *yfull = nullptr;    // [3*nat]
+  // solver storage (compact, 3*nat0 each)
+  float2 *sx = nullptr, *sy = nullptr, *sb = nullptr, *wrk = nullptr;
+  int wrk_vectors = 0;
+  double* red = nullptr;        // reduction partials
+  double* scal = nullptr;       // device scalar block (doubles)
+  double* hscal = nullptr;      // pinned host mirror
+  bool solution_valid = false;
+  unsigned long long matvec_count = 0;
+  double green_build_ms = 0.0;
+};
+
+Context& ctx();
+void release_context(Context& c);
+int  ensure_solver_workspace(Context& c, int nvec);
+void release_solver_workspace(Context& c);
+
+// MATVEC: y = A x.  layout 0: full [3][nat] vectors (iocc mask applied);
+// layout 1: compact [3][nat0] vectors (adia_c/aoff_c).  conj_mode: 'N' or 'C'.
+// ev (optional): 6 events recorded before passes 1..5 and after pass 5.
+int matvec_launch(Context& c, int cwhat, const float2* x, float2* y, const float2* adia,
+                  const float2* aoff, int layout, hipStream_t st, hipEvent_t* ev = nullptr);
+int matvec_setup(Context& c);
+void matvec_release(Context& c);
+
+// Green tensor build on device (IPBC=0), result in c.green.
+int green_build(Context& c, int idipint, double akd, const double dx[3]);
+
+}  // namespace ddhip

@@ -1,0 +1,264 @@
+"""ctypes binding of the drop-in C ABI (include/ddscat_matvec_hip.h).
+
+Mirrors what the reference's Fortran bridge does through ISO_C_BINDING
+(ddscat_cuda_unified/fortran/cuda_matvec_bridge.f90:14-115): host arrays in,
+host arrays out, integer return codes with the library's last_error text.
+The binding refuses to run if the HIP library is not built: there is no CPU
+fallback on this path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+PKG_ROOT = Path(__file__).resolve().parent.parent
+LIB_DIR = PKG_ROOT / "lib"
+LIB_FFT3D = LIB_DIR / "libddscat_matvec_cuda.so"
+LIB_SLICES = LIB_DIR / "libddscat_matvec_cuda_slice.so"
+
+SOLVERS = ("gpbicg", "qmrccg", "pbcgst", "pbcgs2", "petrkp", "bicgstab4", "gpbicgstab2", "gpbicgstab4")
+REFERENCE_SYMBOLS = (
+    ["ddscat_cuda_prepare_green_f32", "ddscat_cuda_prepare_f32", "ddscat_cuda_apply_f32"]
+    + [f"ddscat_cuda_solve_{s}_f32" for s in SOLVERS]
+    + ["ddscat_cuda_evalq_f32", "ddscat_cuda_scat_f32", "ddscat_cuda_release", "ddscat_cuda_last_error"]
+)
+EXTENSION_SYMBOLS = [
+    "ddscat_hip_set_backend", "ddscat_hip_set_precision", "ddscat_hip_apply_device",
+    "ddscat_hip_get_green", "ddscat_hip_info", "ddscat_hip_stream", "ddscat_hip_sync",
+    "ddscat_hip_time_matvec",
+]
+INFO_FIELDS = ("nx", "ny", "nz", "nat", "nat0", "green_count", "matvec_count", "backend", "precision",
+               "gx", "gy", "gz", "rx_fwd", "cy", "cz", "rx_inv", "green_build_us")
+
+BACKEND_FFT3D, BACKEND_SLICES = 0, 1
+PREC_MIXED, PREC_FP32 = 0, 1
+
+
+class DDSCATError(RuntimeError):
+    def __init__(self, where: str, code: int, text: str):
+        super().__init__(f"{where} failed with code {code}: {text}")
+        self.code = code
+        self.text = text
+
+
+def _c64(a, n=None):
+    a = np.ascontiguousarray(a, dtype=np.complex64)
+    if n is not None and a.size != n:
+        raise ValueError(f"expected {n} complex values, got {a.size}")
+    return a
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class Library:
+    """One loaded copy of the plugin.  The C state is a process-wide singleton,
+    exactly as in the reference (one prepared target at a time)."""
+
+    def __init__(self, path: str | os.PathLike | None = None, slices: bool = False):
+        if path is None:
+            path = os.environ.get("DDSCAT_CUDA_DLL") or (LIB_SLICES if slices else LIB_FFT3D)
+        path = Path(path)
+        if not path.exists():
+            raise FileNotFoundError(
+                f"HIP library {path} is not built; run __graft_entry__.build() (no CPU fallback exists)")
+        self.path = path
+        self.lib = C.CDLL(str(path), mode=C.RTLD_LOCAL)
+        L = self.lib
+        i, f, vp, cp = C.c_int, C.c_float, C.c_void_p, C.c_char_p
+        L.ddscat_cuda_prepare_green_f32.argtypes = [i] * 7 + [f] * 9 + [vp]
+        L.ddscat_cuda_prepare_green_f32.restype = i
+        L.ddscat_cuda_prepare_f32.argtypes = [i] * 6 + [vp, vp]
+        L.ddscat_cuda_prepare_f32.restype = i
+        L.ddscat_cuda_apply_f32.argtypes = [i, vp, vp, vp, vp]
+        L.ddscat_cuda_apply_f32.restype = i
+        for s in SOLVERS:
+            fn = getattr(L, f"ddscat_cuda_solve_{s}_f32")
+            fn.argtypes = [vp, vp, vp, vp, i, i, f, vp, vp, vp]
+            fn.restype = i
+        L.ddscat_cuda_evalq_f32.argtypes = [i, vp, vp, vp, vp, f, f, f, f, vp, vp, vp]
+        L.ddscat_cuda_evalq_f32.restype = i
+        L.ddscat_cuda_scat_f32.argtypes = [i, i, vp, vp, vp, vp, vp, f, f, vp, vp, vp, vp, vp, vp, vp, vp, vp]
+        L.ddscat_cuda_scat_f32.restype = i
+        L.ddscat_cuda_release.argtypes = []
+        L.ddscat_cuda_release.restype = i
+        L.ddscat_cuda_last_error.argtypes = []
+        L.ddscat_cuda_last_error.restype = cp
+        L.ddscat_hip_set_backend.argtypes = [i]
+        L.ddscat_hip_set_backend.restype = i
+        L.ddscat_hip_set_precision.argtypes = [i]
+        L.ddscat_hip_set_precision.restype = i
+        L.ddscat_hip_apply_device.argtypes = [i, vp, vp, vp, vp, i, vp]
+        L.ddscat_hip_apply_device.restype = i
+        L.ddscat_hip_get_green.argtypes = [vp]
+        L.ddscat_hip_get_green.restype = i
+        L.ddscat_hip_info.argtypes = [vp, i]
+        L.ddscat_hip_info.restype = i
+        L.ddscat_hip_stream.argtypes = []
+        L.ddscat_hip_stream.restype = vp
+        L.ddscat_hip_sync.argtypes = []
+        L.ddscat_hip_sync.restype = i
+        L.ddscat_hip_time_matvec.argtypes = [i, vp, vp, vp, vp, i, i, vp]
+        L.ddscat_hip_time_matvec.restype = i
+        self.nat = self.nat0 = 0
+
+    # ---------------------------------------------------------------- errors
+    def last_error(self) -> str:
+        t = self.lib.ddscat_cuda_last_error()
+        return t.decode() if t else ""
+
+    def _check(self, where, rc):
+        if rc != 0:
+            raise DDSCATError(where, rc, self.last_error())
+        return rc
+
+    # --------------------------------------------------------------- prepare
+    def prepare_green(self, nx, ny, nz, iocc, ak, dx=(1.0, 1.0, 1.0), ipbc=0, idipint=0, gamma=0.0,
+                      pyd=0.0, pzd=0.0, nat=None, nat0=None, check=True):
+        iocc = np.ascontiguousarray(iocc, dtype=np.int16)
+        nat = nx * ny * nz if nat is None else nat
+        nat0 = int(np.count_nonzero(iocc)) if nat0 is None else nat0
+        rc = self.lib.ddscat_cuda_prepare_green_f32(nx, ny, nz, ipbc, nat, nat0, idipint, gamma, pyd, pzd,
+                                                    float(ak[0]), float(ak[1]), float(ak[2]),
+                                                    float(dx[0]), float(dx[1]), float(dx[2]), _p(iocc))
+        if rc == 0:
+            self.nat, self.nat0 = nat, nat0
+        return self._check("prepare_green", rc) if check else rc
+
+    def prepare(self, nx, ny, nz, green, iocc, ipbc=0, nat=None, nat0=None, check=True):
+        iocc = np.ascontiguousarray(iocc, dtype=np.int16)
+        nat = nx * ny * nz if nat is None else nat
+        nat0 = int(np.count_nonzero(iocc)) if nat0 is None else nat0
+        g = _c64(green)
+        rc = self.lib.ddscat_cuda_prepare_f32(nx, ny, nz, ipbc, nat, nat0, _p(g), _p(iocc))
+        if rc == 0:
+            self.nat, self.nat0 = nat, nat0
+        return self._check("prepare", rc) if check else rc
+
+    def release(self):
+        return self.lib.ddscat_cuda_release()
+
+    # ----------------------------------------------------------------- apply
+    def apply(self, cwhat, x, adia, aoff):
+        n = 3 * self.nat
+        x, adia, aoff = _c64(x, n), _c64(adia, n), _c64(aoff, n)
+        y = np.empty(n, dtype=np.complex64)
+        cw = ord(cwhat) if isinstance(cwhat, str) else int(cwhat)
+        self._check("apply", self.lib.ddscat_cuda_apply_f32(cw, _p(x), _p(y), _p(adia), _p(aoff)))
+        return y
+
+    # ---------------------------------------------------------------- solve
+    def solve(self, name, b, adia, aoff, x0=None, maxit=300, tol=1e-5, check=True):
+        """Returns (x, itno, tole, ncompte, rc)."""
+        n = 3 * self.nat
+        b, adia, aoff = _c64(b, n), _c64(adia, n), _c64(aoff, n)
+        x = np.zeros(n, dtype=np.complex64) if x0 is None else _c64(x0, n).copy()
+        it, nc = C.c_int(0), C.c_int(0)
+        te = C.c_float(0.0)
+        fn = getattr(self.lib, f"ddscat_cuda_solve_{name}_f32")
+        rc = fn(_p(b), _p(x), _p(adia), _p(aoff), n, int(maxit), float(tol), C.byref(it), C.byref(te), C.byref(nc))
+        if check:
+            self._check(f"solve_{name}", rc)
+        return x, it.value, te.value, nc.value, rc
+
+    # --------------------------------------------------------------- evalq
+    def evalq(self, ak, e02=1.0, use_resident=True, e_c=None, p_c=None, adia_c=None, aoff_c=None, check=True):
+        out = (C.c_double * 3)()
+        ptrs = [None] * 4
+        keep = []
+        if not use_resident:
+            n = 3 * self.nat0
+            arrs = [_c64(a, n) for a in (e_c, p_c, adia_c, aoff_c)]
+            keep = arrs
+            ptrs = [_p(a) for a in arrs]
+        rc = self.lib.ddscat_cuda_evalq_f32(1 if use_resident else 0, *ptrs, float(ak[0]), float(ak[1]),
+                                            float(ak[2]), float(e02), C.byref(out, 0), C.byref(out, 8),
+                                            C.byref(out, 16))
+        del keep
+        if check:
+            self._check("evalq", rc)
+        return out[0], out[1], out[2], rc
+
+    # ---------------------------------------------------------------- scat
+    def scat(self, ak, dx, x0, e01, etasca, e02=1.0, aks=None, em1=None, em2=None, jpbc=0, check=True):
+        ak = np.ascontiguousarray(ak, dtype=np.float32)
+        dx = np.ascontiguousarray(dx, dtype=np.float32)
+        x0 = np.ascontiguousarray(x0, dtype=np.float32)
+        e01 = _c64(e01, 3)
+        ndir = 0 if aks is None else len(aks)
+        if ndir:
+            aks = np.ascontiguousarray(aks, dtype=np.float32).reshape(ndir, 3)
+            em1 = np.ascontiguousarray(em1, dtype=np.float32).reshape(ndir, 3)
+            em2 = np.ascontiguousarray(em2, dtype=np.float32).reshape(ndir, 3)
+            f1 = np.zeros(ndir, np.complex64)
+            f2 = np.zeros(ndir, np.complex64)
+        else:
+            aks = em1 = em2 = f1 = f2 = np.zeros(3, np.float32)
+        cb, cs, cg2 = C.c_double(), C.c_double(), C.c_double()
+        cg = (C.c_double * 3)()
+        navg = C.c_int(0)
+        rc = self.lib.ddscat_cuda_scat_f32(jpbc, ndir, _p(ak), _p(aks), _p(dx), _p(em1), _p(em2), float(e02),
+                                           float(etasca), _p(e01), _p(x0), C.byref(cb), C.byref(cs), cg,
+                                           C.byref(cg2), C.byref(navg), _p(f1), _p(f2))
+        if check:
+            self._check("scat", rc)
+        res = dict(cbksca=cb.value, csca=cs.value, cscag=np.array(cg[:]), cscag2=cg2.value, navg=navg.value, rc=rc)
+        if ndir:
+            res["f1"], res["f2"] = f1, f2
+        return res
+
+    # ----------------------------------------------------------- extensions
+    def set_backend(self, backend):
+        return self._check("set_backend", self.lib.ddscat_hip_set_backend(int(backend)))
+
+    def set_precision(self, prec):
+        return self._check("set_precision", self.lib.ddscat_hip_set_precision(int(prec)))
+
+    def apply_device(self, cwhat, x_ptr, y_ptr, adia_ptr, aoff_ptr, layout=0, stream=None):
+        cw = ord(cwhat) if isinstance(cwhat, str) else int(cwhat)
+        return self._check("apply_device", self.lib.ddscat_hip_apply_device(
+            cw, C.c_void_p(x_ptr), C.c_void_p(y_ptr), C.c_void_p(adia_ptr), C.c_void_p(aoff_ptr), int(layout),
+            C.c_void_p(stream) if stream else None))
+
+    def info(self) -> dict:
+        buf = (C.c_longlong * len(INFO_FIELDS))()
+        self.lib.ddscat_hip_info(buf, len(INFO_FIELDS))
+        return dict(zip(INFO_FIELDS, buf[:]))
+
+    def get_green(self):
+        n = self.info()["green_count"]
+        g = np.empty(n, dtype=np.complex64)
+        self._check("get_green", self.lib.ddscat_hip_get_green(_p(g)))
+        return g
+
+    def time_matvec(self, cwhat, x_ptr, y_ptr, adia_ptr, aoff_ptr, layout=1, reps=10):
+        """Mean per-pass and per-MATVEC milliseconds (HIP events on the library stream)."""
+        out = (C.c_double * 6)()
+        cw = ord(cwhat) if isinstance(cwhat, str) else int(cwhat)
+        self._check("time_matvec", self.lib.ddscat_hip_time_matvec(
+            cw, C.c_void_p(x_ptr), C.c_void_p(y_ptr), C.c_void_p(adia_ptr), C.c_void_p(aoff_ptr), int(layout),
+            int(reps), out))
+        return list(out)
+
+    def stream(self):
+        return self.lib.ddscat_hip_stream()
+
+    def sync(self):
+        return self._check("sync", self.lib.ddscat_hip_sync())
+
+
+def exported_symbols(path) -> set[str]:
+    """Names the shared object exports (dlsym probe; no GPU call)."""
+    lib = C.CDLL(str(path), mode=C.RTLD_LOCAL)
+    found = set()
+    for name in REFERENCE_SYMBOLS + EXTENSION_SYMBOLS:
+        try:
+            getattr(lib, name)
+            found.add(name)
+        except AttributeError:
+            pass
+    return found

@@ -1,0 +1,142 @@
+/*
+ * C ABI of the MI355X (gfx950) DDSCAT hot-path library.
+ *
+ * The first block is a drop-in for the reference plugin's export set
+ * (ddscat_cuda_unified/cuda/ddscat_matvec_cuda.h:11-51): same names, argument
+ * order, meaning, return codes and lifetime, so the unchanged dlopen loader
+ * (ddscat_cuda_unified/cuda/ddscat_cuda_loader.c:41-102) and Fortran bridge
+ * (ddscat_cuda_unified/fortran/cuda_matvec_bridge.f90) bind it as they bind
+ * libddscat_matvec_cuda.so.  All pointers in that block are caller-owned host
+ * memory; complex values are interleaved complex64; full vectors are
+ * component-major [3][NAT] with site j = ix + NX*(iy + NY*iz).
+ *
+ * The second block (ddscat_hip_*) holds extensions the reference loader never
+ * resolves: device-pointer MATVEC, backend / precision selection, inspection.
+ */
+#ifndef DDSCAT_MATVEC_HIP_H
+#define DDSCAT_MATVEC_HIP_H
+#include <stdint.h>
+
+#define DDSCAT_HIP_API __attribute__((visibility("default")))
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- reference export set ------------------------------------------------ */
+
+/* Build the spectral Green tensor on the device (FP64) and allocate the
+ * MATVEC / solver state.  Replaces ddscat_matvec_cuda.h:11-13
+ * (implementation ddscat_matvec_cuda.cu:1005-1017).  Codes: 201 invalid
+ * argument, 202 GAMMA<=0 with PBC, 203 device setup, 204 allocation,
+ * 206 unsupported grid, 208 IOCC count != NAT0, 209 IPBC=1 not built,
+ * 220+g Green build failure. */
+DDSCAT_HIP_API int ddscat_cuda_prepare_green_f32(int nx, int ny, int nz, int ipbc, int nat, int nat0,
+                                                 int idipint, float gamma, float pyd, float pzd,
+                                                 float akx, float aky, float akz, float dx1,
+                                                 float dx2, float dx3, const int16_t *iocc);
+
+/* Same lifetime with a host-supplied transformed Green tensor CXZC
+ * [6][NZ+1][NY+1][NX+1] complex64 (IPBC=0).  Replaces ddscat_matvec_cuda.h:14-15
+ * (ddscat_matvec_cuda.cu:1018-1026).  Codes 1-6 as the reference. */
+DDSCAT_HIP_API int ddscat_cuda_prepare_f32(int nx, int ny, int nz, int ipbc, int nat, int nat0,
+                                           const void *green, const int16_t *iocc);
+
+/* y = A x ('N', cwhat=78) or conj(A conj(x)) ('C', cwhat=67) on full host
+ * vectors.  Replaces ddscat_matvec_cuda.h:16-17 (ddscat_matvec_cuda.cu:1027-1029).
+ * Codes: 10 not prepared, 11 bad argument, 12-15 copies, 20-29 MATVEC. */
+DDSCAT_HIP_API int ddscat_cuda_apply_f32(int cwhat, const void *x, void *y, const void *adia,
+                                         const void *aoff);
+
+/* Krylov solvers, ddscat_matvec_cuda.h:18-41.  b, x full [3*NAT] host vectors
+ * (x: initial guess in, solution out), ndim must be 3*NAT.  itno = iterations,
+ * tole = achieved relative residual, ncompte = MATVEC count.  Non-convergence
+ * returns 0 with the achieved residual except PBCGST (87), as the reference. */
+DDSCAT_HIP_API int ddscat_cuda_solve_gpbicg_f32(const void *b, void *x, const void *adia,
+                                                const void *aoff, int ndim, int maxit, float tol,
+                                                int *itno, float *tole, int *ncompte);
+DDSCAT_HIP_API int ddscat_cuda_solve_qmrccg_f32(const void *b, void *x, const void *adia,
+                                                const void *aoff, int ndim, int maxit, float tol,
+                                                int *itno, float *tole, int *ncompte);
+DDSCAT_HIP_API int ddscat_cuda_solve_pbcgst_f32(const void *b, void *x, const void *adia,
+                                                const void *aoff, int ndim, int maxit, float tol,
+                                                int *itno, float *tole, int *ncompte);
+DDSCAT_HIP_API int ddscat_cuda_solve_pbcgs2_f32(const void *b, void *x, const void *adia,
+                                                const void *aoff, int ndim, int maxit, float tol,
+                                                int *itno, float *tole, int *ncompte);
+DDSCAT_HIP_API int ddscat_cuda_solve_petrkp_f32(const void *b, void *x, const void *adia,
+                                                const void *aoff, int ndim, int maxit, float tol,
+                                                int *itno, float *tole, int *ncompte);
+DDSCAT_HIP_API int ddscat_cuda_solve_bicgstab4_f32(const void *b, void *x, const void *adia,
+                                                   const void *aoff, int ndim, int maxit, float tol,
+                                                   int *itno, float *tole, int *ncompte);
+DDSCAT_HIP_API int ddscat_cuda_solve_gpbicgstab2_f32(const void *b, void *x, const void *adia,
+                                                     const void *aoff, int ndim, int maxit,
+                                                     float tol, int *itno, float *tole,
+                                                     int *ncompte);
+DDSCAT_HIP_API int ddscat_cuda_solve_gpbicgstab4_f32(const void *b, void *x, const void *adia,
+                                                     const void *aoff, int ndim, int maxit,
+                                                     float tol, int *itno, float *tole,
+                                                     int *ncompte);
+
+/* Cabs, Cext, Cpha from the resident solution (use_resident=1) or compact
+ * host arrays [3][NAT0].  Replaces ddscat_matvec_cuda.h:42-44
+ * (ddscat_matvec_cuda.cu:919-955).  Codes 300-307. */
+DDSCAT_HIP_API int ddscat_cuda_evalq_f32(int use_resident, const void *e_compact,
+                                         const void *p_compact, const void *adia_compact,
+                                         const void *aoff_compact, float akx, float aky, float akz,
+                                         float e02, double *cabs, double *cext, double *cpha);
+
+/* Far-field sums over the resident solution.  Replaces ddscat_matvec_cuda.h:45-49
+ * (ddscat_matvec_cuda.cu:957-1002).  Codes 320-333. */
+DDSCAT_HIP_API int ddscat_cuda_scat_f32(int jpbc, int ndir, const float *ak, const float *aks,
+                                        const float *dx, const float *em1, const float *em2,
+                                        float e02, float etasca, const void *e01, const float *x0,
+                                        double *cbksca, double *csca, double *cscag,
+                                        double *cscag2, int *navg, void *f1, void *f2);
+
+/* Free all device state.  Replaces ddscat_matvec_cuda.h:50 (ddscat_matvec_cuda.cu:915). */
+DDSCAT_HIP_API int ddscat_cuda_release(void);
+/* Text of the last error.  Replaces ddscat_matvec_cuda.h:51 (ddscat_matvec_cuda.cu:916). */
+DDSCAT_HIP_API const char *ddscat_cuda_last_error(void);
+
+/* ---- extensions (no counterpart in the reference) ------------------------- */
+
+enum { DDSCAT_HIP_BACKEND_FFT3D = 0, DDSCAT_HIP_BACKEND_SLICES = 1 };
+enum { DDSCAT_HIP_PREC_MIXED = 0, DDSCAT_HIP_PREC_FP32 = 1 };
+
+/* MATVEC backend used by the next prepare call (default: FFT3D, or SLICES
+ * for libddscat_matvec_cuda_slice.so). */
+DDSCAT_HIP_API int ddscat_hip_set_backend(int backend);
+/* Solver reduction precision: MIXED = FP64 dots/scalars (reference policy),
+ * FP32 = plain single precision throughout. */
+DDSCAT_HIP_API int ddscat_hip_set_precision(int precision);
+
+/* y = A x on DEVICE pointers, stream-ordered on `stream` (NULL = library
+ * stream), no synchronisation.  layout 0: full [3][NAT] vectors and
+ * operators; layout 1: compact [3][NAT0] vectors and compact operators. */
+DDSCAT_HIP_API int ddscat_hip_apply_device(int cwhat, const void *x, void *y, const void *adia,
+                                           const void *aoff, int layout, void *stream);
+
+/* Copy the resident spectral Green tensor (ddscat_hip_info field 5 complex64
+ * values) to host memory. */
+DDSCAT_HIP_API int ddscat_hip_get_green(void *out);
+
+/* Fill up to n int64 fields: nx ny nz nat nat0 green_count matvec_count
+ * backend precision gx gy gz rx_fwd cy cz rx_inv green_build_us. */
+DDSCAT_HIP_API int ddscat_hip_info(long long *out, int n);
+
+/* Run `reps` MATVECs on device pointers with HIP events between the five
+ * FFT3D passes: ms_out[0..4] mean per-pass milliseconds, ms_out[5] mean
+ * MATVEC milliseconds (SLICES backend: only ms_out[5]). */
+DDSCAT_HIP_API int ddscat_hip_time_matvec(int cwhat, const void *x, void *y, const void *adia,
+                                          const void *aoff, int layout, int reps, double *ms_out);
+
+/* The library's HIP stream (hipStream_t) and a blocking wait on it. */
+DDSCAT_HIP_API void *ddscat_hip_stream(void);
+DDSCAT_HIP_API int ddscat_hip_sync(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
