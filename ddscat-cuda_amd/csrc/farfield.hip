@@ -1,0 +1,312 @@
+// EVALQ and SCAT on the device (ddscat_cuda_unified/cuda/ddscat_matvec_cuda.cu:763-1002).
+//
+// EVALQ: q = (alpha^-1 + i (2/3) k^3) P + offdiag P on the occupied sites,
+// Cabs = -4 pi k Im<P,q>/E0^2, Cext = 4 pi k Im<E,P>/E0^2,
+// Cpha = 2 pi k Re<E,P>/E0^2 with FP64 dot products.
+//
+// SCAT: S(k_s) = sum_j P_j exp(-i k_s . r_j), r_j = ((i+1) + x0) dx per axis.
+// The lattice makes the phase separable, exp(-i k.r) = Ex(ix) Ey(iy) Ez(iz),
+// so per direction and lattice row (iy,iz) the kernel forms
+// T = sum_ix Ex(ix) P(ix,iy,iz) over the row's occupied sites (a contiguous
+// compact range) and adds Ey Ez T: three complex FP64 multiply-adds per site
+// and direction, no transcendental per site.  Per-axis phase tables are built
+// in FP64 on the device; row-chunk partials are reduced in a fixed order.
+// Angular quadrature (directions, weights) is restated from cu:849-875 on
+// the host.
+#include "solvers.hpp"
+#include "../../include/ddscat_matvec_hip.h"
+#include <vector>
+#include <cmath>
+#include <complex>
+
+namespace ddhip {
+
+__global__ void k_evalq_q(const float2* __restrict__ p, float2* __restrict__ q, const float2* __restrict__ ad,
+                          const float2* __restrict__ ao, int nat0, float rabs) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nat0) return;
+  const float2 p1 = p[j], p2 = p[j + nat0], p3 = p[j + 2 * nat0];
+  const float2 d1 = ad[j], d2 = ad[j + nat0], d3 = ad[j + 2 * nat0];
+  const float2 a23 = ao[j], a31 = ao[j + nat0], a12 = ao[j + 2 * nat0];
+  const float2 ri = make_float2(0.f, rabs);
+  q[j] = cadd(cadd(cmul(cadd(d1, ri), p1), cmul(a31, p3)), cmul(a12, p2));
+  q[j + nat0] = cadd(cadd(cmul(cadd(d2, ri), p2), cmul(a12, p1)), cmul(a23, p3));
+  q[j + 2 * nat0] = cadd(cadd(cmul(cadd(d3, ri), p3), cmul(a23, p2)), cmul(a31, p1));
+}
+
+int evalq(Context& c, int use_resident, const void* e_c, const void* p_c, const void* ad_c, const void* ao_c,
+          float akx, float aky, float akz, float e02, double* cabs, double* cext, double* cpha) {
+  clear_error();
+  if (!c.ready || !cabs || !cext || !cpha || !(e02 > 0.f)) { set_error("GPU EVALQ", "invalid argument"); return 300; }
+  const size_t cb = (size_t)3 * c.nat0 * sizeof(float2);
+  hipStream_t st = c.stream;
+  if (use_resident) {
+    if (!c.solution_valid) { set_error("GPU EVALQ", "no resident final polarization published by solver"); return 301; }
+  } else {
+    if (!e_c || !p_c || !ad_c || !ao_c) { set_error("GPU EVALQ", "missing compact host arrays"); return 302; }
+    DDH_TRY(hipMemcpyAsync(c.sb, e_c, cb, hipMemcpyHostToDevice, st), "GPU EVALQ E H2D", 303);
+    DDH_TRY(hipMemcpyAsync(c.sx, p_c, cb, hipMemcpyHostToDevice, st), "GPU EVALQ P H2D", 303);
+    DDH_TRY(hipMemcpyAsync(c.adia_c, ad_c, cb, hipMemcpyHostToDevice, st), "GPU EVALQ ADIA H2D", 305);
+    DDH_TRY(hipMemcpyAsync(c.aoff_c, ao_c, cb, hipMemcpyHostToDevice, st), "GPU EVALQ AOFF H2D", 305);
+    c.solution_valid = true;
+  }
+  const double ak2 = (double)akx * akx + (double)aky * aky + (double)akz * akz, akk = std::sqrt(ak2),
+               ak3 = akk * ak2;
+  hipLaunchKernelGGL(k_evalq_q, dim3((c.nat0 + 255) / 256), dim3(256), 0, st, c.sx, c.sy, c.adia_c, c.aoff_c,
+                     c.nat0, (float)(ak3 / 1.5));
+  DDH_LAUNCH_OK("GPU EVALQ local alpha^-1 P", 306);
+  DotSet d;
+  d.add(c.sx, c.sy, DOT_C).add(c.sb, c.sx, DOT_C);
+  // EVALQ sums are always FP64, whatever the solver precision mode
+  const int prec = c.precision;
+  c.precision = PREC_MIXED;
+  const int drc = dots(c, d, (size_t)3 * c.nat0, SL_D, st);
+  c.precision = prec;
+  if (drc || fetch_scalars(c, SL_D, 4, st)) return 307;
+  const double pi = 3.14159265358979323846;
+  const double zabs_im = c.hscal[SL_D + 1], zep_re = c.hscal[SL_D + 2], zep_im = c.hscal[SL_D + 3];
+  *cabs = -4.0 * pi * akk * zabs_im / (double)e02;
+  *cext = 4.0 * pi * akk * zep_im / (double)e02;
+  *cpha = 2.0 * pi * akk * zep_re / (double)e02;
+  return 0;
+}
+
+// ------------------------------------------------------------------ SCAT --
+struct ScatDir {
+  float k[3];
+  double w[5];
+};
+
+static long long round_half_away(double v) { return v >= 0 ? (long long)std::floor(v + 0.5) : -(long long)std::floor(-v + 0.5); }
+
+static int scat_integration_dirs(const float ak[3], const float2 e01[3], float etasca, int nat0,
+                                 std::vector<ScatDir>& dirs) {
+  const double pi = 3.14159265358979323846;
+  const double ak2 = (double)ak[0] * ak[0] + (double)ak[1] * ak[1] + (double)ak[2] * ak[2], akk = std::sqrt(ak2);
+  double a1[3] = {(double)e01[0].x, (double)e01[1].x, (double)e01[2].x};
+  const double einc = std::sqrt(a1[0] * a1[0] + a1[1] * a1[1] + a1[2] * a1[2]);
+  if (!(einc > 0.0)) { set_error("GPU SCAT", "CXE01_TF is pure imaginary"); return 1; }
+  for (double& v : a1) v *= akk / einc;
+  const double a2[3] = {(ak[1] * a1[2] - ak[2] * a1[1]) / akk, (ak[2] * a1[0] - ak[0] * a1[2]) / akk,
+                        (ak[0] * a1[1] - ak[1] * a1[0]) / akk};
+  const double x = akk * std::cbrt(3.0 * (double)nat0 / (4.0 * pi)), theta0 = 2.0 * pi / (1.0 + x), afac = 1.0;
+  const int ntheta = 1 + (int)round_half_away((2.0 * (3.0 + x) / (double)etasca) * (1.0 + afac / (pi + theta0)) /
+                                              (1.0 + afac * theta0 / ((pi + theta0) * (pi + theta0))));
+  double theta = 0.0, thetau = 0.0;
+  dirs.clear();
+  for (int ic = 1; ic <= ntheta; ++ic) {
+    const double thetal = theta;
+    theta = thetau;
+    if (ic < ntheta) {
+      const double si = (double)ic * (pi + afac * pi / (theta0 + pi)) / (double)(ntheta - 1), term = si - afac - theta0;
+      thetau = 0.5 * (term + std::sqrt(term * term + 4.0 * theta0 * si));
+    } else {
+      thetau = pi;
+    }
+    const double domega = 2.0 * pi * (std::cos(0.5 * (thetal + theta)) - std::cos(0.5 * (theta + thetau)));
+    const double ct = std::cos(theta), stt = std::sin(theta);
+    int nphi = (ic == 1 || ic == ntheta) ? 1 : (int)round_half_away(4.0 * pi * stt / (thetau - thetal));
+    if (nphi < 3 && ic != 1 && ic != ntheta) nphi = 3;
+    const double w = domega / (double)nphi, dphi = 2.0 * pi / (double)nphi;
+    for (int ip = 1; ip <= nphi; ++ip) {
+      const double phi = dphi * ((double)ip - 0.5 * (double)(ic % 2)), sp = std::sin(phi), cp = std::cos(phi);
+      ScatDir d{};
+      for (int k = 0; k < 3; ++k) d.k[k] = (float)(ct * (double)ak[k] + stt * (cp * a1[k] + sp * a2[k]));
+      d.w[0] = w; d.w[1] = w * ct; d.w[2] = w * stt * cp; d.w[3] = w * stt * sp; d.w[4] = w * ct * ct;
+      dirs.push_back(d);
+    }
+  }
+  return 0;
+}
+
+// per-axis phase tables tab[(axis offset + i) * ndir + d] = exp(-i k_a ((i+1)+x0_a) dx_a), FP64
+// (direction fastest, so the lanes of a workgroup read consecutive entries)
+__global__ void k_phase_tables(const float* __restrict__ kd, int ndir, int nx, int ny, int nz, double dx1, double dx2,
+                               double dx3, double x01, double x02, double x03, double2* __restrict__ tab) {
+  const int L = nx + ny + nz;
+  const size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= (size_t)ndir * L) return;
+  const int d = (int)(id / L), i = (int)(id - (size_t)d * L);
+  double r, k;
+  if (i < nx) { r = ((double)(i + 1) + x01) * dx1; k = kd[3 * d]; }
+  else if (i < nx + ny) { r = ((double)(i - nx + 1) + x02) * dx2; k = kd[3 * d + 1]; }
+  else { r = ((double)(i - nx - ny + 1) + x03) * dx3; k = kd[3 * d + 2]; }
+  double s, co;
+  sincos(k * r, &s, &co);
+  tab[(size_t)i * ndir + d] = make_double2(co, -s);
+}
+
+constexpr int FF_DIRS = 64;    // directions per workgroup (one per lane)
+constexpr int FF_ROWS = 32;    // lattice rows per workgroup
+
+// partial[chunk][d][3] = sum over the chunk's rows of Ey Ez sum_row Ex P
+__global__ __launch_bounds__(FF_DIRS) void k_farfield_rows(
+    const float2* __restrict__ p, const int* __restrict__ occ, const int* __restrict__ rowptr,
+    const double2* __restrict__ tab, int ndir, int nx, int ny, int nz, int nat0, double2* __restrict__ part) {
+  const int chunk = blockIdx.x;
+  const int d = blockIdx.y * FF_DIRS + threadIdx.x;
+  const int nrows = ny * nz;
+  const int g0 = chunk * FF_ROWS, g1 = min(nrows, g0 + FF_ROWS);
+  double2 s0 = make_double2(0, 0), s1 = s0, s2 = s0;
+  if (d < ndir) {
+    const double2* t = tab + d;
+    const size_t sd = (size_t)ndir;
+    for (int g = g0; g < g1; ++g) {
+      const int q0 = rowptr[g], q1 = rowptr[g + 1];
+      if (q0 == q1) continue;
+      double2 r0 = make_double2(0, 0), r1 = r0, r2 = r0;
+      const int base = g * nx;
+      for (int q = q0; q < q1; ++q) {
+        const double2 e = t[(size_t)(occ[q] - base) * sd];
+        const float2 a = p[q], b = p[q + nat0], cc = p[q + 2 * nat0];
+        r0.x += e.x * a.x - e.y * a.y; r0.y += e.x * a.y + e.y * a.x;
+        r1.x += e.x * b.x - e.y * b.y; r1.y += e.x * b.y + e.y * b.x;
+        r2.x += e.x * cc.x - e.y * cc.y; r2.y += e.x * cc.y + e.y * cc.x;
+      }
+      const int iy = g % ny, iz = g / ny;
+      const double2 eyz = zmul(t[(size_t)(nx + iy) * sd], t[(size_t)(nx + ny + iz) * sd]);
+      s0 = zadd(s0, zmul(eyz, r0));
+      s1 = zadd(s1, zmul(eyz, r1));
+      s2 = zadd(s2, zmul(eyz, r2));
+    }
+    double2* o = part + ((size_t)chunk * ndir + d) * 3;
+    o[0] = s0; o[1] = s1; o[2] = s2;
+  }
+}
+
+__global__ void k_farfield_reduce(const double2* __restrict__ part, int nchunks, int ndir, double2* __restrict__ S) {
+  const size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= (size_t)ndir * 3) return;
+  double2 acc = make_double2(0, 0);
+  for (int ch = 0; ch < nchunks; ++ch) acc = zadd(acc, part[(size_t)ch * ndir * 3 + id]);
+  S[id] = acc;
+}
+
+// S[d][3] for all directions (host result)
+static int farfield_sums(Context& c, const std::vector<float>& kdirs, const float dx[3], const float x0[3],
+                         std::vector<std::complex<double>>& S) {
+  const int ndir = (int)(kdirs.size() / 3);
+  S.assign((size_t)ndir * 3, 0.0);
+  if (ndir == 0) return 0;
+  hipStream_t st = c.stream;
+  const int L = c.nx + c.ny + c.nz;
+  const int nrows = c.ny * c.nz;
+  const int nchunks = (nrows + FF_ROWS - 1) / FF_ROWS;
+  float* dk = nullptr;
+  double2 *tab = nullptr, *part = nullptr, *dS = nullptr;
+  int rc = 0;
+  auto cleanup = [&]() { hipFree(dk); hipFree(tab); hipFree(part); hipFree(dS); };
+  // bound the partial buffer: process directions in slabs
+  const int slab = std::max(FF_DIRS, std::min(ndir, (int)std::max<size_t>(FF_DIRS, (size_t)(256u << 20) / ((size_t)nchunks * 48))));
+  if (hipMalloc((void**)&dk, kdirs.size() * sizeof(float)) != hipSuccess ||
+      hipMalloc((void**)&tab, (size_t)slab * L * sizeof(double2)) != hipSuccess ||
+      hipMalloc((void**)&part, (size_t)nchunks * slab * 3 * sizeof(double2)) != hipSuccess ||
+      hipMalloc((void**)&dS, (size_t)ndir * 3 * sizeof(double2)) != hipSuccess) {
+    set_error("GPU SCAT", "far-field workspace allocation failed");
+    cleanup();
+    return 1;
+  }
+  if (hipMemcpyAsync(dk, kdirs.data(), kdirs.size() * sizeof(float), hipMemcpyHostToDevice, st) != hipSuccess) {
+    set_error("GPU SCAT", "directions H2D failed");
+    cleanup();
+    return 1;
+  }
+  for (int d0 = 0; d0 < ndir && !rc; d0 += slab) {
+    const int nd = std::min(slab, ndir - d0);
+    const size_t nt = (size_t)nd * L;
+    hipLaunchKernelGGL(k_phase_tables, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, st, dk + 3 * d0, nd, c.nx,
+                       c.ny, c.nz, (double)dx[0], (double)dx[1], (double)dx[2], (double)x0[0], (double)x0[1],
+                       (double)x0[2], tab);
+    hipLaunchKernelGGL(k_farfield_rows, dim3(nchunks, (nd + FF_DIRS - 1) / FF_DIRS), dim3(FF_DIRS), 0, st, c.sx,
+                       c.occ, c.rowptr, tab, nd, c.nx, c.ny, c.nz, c.nat0, part);
+    hipLaunchKernelGGL(k_farfield_reduce, dim3((unsigned)((nd * 3 + 255) / 256)), dim3(256), 0, st, part, nchunks,
+                       nd, dS + (size_t)3 * d0);
+    if (hipGetLastError() != hipSuccess) { set_error("GPU SCAT", "far-field kernel launch failed"); rc = 1; }
+  }
+  if (!rc && (hipMemcpyAsync(S.data(), dS, (size_t)ndir * 3 * sizeof(double2), hipMemcpyDeviceToHost, st) != hipSuccess ||
+              hipStreamSynchronize(st) != hipSuccess)) {
+    set_error("GPU SCAT", "far-field sums D2H failed");
+    rc = 1;
+  }
+  cleanup();
+  return rc;
+}
+
+int scat(Context& c, int jpbc, int ndir, const float* ak, const float* aks, const float* dx, const float* em1,
+         const float* em2, float e02, float etasca, const void* e01v, const float* x0, double* cbksca, double* csca,
+         double* cscag, double* cscag2, int* navg, void* f1v, void* f2v) {
+  clear_error();
+  if (!c.ready || !c.solution_valid || !ak || !dx || !x0 || !cbksca || !csca || !cscag || !cscag2 || !navg ||
+      !(e02 > 0.f)) {
+    set_error("GPU SCAT", "invalid state/argument");
+    return 320;
+  }
+  if (ndir > 0 && (!aks || !em1 || !em2 || !f1v || !f2v)) { set_error("GPU SCAT", "missing selected-direction arrays"); return 321; }
+  const double ak2 = (double)ak[0] * ak[0] + (double)ak[1] * ak[1] + (double)ak[2] * ak[2], akk = std::sqrt(ak2),
+               ak3 = akk * ak2;
+  *cbksca = 0.0; *csca = 0.0; cscag[0] = cscag[1] = cscag[2] = 0.0; *cscag2 = 0.0; *navg = 0;
+  if (jpbc == 0) {
+    if (!e01v || !(etasca > 0.f)) { set_error("GPU SCAT", "finite-target integration needs CXE01 and ETASCA>0"); return 323; }
+    std::vector<ScatDir> dirs;
+    if (scat_integration_dirs(ak, (const float2*)e01v, etasca, c.nat0, dirs)) return 324;
+    *navg = (int)dirs.size();
+    std::vector<float> kd(3 * dirs.size());
+    for (size_t i = 0; i < dirs.size(); ++i)
+      for (int k = 0; k < 3; ++k) kd[3 * i + k] = dirs[i].k[k];
+    std::vector<std::complex<double>> S;
+    if (farfield_sums(c, kd, dx, x0, S)) return 325;
+    double acc[5] = {0, 0, 0, 0, 0}, back = 0.0;
+    for (size_t i = 0; i < dirs.size(); ++i) {
+      const double nxv = (double)kd[3 * i] / akk, nyv = (double)kd[3 * i + 1] / akk, nzv = (double)kd[3 * i + 2] / akk;
+      const std::complex<double> sx = S[3 * i], sy = S[3 * i + 1], sz = S[3 * i + 2];
+      const std::complex<double> nd = nxv * sx + nyv * sy + nzv * sz;
+      const double es2 = std::norm(sx - nxv * nd) + std::norm(sy - nyv * nd) + std::norm(sz - nzv * nd);
+      for (int k = 0; k < 5; ++k) acc[k] += dirs[i].w[k] * es2;
+      if (i + 1 == dirs.size()) back = es2;
+    }
+    const double fac = ak2 * ak2 / (double)e02;
+    *csca = fac * acc[0];
+    cscag[0] = fac * acc[1]; cscag[1] = fac * acc[2]; cscag[2] = fac * acc[3];
+    *cscag2 = fac * acc[4];
+    *cbksca = fac * back;
+  }
+  if (ndir > 0) {
+    std::vector<float> kd(aks, aks + 3 * (size_t)ndir);
+    std::vector<std::complex<double>> S;
+    if (farfield_sums(c, kd, dx, x0, S)) return 330;
+    const double term = ak3 / std::sqrt((double)e02);
+    float2* f1 = (float2*)f1v;
+    float2* f2 = (float2*)f2v;
+    for (int i = 0; i < ndir; ++i) {
+      std::complex<double> a1 = 0.0, a2 = 0.0;
+      for (int k = 0; k < 3; ++k) {
+        a1 += (double)em1[3 * i + k] * S[3 * i + k];
+        a2 += (double)em2[3 * i + k] * S[3 * i + k];
+      }
+      f1[i] = make_float2((float)(term * a1.real()), (float)(term * a1.imag()));
+      f2[i] = make_float2((float)(term * a2.real()), (float)(term * a2.imag()));
+    }
+  }
+  return 0;
+}
+
+}  // namespace ddhip
+
+using namespace ddhip;
+
+extern "C" DDSCAT_HIP_API int ddscat_cuda_evalq_f32(int use_resident, const void* e_compact, const void* p_compact,
+                                                    const void* adia_compact, const void* aoff_compact, float akx,
+                                                    float aky, float akz, float e02, double* cabs, double* cext,
+                                                    double* cpha) {
+  return evalq(ctx(), use_resident, e_compact, p_compact, adia_compact, aoff_compact, akx, aky, akz, e02, cabs, cext,
+               cpha);
+}
+
+extern "C" DDSCAT_HIP_API int ddscat_cuda_scat_f32(int jpbc, int ndir, const float* ak, const float* aks,
+                                                   const float* dx, const float* em1, const float* em2, float e02,
+                                                   float etasca, const void* e01, const float* x0, double* cbksca,
+                                                   double* csca, double* cscag, double* cscag2, int* navg, void* f1,
+                                                   void* f2) {
+  return scat(ctx(), jpbc, ndir, ak, aks, dx, em1, em2, e02, etasca, e01, x0, cbksca, csca, cscag, cscag2, navg, f1,
+              f2);
+}

@@ -6,6 +6,7 @@ namespace ddhip {
 struct DotLaunch {
   const float2* a[DOT_MAX];
   const float2* b[DOT_MAX];
+  const float2* c[DOT_MAX];
   int kind[DOT_MAX];
   int nt;
 };
@@ -28,8 +29,18 @@ __global__ __launch_bounds__(RED_THREADS) void k_dots(DotLaunch L, size_t n, dou
     for (int t = 0; t < DOT_MAX; ++t) {
       if (t < L.nt) {
         const float2 a = L.a[t][i];
-        if (L.kind[t] == 2) {
+        const int kd = L.kind[t];
+        if (kd == DOT_N2) {
           re[t] += (Acc)a.x * (Acc)a.x + (Acc)a.y * (Acc)a.y;
+        } else if (kd >= DOT_D2) {
+          const float2 b = L.b[t][i];
+          Acc dx = (Acc)a.x - (Acc)b.x, dy = (Acc)a.y - (Acc)b.y;
+          if (kd == DOT_G2) {
+            const float2 cc = L.c[t][i];
+            dx -= (Acc)cc.x;
+            dy -= (Acc)cc.y;
+          }
+          re[t] += dx * dx + dy * dy;
         } else {
           const float2 b = L.b[t][i];
           const Acc ay = L.kind[t] == 0 ? -(Acc)a.y : (Acc)a.y;
@@ -81,9 +92,10 @@ int dots(Context& c, const DotSet& s, size_t n, int dst, hipStream_t st) {
   for (int t = 0; t < s.n; ++t) {
     L.a[t] = s.t[t].a;
     L.b[t] = s.t[t].b ? s.t[t].b : s.t[t].a;
+    L.c[t] = s.t[t].c ? s.t[t].c : L.b[t];
     L.kind[t] = s.t[t].kind;
   }
-  for (int t = s.n; t < DOT_MAX; ++t) { L.a[t] = L.a[0]; L.b[t] = L.b[0]; L.kind[t] = 2; }
+  for (int t = s.n; t < DOT_MAX; ++t) { L.a[t] = L.a[0]; L.b[t] = L.b[0]; L.c[t] = L.c[0]; L.kind[t] = DOT_N2; }
   size_t want = (n + (size_t)RED_THREADS * 8 - 1) / ((size_t)RED_THREADS * 8);
   int nb = (int)std::min<size_t>(RED_MAX_BLOCKS, std::max<size_t>(1, want));
   if (c.precision == PREC_FP32) {

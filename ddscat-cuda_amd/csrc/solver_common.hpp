@@ -17,16 +17,22 @@ constexpr int RED_MAX_TERMS = 32;   // real-valued sums per launch
 constexpr int DOT_MAX = 12;         // complex dots per launch
 constexpr int SCAL_COUNT = 512;     // doubles in the device scalar block
 
-// kind: 0 = sum conj(a) b, 1 = sum a b (unconjugated), 2 = sum |a|^2 (real)
+// kind: 0 = sum conj(a) b, 1 = sum a b (unconjugated), 2 = sum |a|^2,
+//       3 = sum |a - b|^2, 4 = sum |a - b - c|^2  (kinds 2-4 real)
+enum { DOT_C = 0, DOT_U = 1, DOT_N2 = 2, DOT_D2 = 3, DOT_G2 = 4 };
 struct DotTerm {
   const float2* a;
   const float2* b;
+  const float2* c;
   int kind;
 };
 struct DotSet {
   DotTerm t[DOT_MAX];
   int n = 0;
-  void add(const float2* a, const float2* b, int kind) { t[n++] = DotTerm{a, b, kind}; }
+  DotSet& add(const float2* a, const float2* b, int kind, const float2* c = nullptr) {
+    t[n++] = DotTerm{a, b, c, kind};
+    return *this;
+  }
 };
 
 // Computes the dots of `s` over n complex elements; result i is written as
@@ -46,8 +52,14 @@ struct LinTerm {
 struct LinComb {
   LinTerm t[LC_MAX];
   int n = 0;
-  void lit(const float2* v, double re, double im = 0.0) { t[n++] = LinTerm{v, -1, re, im, 0}; }
-  void dev(const float2* v, int sidx, int neg = 0) { t[n++] = LinTerm{v, sidx, 0.0, 0.0, neg}; }
+  LinComb& lit(const float2* v, double re, double im = 0.0) {
+    t[n++] = LinTerm{v, -1, re, im, 0};
+    return *this;
+  }
+  LinComb& dev(const float2* v, int sidx, int neg = 0) {
+    t[n++] = LinTerm{v, sidx, 0.0, 0.0, neg};
+    return *this;
+  }
 };
 int lincomb(Context& c, float2* d, const LinComb& lc, size_t n, hipStream_t st);
 

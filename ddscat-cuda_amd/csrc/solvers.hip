@@ -1,0 +1,862 @@
+// Device-resident Krylov solvers behind the reference solver entry points.
+//
+// Vectors are complex64 on the occupied sites only (compact [3][NAT0],
+// ddscat_cuda_unified/cuda/ddscat_matvec_cuda.cu:120-134); dot products are
+// FP64 reductions (reduce.hip); every recurrence scalar is FP64 and lives in
+// the device scalar block, updated by single-lane kernels, so an iteration
+// issues no host round trip except one small read-back of the residual and
+// status word.  Algorithms, breakdown codes and the reliable-residual policy
+// (every 20 iterations and on recursive convergence: restart if the gap
+// between true and recursive residual is >= 1e-3 or convergence was false,
+// cu:221-222, cu:543-556) follow the reference:
+//   PBCGS2  cu:459-480, cu:1168-1278   (ZBCG2 / BiCGStab(2))
+//   QMRCCG  cu:434-443, cu:1071-1103   (complex-symmetric QMR)
+//   PETRKP  cu:482-493, cu:1280-1357   (CG on the normal equations)
+//   GPBICG  cu:420-432, cu:1031-1069
+//   PBCGST  cu:445-457, cu:1105-1165   (PIM BiCGStab, left diagonal preconditioner)
+#include "solver_common.hpp"
+#include "solvers.hpp"
+#include "../../include/ddscat_matvec_hip.h"
+#include <cmath>
+#include <cstdio>
+
+namespace ddhip {
+
+// ------------------------------------------------------------ scalar block --
+__device__ __forceinline__ double2 ld2(const double* s, int i) { return make_double2(s[i], s[i + 1]); }
+__device__ __forceinline__ void st2(double* s, int i, double2 v) { s[i] = v.x; s[i + 1] = v.y; }
+__device__ __forceinline__ void fail(double* s, int code) {
+  if (s[SL_STATUS] == 0.0) s[SL_STATUS] = (double)code;
+}
+
+int solver_begin(Context& c, const SolverArgs& a, int code_arg, int code_io, const char* name, float2* xs) {
+  if (!c.ready || !a.b || !a.x || !a.adia || !a.aoff || !a.itno || !a.tole || !a.ncompte ||
+      a.ndim != 3 * c.nat || a.maxit <= 0 || !(a.tol > 0.f)) {
+    set_error(name, "invalid argument");
+    return code_arg;
+  }
+  if (int rc = ensure_solver_workspace(c, 13)) return rc;
+  c.solution_valid = false;
+  const size_t fb = (size_t)3 * c.nat * sizeof(float2);
+  hipStream_t st = c.stream;
+  DDH_TRY(hipMemcpyAsync(c.xfull, a.b, fb, hipMemcpyHostToDevice, st), name, code_io);
+  if (gather_compact(c, c.xfull, c.sb, st)) return code_io;
+  DDH_TRY(hipMemcpyAsync(c.adia, a.adia, fb, hipMemcpyHostToDevice, st), name, code_io);
+  DDH_TRY(hipMemcpyAsync(c.aoff, a.aoff, fb, hipMemcpyHostToDevice, st), name, code_io);
+  if (gather_compact(c, c.adia, c.adia_c, st) || gather_compact(c, c.aoff, c.aoff_c, st)) return code_io;
+  DDH_TRY(hipMemcpyAsync(c.xfull, a.x, fb, hipMemcpyHostToDevice, st), name, code_io);
+  if (gather_compact(c, c.xfull, xs, st)) return code_io;
+  DDH_TRY(hipMemsetAsync(c.scal, 0, SCAL_COUNT * sizeof(double), st), name, code_io);
+  return 0;
+}
+
+int solver_end(Context& c, const SolverArgs& a, const float2* xs, int it, double resid, int nc, int code) {
+  hipStream_t st = c.stream;
+  const size_t cb = (size_t)3 * c.nat0 * sizeof(float2);
+  if (xs != c.sx) DDH_TRY(hipMemcpyAsync(c.sx, xs, cb, hipMemcpyDeviceToDevice, st), "publish solution", code);
+  c.solution_valid = true;
+  if (scatter_full(c, c.sx, c.xfull, st)) return code;
+  DDH_TRY(hipMemcpyAsync(a.x, c.xfull, (size_t)3 * c.nat * sizeof(float2), hipMemcpyDeviceToHost, st),
+          "solution D2H", code);
+  DDH_TRY(hipStreamSynchronize(st), "solution D2H", code);
+  *a.itno = it;
+  *a.tole = (float)resid;
+  *a.ncompte = nc;
+  return 0;
+}
+
+// true residual ||rhs - A x|| and gap ||rhs - A x - rrec|| in FP64 (cu:543-556);
+// leaves A x in c.sy
+int reliable_check(Context& c, const float2* rhs, const float2* xcur, const float2* rrec, double rhsnorm,
+                   double rec_res, double tol, int& nc, Reliable& rr) {
+  const size_t n = (size_t)3 * c.nat0;
+  if (int rc = run_mv(c, 'N', xcur, c.sy, nc)) return rc;
+  DotSet d;
+  d.add(rhs, c.sy, DOT_D2).add(rhs, c.sy, DOT_G2, rrec);
+  if (dots(c, d, n, SL_D, c.stream)) return 1;
+  if (fetch_scalars(c, SL_D, 4, c.stream)) return 1;
+  const double tr = std::sqrt(std::max(0.0, c.hscal[SL_D]));
+  const double gapn = std::sqrt(std::max(0.0, c.hscal[SL_D + 2]));
+  rr.true_res = tr / rhsnorm;
+  rr.gap = tr > 0.0 ? gapn / tr : (gapn == 0.0 ? 0.0 : INFINITY);
+  rr.converged = rr.true_res <= tol;
+  rr.restart = rr.gap >= RELIABLE_GAP || (rec_res <= tol && !rr.converged);
+  return 0;
+}
+
+static double host_norm(Context& c, const float2* v) {
+  DotSet d;
+  d.add(v, v, DOT_N2);
+  if (dots(c, d, (size_t)3 * c.nat0, SL_D, c.stream) || fetch_scalars(c, SL_D, 2, c.stream)) return -1.0;
+  return std::sqrt(std::max(0.0, c.hscal[SL_D]));
+}
+
+// =============================================================== PBCGS2 ====
+namespace p2 {
+enum { ALPHA = SL_X, BETA = SL_X + 2, OMEGA = SL_X + 4, RHO0 = SL_X + 6, RHO1 = SL_X + 8, SIGMA = SL_X + 10,
+       Y1 = SL_X + 12, Y2 = SL_X + 14 };
+enum { RNRM0 = SL_R, RNRM = SL_R + 1, MXNRMX = SL_R + 2, MXNRMR = SL_R + 3 };
+
+__global__ void k_init(double* s) {
+  st2(s, ALPHA, make_double2(0, 0)); st2(s, OMEGA, make_double2(1, 0));
+  st2(s, SIGMA, make_double2(1, 0)); st2(s, RHO0, make_double2(1, 0));
+}
+__global__ void k_init_reals(double* s) {
+  const double r0 = sqrt(fmax(s[SL_D], 0.0));
+  s[RNRM0] = r0; s[RNRM] = r0; s[MXNRMX] = r0; s[MXNRMR] = r0;
+}
+__global__ void k_outer(double* s) { st2(s, RHO0, zscale(zmul(ld2(s, OMEGA), ld2(s, RHO0)), -1.0)); }
+// rho1 from dot slot 0; optional |r0|^2 in dot slot 1 updates the running maxima
+__global__ void k_beta(double* s, int has_norm) {
+  if (has_norm) {
+    const double r = sqrt(fmax(s[SL_D + 2], 0.0));
+    s[RNRM] = r; s[MXNRMX] = fmax(s[MXNRMX], r); s[MXNRMR] = fmax(s[MXNRMR], r);
+  }
+  const double2 rho1 = ld2(s, SL_D), rho0 = ld2(s, RHO0);
+  st2(s, RHO1, rho1);
+  if (zzero(rho0)) { fail(s, 95); return; }
+  st2(s, BETA, zmul(ld2(s, ALPHA), zdiv(rho1, rho0)));
+  st2(s, RHO0, rho1);
+}
+__global__ void k_alpha(double* s) {
+  const double2 sg = ld2(s, SL_D);
+  st2(s, SIGMA, sg);
+  if (zzero(sg)) { fail(s, 98); return; }
+  st2(s, ALPHA, zdiv(ld2(s, RHO1), sg));
+}
+__device__ double2 inner3(const double2* a, const double2* b) {
+  double2 z = make_double2(0, 0);
+  for (int i = 0; i < 3; ++i) z = zadd(z, zmul(zconj(a[i]), b[i]));
+  return z;
+}
+// Gram dots in slots: 0 Z11=<r0,r0>, 1 Z21=<r1,r0>, 2 Z22=<r1,r1>, 3 Z31=<r2,r0>,
+// 4 Z32=<r2,r1>, 5 Z33=<r2,r2>; minimal-residual polynomial with the convex
+// correction (cu:472), then the reliable-update flags (cu:474).
+__global__ void k_convex(double* s) {
+  double2 Z[3][3];
+  Z[0][0] = ld2(s, SL_D + 0); Z[1][0] = ld2(s, SL_D + 2); Z[1][1] = ld2(s, SL_D + 4);
+  Z[2][0] = ld2(s, SL_D + 6); Z[2][1] = ld2(s, SL_D + 8); Z[2][2] = ld2(s, SL_D + 10);
+  Z[0][1] = zconj(Z[1][0]); Z[0][2] = zconj(Z[2][0]); Z[1][2] = zconj(Z[2][1]);
+  // running maxima with |r0| after the second half-step
+  {
+    const double r = sqrt(fmax(Z[0][0].x, 0.0));
+    s[RNRM] = r; s[MXNRMX] = fmax(s[MXNRMX], r); s[MXNRMR] = fmax(s[MXNRMR], r);
+  }
+  if (zzero(Z[1][1])) { fail(s, 102); return; }
+  double2 y0[3] = {make_double2(-1, 0), zdiv(Z[1][0], Z[1][1]), make_double2(0, 0)};
+  double2 yl[3] = {make_double2(0, 0), zdiv(Z[1][2], Z[1][1]), make_double2(-1, 0)};
+  double2 zy0[3], zyl[3];
+  for (int i = 0; i < 3; ++i) {
+    zy0[i] = zyl[i] = make_double2(0, 0);
+    for (int j = 0; j < 3; ++j) {
+      zy0[i] = zadd(zy0[i], zmul(Z[i][j], y0[j]));
+      zyl[i] = zadd(zyl[i], zmul(Z[i][j], yl[j]));
+    }
+  }
+  const double2 q0 = inner3(y0, zy0), ql = inner3(yl, zyl);
+  const double k0 = sqrt(sqrt(zabs2(q0))), kl = sqrt(sqrt(zabs2(ql)));
+  if (k0 == 0.0 || kl == 0.0) { fail(s, 102); return; }
+  const double2 vr = zscale(inner3(yl, zy0), 1.0 / (k0 * kl));
+  const double avr = sqrt(zabs2(vr));
+  if (avr == 0.0) { fail(s, 102); return; }
+  const double2 hg = zscale(vr, fmax(avr, 0.7) / avr);
+  const double2 fac = zscale(hg, k0 / kl);
+  for (int i = 0; i < 3; ++i) y0[i] = zsub(y0[i], zmul(fac, yl[i]));
+  st2(s, Y1, y0[1]);
+  st2(s, Y2, y0[2]);
+  st2(s, OMEGA, y0[2]);
+  for (int i = 0; i < 3; ++i) {
+    zy0[i] = make_double2(0, 0);
+    for (int j = 0; j < 3; ++j) zy0[i] = zadd(zy0[i], zmul(Z[i][j], y0[j]));
+  }
+  const double2 qr = inner3(y0, zy0);
+  const double rn = sqrt(sqrt(zabs2(qr)));
+  s[RNRM] = rn;
+  s[MXNRMX] = fmax(s[MXNRMX], rn);
+  s[MXNRMR] = fmax(s[MXNRMR], rn);
+  const double delta = 1.e-2, r0 = s[RNRM0];
+  const bool xpdt = (rn < delta * r0 && r0 < s[MXNRMX]);
+  const bool rcmp = ((rn < delta * s[MXNRMR] && r0 < s[MXNRMR]) || xpdt);
+  s[SL_FLAGS] = (double)((rcmp ? 1 : 0) | (xpdt ? 2 : 0));
+  s[SL_RESID] = r0 > 0.0 ? rn / r0 : 0.0;
+}
+__global__ void k_reliable_reals(double* s, int xpdt) {
+  s[MXNRMR] = s[RNRM];
+  if (xpdt) s[MXNRMX] = s[RNRM];
+}
+__global__ void k_restart_reals(double* s, double rnorm) {
+  s[RNRM] = rnorm; s[MXNRMX] = rnorm; s[MXNRMR] = rnorm;
+  st2(s, ALPHA, make_double2(0, 0)); st2(s, OMEGA, make_double2(1, 0));
+  st2(s, SIGMA, make_double2(1, 0)); st2(s, RHO0, make_double2(1, 0));
+}
+}  // namespace p2
+
+int solve_pbcgs2(Context& c, const SolverArgs& a) {
+  const char* nm = "PBCGS2";
+  clear_error();
+  if (!c.ready) { set_error(nm, "invalid argument"); return 90; }
+  if (int rc = ensure_solver_workspace(c, 13)) return rc;
+  const size_t n = (size_t)3 * c.nat0;
+  hipStream_t st = c.stream;
+  // work layout [rr, r0, r1, r2, u0, u1, u2, xp, bp, x] (cu:1173-1174)
+  float2* W = c.wrk;
+  float2 *rr = W, *r0 = W + n, *r1 = W + 2 * n, *r2 = W + 3 * n, *u0 = W + 4 * n, *u1 = W + 5 * n,
+         *u2 = W + 6 * n, *xp = W + 7 * n, *bp = W + 8 * n, *x = W + 9 * n;
+  if (int rc = solver_begin(c, a, 90, 91, nm, xp)) return rc;
+  int nc = 0;
+  // Nonzero initial guess: solve for the correction, r = b - A xp (ZBCG2 with
+  // NONZERO_X); the reference's DDSCAT caller always passes x = 0, where this
+  // reduces to r = b without a MATVEC.
+  const double x0n = host_norm(c, xp);
+  if (x0n < 0) return 92;
+  {
+    LinComb l;
+    if (x0n > 0.0) {
+      if (int rc = run_mv(c, 'N', xp, c.sy, nc)) return rc;
+      l.lit(c.sb, 1.0).lit(c.sy, -1.0);
+    } else {
+      l.lit(c.sb, 1.0);
+    }
+    if (lincomb(c, r0, l, n, st)) return 92;
+    LinComb cp; cp.lit(r0, 1.0);
+    if (lincomb(c, rr, cp, n, st) || lincomb(c, bp, cp, n, st)) return 92;
+    LinComb z; z.lit(r0, 0.0);
+    for (float2* v : {r1, r2, u0, u1, u2, x})
+      if (lincomb(c, v, z, n, st)) return 92;
+  }
+  DotSet dn; dn.add(r0, r0, DOT_N2);
+  if (dots(c, dn, n, SL_D, st)) return 92;
+  hipLaunchKernelGGL(p2::k_init_reals, dim3(1), dim3(1), 0, st, c.scal);
+  hipLaunchKernelGGL(p2::k_init, dim3(1), dim3(1), 0, st, c.scal);
+  if (fetch_scalars(c, 0, SL_R + 8, st)) return 92;
+  const double rnrm0 = c.hscal[p2::RNRM0];
+  double resid = rnrm0 > 0.0 ? 1.0 : 0.0;
+  int it = 0;
+  float2* rk[3] = {r0, r1, r2};
+  float2* uk[3] = {u0, u1, u2};
+  while (resid > a.tol && it < a.maxit) {
+    ++it;
+    hipLaunchKernelGGL(p2::k_outer, dim3(1), dim3(1), 0, st, c.scal);
+    for (int k = 1; k <= 2; ++k) {
+      DotSet d;
+      d.add(rr, rk[k - 1], DOT_C);
+      if (k == 2) d.add(r0, r0, DOT_N2);
+      if (dots(c, d, n, SL_D, st)) return 94;
+      hipLaunchKernelGGL(p2::k_beta, dim3(1), dim3(1), 0, st, c.scal, k == 2 ? 1 : 0);
+      for (int j = 0; j < k; ++j) {
+        LinComb l; l.lit(rk[j], 1.0).dev(uk[j], p2::BETA, 1);
+        if (lincomb(c, uk[j], l, n, st)) return 96;
+      }
+      if (int rc = run_mv(c, 'N', uk[k - 1], uk[k], nc)) return rc;
+      DotSet ds; ds.add(rr, uk[k], DOT_C);
+      if (dots(c, ds, n, SL_D, st)) return 97;
+      hipLaunchKernelGGL(p2::k_alpha, dim3(1), dim3(1), 0, st, c.scal);
+      { LinComb l; l.lit(x, 1.0).dev(u0, p2::ALPHA); if (lincomb(c, x, l, n, st)) return 99; }
+      for (int j = 0; j < k; ++j) {
+        LinComb l; l.lit(rk[j], 1.0).dev(uk[j + 1], p2::ALPHA, 1);
+        if (lincomb(c, rk[j], l, n, st)) return 99;
+      }
+      if (int rc = run_mv(c, 'N', rk[k - 1], rk[k], nc)) return rc;
+    }
+    DotSet g;
+    g.add(r0, r0, DOT_C).add(r1, r0, DOT_C).add(r1, r1, DOT_C).add(r2, r0, DOT_C).add(r2, r1, DOT_C).add(r2, r2, DOT_C);
+    if (dots(c, g, n, SL_D, st)) return 101;
+    hipLaunchKernelGGL(p2::k_convex, dim3(1), dim3(1), 0, st, c.scal);
+    {
+      LinComb lu; lu.lit(u0, 1.0).dev(u1, p2::Y1, 1).dev(u2, p2::Y2, 1);
+      LinComb lx; lx.lit(x, 1.0).dev(r0, p2::Y1).dev(r1, p2::Y2);
+      LinComb lr; lr.lit(r0, 1.0).dev(r1, p2::Y1, 1).dev(r2, p2::Y2, 1);
+      if (lincomb(c, u0, lu, n, st) || lincomb(c, x, lx, n, st) || lincomb(c, r0, lr, n, st)) return 103;
+    }
+    if (fetch_scalars(c, 0, 4, st)) return 103;
+    const int status = (int)c.hscal[SL_STATUS];
+    if (status) {
+      char msg[96];
+      std::snprintf(msg, sizeof(msg), "ZBCG2 breakdown (code %d)", status);
+      set_error(nm, msg);
+      return status;
+    }
+    resid = c.hscal[SL_RESID];
+    const int flags = (int)c.hscal[SL_FLAGS];
+    const bool rcmp = flags & 1, xpdt = (flags & 2) != 0;
+    if ((it % RELIABLE_PERIOD) == 0 || resid <= a.tol) {
+      Reliable chk;
+      if (int rc = reliable_check(c, bp, x, r0, rnrm0, resid, a.tol, nc, chk)) return rc > 1 ? rc : 104;
+      if (chk.converged) { resid = chk.true_res; break; }
+      if (chk.restart) {
+        LinComb lr; lr.lit(bp, 1.0).lit(c.sy, -1.0);
+        LinComb lx; lx.lit(x, 1.0).lit(xp, 1.0);
+        LinComb cp; cp.lit(r0, 1.0);
+        LinComb z; z.lit(r0, 0.0);
+        if (lincomb(c, r0, lr, n, st) || lincomb(c, xp, lx, n, st) || lincomb(c, x, z, n, st) ||
+            lincomb(c, bp, cp, n, st) || lincomb(c, rr, cp, n, st))
+          return 104;
+        for (float2* v : {r1, r2, u0, u1, u2})
+          if (lincomb(c, v, z, n, st)) return 104;
+        hipLaunchKernelGGL(p2::k_restart_reals, dim3(1), dim3(1), 0, st, c.scal, chk.true_res * rnrm0);
+        resid = chk.true_res;
+        continue;
+      }
+    }
+    if (rcmp) {
+      if (int rc = run_mv(c, 'N', x, c.sy, nc)) return rc;
+      LinComb lr; lr.lit(bp, 1.0).lit(c.sy, -1.0);
+      if (lincomb(c, r0, lr, n, st)) return 104;
+      if (xpdt) {
+        LinComb lx; lx.lit(x, 1.0).lit(xp, 1.0);
+        LinComb z; z.lit(r0, 0.0);
+        LinComb cp; cp.lit(r0, 1.0);
+        if (lincomb(c, xp, lx, n, st) || lincomb(c, x, z, n, st) || lincomb(c, bp, cp, n, st)) return 104;
+      }
+      hipLaunchKernelGGL(p2::k_reliable_reals, dim3(1), dim3(1), 0, st, c.scal, xpdt ? 1 : 0);
+    }
+  }
+  // ZBCG2 returns X + XP and always computes one final true residual (cu:1258-1264)
+  { LinComb lx; lx.lit(x, 1.0).lit(xp, 1.0); if (lincomb(c, x, lx, n, st)) return 105; }
+  if (int rc = run_mv(c, 'N', x, c.sy, nc)) return rc;
+  DotSet d; d.add(c.sb, c.sy, DOT_D2);
+  if (dots(c, d, n, SL_D, st) || fetch_scalars(c, SL_D, 2, st)) return 106;
+  // relative to |r_initial| (= |b| for the zero initial guess the driver passes)
+  resid = rnrm0 > 0.0 ? std::sqrt(std::max(0.0, c.hscal[SL_D])) / rnrm0 : 0.0;
+  return solver_end(c, a, x, it, resid, nc, 107);
+}
+
+// =============================================================== QMRCCG ====
+namespace qmr {
+enum { LAMBDA = SL_X, KAPPA = SL_X + 2, THETA = SL_X + 4, GAMMA = SL_X + 6, KSI = SL_X + 8, RHO = SL_X + 10,
+       EPS = SL_X + 12, MU = SL_X + 14, TAU = SL_X + 16, GINV = SL_X + 18, KINV = SL_X + 20, GMU = SL_X + 22,
+       TG = SL_X + 24, TK = SL_X + 26 };
+// dot slots: 0 |v~|^2, 1 |w~|^2, 2 <v~,w~>_u, 3 <v~,A w~>_u
+__device__ void derived(double* s) {
+  const double2 g = ld2(s, GAMMA), k = ld2(s, KSI), mu = ld2(s, MU), tau = ld2(s, TAU);
+  st2(s, GINV, zdiv(make_double2(1, 0), g));
+  st2(s, KINV, zdiv(make_double2(1, 0), k));
+  st2(s, GMU, zmul(g, mu));
+  st2(s, TG, zdiv(tau, g));
+  st2(s, TK, zdiv(tau, k));
+}
+__global__ void k_init_sc(double* s) {
+  st2(s, LAMBDA, make_double2(1, 0)); st2(s, KAPPA, make_double2(-1, 0)); st2(s, THETA, make_double2(-1, 0));
+  st2(s, GAMMA, make_double2(sqrt(fmax(s[SL_D], 0.0)), 0));
+  st2(s, KSI, make_double2(sqrt(fmax(s[SL_D + 2], 0.0)), 0));
+  st2(s, RHO, ld2(s, SL_D + 4));
+  st2(s, EPS, ld2(s, SL_D + 6));
+  st2(s, MU, make_double2(0, 0));
+  if (zzero(ld2(s, RHO))) { fail(s, 5); return; }
+  if (zzero(ld2(s, GAMMA))) { fail(s, 6); return; }
+  if (zzero(ld2(s, KSI))) { fail(s, 7); return; }
+  st2(s, TAU, zdiv(ld2(s, EPS), ld2(s, RHO)));
+  derived(s);
+}
+__global__ void k_recur(double* s) {
+  const double2 g0 = ld2(s, GAMMA), k0 = ld2(s, KSI), rho0 = ld2(s, RHO), tau0 = ld2(s, TAU), kap0 = ld2(s, KAPPA);
+  const double2 g = make_double2(sqrt(fmax(s[SL_D], 0.0)), 0), k = make_double2(sqrt(fmax(s[SL_D + 2], 0.0)), 0);
+  const double2 rho = ld2(s, SL_D + 4), eps = ld2(s, SL_D + 6);
+  st2(s, GAMMA, g); st2(s, KSI, k); st2(s, RHO, rho); st2(s, EPS, eps);
+  if (zzero(g)) { fail(s, 6); return; }
+  if (zzero(k)) { fail(s, 7); return; }
+  const double2 den = zmul(zmul(g, tau0), rho0);
+  if (zzero(den)) { fail(s, 12); return; }
+  const double2 mu = zdiv(zmul(zmul(g0, k0), rho), den);
+  st2(s, MU, mu);
+  if (zzero(rho)) { fail(s, 13); return; }
+  const double2 tau = zsub(zdiv(eps, rho), zmul(g, mu));
+  st2(s, TAU, tau);
+  const double at0 = zabs2(tau0), ag = zabs2(g);
+  const double2 lam = ld2(s, LAMBDA);
+  const double2 den2 = zadd(zscale(lam, at0), make_double2(ag, 0));
+  if (zzero(den2)) { fail(s, 14); return; }
+  st2(s, THETA, zdiv(zscale(zsub(make_double2(1, 0), lam), at0), den2));
+  st2(s, KAPPA, zdiv(zscale(zmul(zmul(g0, zconj(tau0)), kap0), -1.0), den2));
+  st2(s, LAMBDA, zdiv(zscale(lam, at0), den2));
+  derived(s);
+}
+}  // namespace qmr
+
+int solve_qmrccg(Context& c, const SolverArgs& a) {
+  const char* nm = "QMRCCG";
+  clear_error();
+  if (!c.ready) { set_error(nm, "invalid argument"); return 50; }
+  if (int rc = ensure_solver_workspace(c, 13)) return rc;
+  const size_t n = (size_t)3 * c.nat0;
+  hipStream_t st = c.stream;
+  float2* W = c.wrk;
+  float2 *r = W, *p = W + n, *ap = W + 2 * n, *q = W + 3 * n, *d = W + 4 * n, *sv = W + 5 * n, *vt = W + 6 * n,
+         *wt = W + 7 * n, *atw = W + 8 * n, *xs = W + 9 * n;
+  if (int rc = solver_begin(c, a, 50, 51, nm, xs)) return rc;
+  const double bnorm = host_norm(c, c.sb);
+  if (bnorm < 0) return 52;
+  if (bnorm == 0.0) { set_error(nm, "zero RHS norm"); return 52; }
+  int nc = 0;
+  LinComb z; z.lit(r, 0.0);
+  auto restart_from = [&](const float2* rsrc) -> int {
+    // r = rsrc; p = Ap = q = d = s = A w~ = 0; v~ = w~ = r
+    LinComb cp; cp.lit(rsrc, 1.0);
+    if (lincomb(c, r, cp, n, st) || lincomb(c, vt, cp, n, st) || lincomb(c, wt, cp, n, st)) return 53;
+    for (float2* v : {p, ap, q, d, sv, atw})
+      if (lincomb(c, v, z, n, st)) return 53;
+    return 0;
+  };
+  auto init_scalars = [&]() -> int {
+    if (int rc = run_mv(c, 'N', wt, atw, nc)) return rc;
+    DotSet ds;
+    ds.add(vt, vt, DOT_N2).add(wt, wt, DOT_N2).add(vt, wt, DOT_U).add(vt, atw, DOT_U);
+    if (dots(c, ds, n, SL_D, st)) return 54;
+    hipLaunchKernelGGL(qmr::k_init_sc, dim3(1), dim3(1), 0, st, c.scal);
+    if (fetch_scalars(c, 0, 2, st)) return 55;
+    if (c.hscal[SL_STATUS] != 0.0) {
+      char msg[80];
+      std::snprintf(msg, sizeof(msg), "breakdown at step %d", (int)c.hscal[SL_STATUS]);
+      set_error(nm, msg);
+      return 55;
+    }
+    return 0;
+  };
+  if (int rc = run_mv(c, 'N', xs, c.sy, nc)) return rc;
+  {
+    LinComb l; l.lit(c.sb, 1.0).lit(c.sy, -1.0);
+    if (lincomb(c, c.sy, l, n, st)) return 53;
+    if (int rc = restart_from(c.sy)) return rc;
+  }
+  if (int rc = init_scalars()) return rc;
+  double resid = host_norm(c, r) / bnorm;
+  int it = 0;
+  while (it < a.maxit && resid > a.tol) {
+    ++it;
+    // p = v~/gamma - mu p ; q = (A w~ - gamma mu q)/ksi   (cu:437)
+    { LinComb l; l.dev(vt, qmr::GINV).dev(p, qmr::MU, 1); if (lincomb(c, p, l, n, st)) return 56; }
+    {
+      LinComb l1; l1.lit(atw, 1.0).dev(q, qmr::GMU, 1);
+      if (lincomb(c, q, l1, n, st)) return 56;
+      LinComb l2; l2.dev(q, qmr::KINV);
+      if (lincomb(c, q, l2, n, st)) return 56;
+    }
+    if (int rc = run_mv(c, 'N', p, ap, nc)) return rc;
+    // v~ = Ap - (tau/gamma) v~ ; w~ = q - (tau/ksi) w~   (cu:438)
+    { LinComb l; l.lit(ap, 1.0).dev(vt, qmr::TG, 1); if (lincomb(c, vt, l, n, st)) return 56; }
+    { LinComb l; l.lit(q, 1.0).dev(wt, qmr::TK, 1); if (lincomb(c, wt, l, n, st)) return 56; }
+    if (int rc = run_mv(c, 'N', wt, atw, nc)) return rc;
+    DotSet ds;
+    ds.add(vt, vt, DOT_N2).add(wt, wt, DOT_N2).add(vt, wt, DOT_U).add(vt, atw, DOT_U);
+    if (dots(c, ds, n, SL_D, st)) return 57;
+    hipLaunchKernelGGL(qmr::k_recur, dim3(1), dim3(1), 0, st, c.scal);
+    // d = theta d + kappa p ; s = theta s + kappa Ap ; x += d ; r -= s   (cu:443)
+    {
+      LinComb ld; ld.dev(d, qmr::THETA).dev(p, qmr::KAPPA);
+      LinComb ls; ls.dev(sv, qmr::THETA).dev(ap, qmr::KAPPA);
+      LinComb lx; lx.lit(xs, 1.0).lit(d, 1.0);
+      LinComb lr; lr.lit(r, 1.0).lit(sv, -1.0);
+      if (lincomb(c, d, ld, n, st) || lincomb(c, sv, ls, n, st) || lincomb(c, xs, lx, n, st) ||
+          lincomb(c, r, lr, n, st))
+        return 59;
+    }
+    DotSet dr; dr.add(r, r, DOT_N2);
+    if (dots(c, dr, n, SL_D + 8, st)) return 59;
+    if (fetch_scalars(c, 0, SL_D + 10, st)) return 59;
+    if (c.hscal[SL_STATUS] != 0.0) {
+      char msg[80];
+      std::snprintf(msg, sizeof(msg), "breakdown at step %d", (int)c.hscal[SL_STATUS]);
+      set_error(nm, msg);
+      return 58;
+    }
+    resid = std::sqrt(std::max(0.0, c.hscal[SL_D + 8])) / bnorm;
+    if ((it % RELIABLE_PERIOD) == 0 || resid <= a.tol) {
+      Reliable rr;
+      if (int rc = reliable_check(c, c.sb, xs, r, bnorm, resid, a.tol, nc, rr)) return rc > 1 ? rc : 59;
+      if (rr.converged) { resid = rr.true_res; break; }
+      if (rr.restart) {
+        LinComb l; l.lit(c.sb, 1.0).lit(c.sy, -1.0);
+        if (lincomb(c, c.sy, l, n, st)) return 59;
+        if (int rc = restart_from(c.sy)) return rc;
+        if (int rc = init_scalars()) return rc;
+        resid = rr.true_res;
+      }
+    }
+  }
+  if (resid > a.tol) {
+    Reliable rr;
+    if (int rc = reliable_check(c, c.sb, xs, r, bnorm, resid, a.tol, nc, rr)) return rc > 1 ? rc : 60;
+    resid = rr.true_res;
+  }
+  return solver_end(c, a, xs, it, resid, nc, 61);
+}
+
+// =============================================================== PETRKP ====
+namespace petr {
+enum { BB = SL_R, GG = SL_R + 1, OLDGG = SL_R + 2, QQ = SL_R + 3, ALPHA = SL_X, BETA = SL_X + 2 };
+// dot slot 0 = |GI|^2 or |QI|^2 etc.
+__global__ void k_alpha(double* s, int code) {
+  const double q = s[QQ], g = s[GG];
+  if (q == 0.0) { fail(s, code); return; }
+  st2(s, ALPHA, make_double2(g / q, 0));
+}
+__global__ void k_set(double* s, int dst, int slot) { s[dst] = s[SL_D + 2 * slot]; }
+__global__ void k_beta(double* s) {
+  s[OLDGG] = s[GG];
+  s[GG] = s[SL_D];
+  if (s[OLDGG] == 0.0) { fail(s, 119); return; }
+  st2(s, BETA, make_double2(s[GG] / s[OLDGG], 0));
+}
+}  // namespace petr
+
+int solve_petrkp(Context& c, const SolverArgs& a) {
+  const char* nm = "PETRKP";
+  clear_error();
+  if (!c.ready) { set_error(nm, "invalid argument"); return 110; }
+  if (int rc = ensure_solver_workspace(c, 13)) return rc;
+  const size_t n = (size_t)3 * c.nat0;
+  hipStream_t st = c.stream;
+  float2* W = c.wrk;
+  float2 *ace = W, *gi = W + n, *pi = W + 2 * n, *qi = W + 3 * n, *axi = W + 4 * n, *rv = W + 5 * n,
+         *xs = W + 6 * n;
+  if (int rc = solver_begin(c, a, 110, 111, nm, xs)) return rc;
+  int nc = 0;
+  DotSet db; db.add(c.sb, c.sb, DOT_N2);
+  if (dots(c, db, n, SL_D, st)) return 112;
+  hipLaunchKernelGGL(petr::k_set, dim3(1), dim3(1), 0, st, c.scal, (int)petr::BB, 0);
+  // ACE = A^H b ; GI = PI = ACE   (A complex symmetric: A^H b = conj(A conj(b)))
+  if (int rc = run_mv(c, 'C', c.sb, ace, nc)) return rc;
+  { LinComb l; l.lit(ace, 1.0); if (lincomb(c, gi, l, n, st) || lincomb(c, pi, l, n, st)) return 113; }
+  if (int rc = run_mv(c, 'N', pi, qi, nc)) return rc;
+  DotSet d0; d0.add(gi, gi, DOT_N2).add(qi, qi, DOT_N2);
+  if (dots(c, d0, n, SL_D, st)) return 114;
+  hipLaunchKernelGGL(petr::k_set, dim3(1), dim3(1), 0, st, c.scal, (int)petr::GG, 0);
+  hipLaunchKernelGGL(petr::k_set, dim3(1), dim3(1), 0, st, c.scal, (int)petr::QQ, 1);
+  hipLaunchKernelGGL(petr::k_alpha, dim3(1), dim3(1), 0, st, c.scal, 115);
+  { LinComb l; l.lit(xs, 1.0).dev(pi, petr::ALPHA); if (lincomb(c, xs, l, n, st)) return 115; }
+  if (int rc = run_mv(c, 'N', xs, axi, nc)) return rc;
+  if (fetch_scalars(c, 0, 2, st)) return 115;
+  if (c.hscal[SL_STATUS] != 0.0) { set_error(nm, "breakdown: ||QI|| == 0 while computing initial alpha"); return 115; }
+  double resid = 1.0;
+  int it;
+  for (it = 1; it <= a.maxit; ++it) {
+    if (int rc = run_mv(c, 'C', axi, c.sy, nc)) return rc;
+    { LinComb l; l.lit(ace, 1.0).lit(c.sy, -1.0); if (lincomb(c, gi, l, n, st)) return 117; }
+    DotSet dg; dg.add(gi, gi, DOT_N2);
+    if (dots(c, dg, n, SL_D, st)) return 118;
+    hipLaunchKernelGGL(petr::k_beta, dim3(1), dim3(1), 0, st, c.scal);
+    { LinComb l; l.lit(gi, 1.0).dev(pi, petr::BETA); if (lincomb(c, pi, l, n, st)) return 119; }
+    if (int rc = run_mv(c, 'N', pi, qi, nc)) return rc;
+    DotSet dq; dq.add(qi, qi, DOT_N2);
+    if (dots(c, dq, n, SL_D, st)) return 120;
+    hipLaunchKernelGGL(petr::k_set, dim3(1), dim3(1), 0, st, c.scal, (int)petr::QQ, 0);
+    hipLaunchKernelGGL(petr::k_alpha, dim3(1), dim3(1), 0, st, c.scal, 121);
+    { LinComb l; l.lit(xs, 1.0).dev(pi, petr::ALPHA); if (lincomb(c, xs, l, n, st)) return 121; }
+    // AXI recursive, refreshed exactly every 10th iteration (cu:1333-1338)
+    if (it % 10 != 0) {
+      LinComb l; l.lit(axi, 1.0).dev(qi, petr::ALPHA);
+      if (lincomb(c, axi, l, n, st)) return 122;
+    } else {
+      if (int rc = run_mv(c, 'N', xs, axi, nc)) return rc;
+    }
+    DotSet dr; dr.add(axi, c.sb, DOT_D2);
+    if (dots(c, dr, n, SL_D + 2, st)) return 123;
+    if (fetch_scalars(c, 0, SL_R + 4, st)) return 123;
+    const int status = (int)c.hscal[SL_STATUS];
+    if (status == 119) { set_error(nm, "breakdown: previous ||GI|| == 0 while computing beta"); return 119; }
+    if (status == 121) { set_error(nm, "breakdown: ||QI|| == 0 while computing alpha"); return 121; }
+    const double bb = c.hscal[petr::BB], rr = c.hscal[SL_D + 2];
+    resid = bb > 0.0 ? std::sqrt(std::max(rr, 0.0) / bb) : INFINITY;
+    if (resid < a.tol) break;
+  }
+  return solver_end(c, a, xs, it <= a.maxit ? it : a.maxit, resid, nc, 124);
+}
+
+// =============================================================== GPBICG ====
+namespace gpb {
+enum { R0RN = SL_X, BETA = SL_X + 2, ALPHA = SL_X + 4, ETA = SL_X + 6, DZ = SL_X + 8, NALPHA = SL_X + 10,
+       NETA = SL_X + 12, NDZ = SL_X + 14 };
+__global__ void k_set_r0rn(double* s) { st2(s, R0RN, ld2(s, SL_D)); st2(s, BETA, make_double2(0, 0)); }
+__global__ void k_alpha(double* s) {
+  const double2 g0 = ld2(s, SL_D);
+  if (zzero(g0)) { fail(s, 34); return; }
+  const double2 al = zdiv(ld2(s, R0RN), g0);
+  st2(s, ALPHA, al);
+  st2(s, NALPHA, zscale(al, -1.0));
+}
+// slots: 0 G1=<at,at>, 1 G4=<at,t>, [2 G2=<y,y>, 3 G3=<at,y>, 4 G5=<y,t>]
+__global__ void k_dz(double* s, int first) {
+  const double2 G1 = ld2(s, SL_D), G4 = ld2(s, SL_D + 2);
+  double2 dz, eta;
+  if (first) {
+    if (zzero(G1)) { fail(s, 36); return; }
+    eta = make_double2(0, 0);
+    dz = zdiv(G4, G1);
+  } else {
+    const double2 G2 = ld2(s, SL_D + 4), G3 = ld2(s, SL_D + 6), G5 = ld2(s, SL_D + 8);
+    const double2 den = zsub(zmul(G1, G2), zmul(G3, zconj(G3)));
+    if (zzero(den)) { fail(s, 36); return; }
+    dz = zdiv(zsub(zmul(G2, G4), zmul(G5, G3)), den);
+    eta = zdiv(zsub(zmul(G1, G5), zmul(zconj(G3), G4)), den);
+  }
+  st2(s, DZ, dz); st2(s, ETA, eta);
+  st2(s, NDZ, zscale(dz, -1.0)); st2(s, NETA, zscale(eta, -1.0));
+}
+__global__ void k_beta(double* s) {
+  const double2 g0 = ld2(s, SL_D), r0rn = ld2(s, R0RN), dz = ld2(s, DZ);
+  if (zzero(r0rn) || zzero(dz)) { fail(s, 39); return; }
+  st2(s, BETA, zdiv(zdiv(zmul(ld2(s, ALPHA), g0), dz), r0rn));
+  st2(s, R0RN, g0);
+}
+// fused GPBiCG update (cu:430): u, z, x, r
+__global__ void k_update(float2* __restrict__ W, float2* __restrict__ xsol, const double* __restrict__ s, size_t n) {
+  const double2 dz = ld2(s, DZ), eta = ld2(s, ETA), be = ld2(s, BETA), al = ld2(s, ALPHA);
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const float2 p = W[i + n], r = W[i + 2 * n], y = W[i + 3 * n], t = W[i + 4 * n], ap = W[i + 5 * n],
+                 at = W[i + 6 * n], uo = W[i + 7 * n], zo = W[i + 9 * n], to = W[i + 11 * n];
+    auto D = [](float2 v) { return make_double2(v.x, v.y); };
+    const double2 u = zadd(zmul(dz, D(ap)), zmul(eta, zadd(zsub(D(to), D(r)), zmul(be, D(uo)))));
+    const double2 z = zsub(zadd(zmul(dz, D(r)), zmul(eta, D(zo))), zmul(al, u));
+    const double2 xn = zadd(zadd(D(xsol[i]), zmul(al, D(p))), z);
+    const double2 rn = zsub(zsub(D(t), zmul(eta, D(y))), zmul(dz, D(at)));
+    W[i + 7 * n] = make_float2((float)u.x, (float)u.y);
+    W[i + 9 * n] = make_float2((float)z.x, (float)z.y);
+    xsol[i] = make_float2((float)xn.x, (float)xn.y);
+    W[i + 2 * n] = make_float2((float)rn.x, (float)rn.y);
+  }
+}
+}  // namespace gpb
+
+int solve_gpbicg(Context& c, const SolverArgs& a) {
+  const char* nm = "GPBICG";
+  clear_error();
+  if (!c.ready) { set_error(nm, "invalid argument"); return 30; }
+  if (int rc = ensure_solver_workspace(c, 13)) return rc;
+  const size_t n = (size_t)3 * c.nat0;
+  hipStream_t st = c.stream;
+  float2* W = c.wrk;
+  float2 *r0 = W, *p = W + n, *r = W + 2 * n, *yv = W + 3 * n, *t = W + 4 * n, *ap = W + 5 * n, *at = W + 6 * n,
+         *u = W + 7 * n, *wv = W + 8 * n, *zv = W + 9 * n, *xsol = W + 10 * n, *told = W + 11 * n;
+  if (int rc = solver_begin(c, a, 30, 31, nm, xsol)) return rc;
+  const double bnorm = host_norm(c, c.sb);
+  if (bnorm < 0) return 32;
+  if (bnorm == 0.0) { set_error(nm, "zero RHS norm"); return 32; }
+  int nc = 0;
+  LinComb z0; z0.lit(r0, 0.0);
+  auto restart_from = [&](const float2* src) -> int {
+    LinComb cp; cp.lit(src, 1.0);
+    if (lincomb(c, r0, cp, n, st) || lincomb(c, r, cp, n, st)) return 37;
+    for (float2* v : {p, yv, t, ap, at, u, wv, zv, told})
+      if (lincomb(c, v, z0, n, st)) return 37;
+    DotSet d; d.add(r0, r0, DOT_C);
+    if (dots(c, d, n, SL_D, st)) return 37;
+    hipLaunchKernelGGL(gpb::k_set_r0rn, dim3(1), dim3(1), 0, st, c.scal);
+    return 0;
+  };
+  if (int rc = run_mv(c, 'N', xsol, c.sy, nc)) return rc;
+  { LinComb l; l.lit(c.sb, 1.0).lit(c.sy, -1.0); if (lincomb(c, c.sy, l, n, st)) return 32; }
+  if (int rc = restart_from(c.sy)) return rc;
+  double resid = host_norm(c, r) / bnorm;
+  int it = 0;
+  bool first = true;
+  while (it < a.maxit && resid > a.tol) {
+    ++it;
+    // p = r + beta (p - u)
+    { LinComb l; l.lit(r, 1.0).dev(p, gpb::BETA).dev(u, gpb::BETA, 1); if (lincomb(c, p, l, n, st)) return 33; }
+    if (int rc = run_mv(c, 'N', p, ap, nc)) return rc;
+    DotSet d0; d0.add(r0, ap, DOT_C);
+    if (dots(c, d0, n, SL_D, st)) return 33;
+    hipLaunchKernelGGL(gpb::k_alpha, dim3(1), dim3(1), 0, st, c.scal);
+    // y = t - r + alpha (ap - w) ; t_old = t ; t = r - alpha ap   (cu:425)
+    {
+      LinComb ly; ly.lit(t, 1.0).lit(r, -1.0).dev(ap, gpb::ALPHA).dev(wv, gpb::NALPHA);
+      LinComb lo; lo.lit(t, 1.0);
+      LinComb lt; lt.lit(r, 1.0).dev(ap, gpb::NALPHA);
+      if (lincomb(c, yv, ly, n, st) || lincomb(c, told, lo, n, st) || lincomb(c, t, lt, n, st)) return 34;
+    }
+    if (int rc = run_mv(c, 'N', t, at, nc)) return rc;
+    DotSet dd; dd.add(at, at, DOT_C).add(at, t, DOT_C);
+    if (!first) dd.add(yv, yv, DOT_C).add(at, yv, DOT_C).add(yv, t, DOT_C);
+    if (dots(c, dd, n, SL_D, st)) return 35;
+    hipLaunchKernelGGL(gpb::k_dz, dim3(1), dim3(1), 0, st, c.scal, first ? 1 : 0);
+    {
+      const int th = 256;
+      const unsigned nb = (unsigned)std::min<size_t>(8192, (n + th * 4 - 1) / (th * 4));
+      hipLaunchKernelGGL(gpb::k_update, dim3(nb), dim3(th), 0, st, W, xsol, (const double*)c.scal, n);
+    }
+    DotSet dr; dr.add(r, r, DOT_N2);
+    if (dots(c, dr, n, SL_D + 10, st) || fetch_scalars(c, 0, SL_D + 12, st)) return 37;
+    if (c.hscal[SL_STATUS] != 0.0) {
+      const int code = (int)c.hscal[SL_STATUS];
+      set_error(nm, code == 34 ? "alpha breakdown" : code == 36 ? "dzeta/eta breakdown" : "beta breakdown");
+      return code;
+    }
+    resid = std::sqrt(std::max(0.0, c.hscal[SL_D + 10])) / bnorm;
+    if ((it % RELIABLE_PERIOD) == 0 || resid <= a.tol) {
+      Reliable rr;
+      if (int rc = reliable_check(c, c.sb, xsol, r, bnorm, resid, a.tol, nc, rr)) return rc > 1 ? rc : 37;
+      if (rr.converged) { resid = rr.true_res; break; }
+      if (rr.restart) {
+        LinComb l; l.lit(c.sb, 1.0).lit(c.sy, -1.0);
+        if (lincomb(c, c.sy, l, n, st)) return 37;
+        if (int rc = restart_from(c.sy)) return rc;
+        first = true;
+        resid = rr.true_res;
+        continue;
+      }
+    }
+    DotSet db; db.add(r0, r, DOT_C);
+    if (dots(c, db, n, SL_D, st)) return 39;
+    hipLaunchKernelGGL(gpb::k_beta, dim3(1), dim3(1), 0, st, c.scal);
+    { LinComb l; l.lit(at, 1.0).dev(ap, gpb::BETA); if (lincomb(c, wv, l, n, st)) return 39; }
+    first = false;
+  }
+  if (fetch_scalars(c, 0, 1, st)) return 39;
+  if (c.hscal[SL_STATUS] != 0.0) { set_error(nm, "beta breakdown"); return 39; }
+  if (resid > a.tol) {
+    Reliable rr;
+    if (int rc = reliable_check(c, c.sb, xsol, r, bnorm, resid, a.tol, nc, rr)) return rc > 1 ? rc : 40;
+    resid = rr.true_res;
+  }
+  return solver_end(c, a, xsol, it, resid, nc, 40);
+}
+
+// =============================================================== PBCGST ====
+namespace pbc {
+enum { RHO = SL_X, ALPHA = SL_X + 2, OMEGA = SL_X + 4, BETA = SL_X + 6, NALPHA = SL_X + 8, NOMEGA = SL_X + 10,
+       BOM = SL_X + 12 };
+__global__ void k_restart(double* s) {
+  st2(s, RHO, make_double2(1, 0)); st2(s, ALPHA, make_double2(1, 0)); st2(s, OMEGA, make_double2(1, 0));
+}
+__global__ void k_beta(double* s) {
+  const double2 nr = ld2(s, SL_D), kap = zmul(ld2(s, RHO), ld2(s, OMEGA));
+  if (zzero(kap)) { fail(s, 75); return; }
+  const double2 be = zdiv(zmul(nr, ld2(s, ALPHA)), kap);
+  st2(s, BETA, be);
+  st2(s, BOM, zscale(zmul(be, ld2(s, OMEGA)), -1.0));
+  st2(s, RHO, nr);
+}
+__global__ void k_alpha(double* s) {
+  const double2 xi = ld2(s, SL_D);
+  if (zzero(xi)) { fail(s, 78); return; }
+  const double2 al = zdiv(ld2(s, RHO), xi);
+  st2(s, ALPHA, al);
+  st2(s, NALPHA, zscale(al, -1.0));
+}
+__global__ void k_omega(double* s) {
+  const double2 tt = ld2(s, SL_D), ts = ld2(s, SL_D + 2);
+  if (zzero(tt)) { fail(s, 82); return; }
+  const double2 om = zdiv(ts, tt);
+  st2(s, OMEGA, om);
+  st2(s, NOMEGA, zscale(om, -1.0));
+}
+// out = in / adia (left diagonal preconditioner DIAGL on compact storage, cu:448)
+__global__ void k_precon(float2* __restrict__ out, const float2* __restrict__ in, const float2* __restrict__ adia,
+                         size_t n) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const float2 a = in[i], d = adia[i];
+    const float den = d.x * d.x + d.y * d.y;
+    out[i] = make_float2((a.x * d.x + a.y * d.y) / den, (a.y * d.x - a.x * d.y) / den);
+  }
+}
+}  // namespace pbc
+
+int solve_pbcgst(Context& c, const SolverArgs& a) {
+  const char* nm = "PBCGST";
+  clear_error();
+  if (!c.ready) { set_error(nm, "invalid argument"); return 70; }
+  if (int rc = ensure_solver_workspace(c, 13)) return rc;
+  const size_t n = (size_t)3 * c.nat0;
+  hipStream_t st = c.stream;
+  float2* W = c.wrk;
+  float2 *r = W, *rt = W + n, *p = W + 2 * n, *sv = W + 3 * n, *v = W + 4 * n, *t = W + 5 * n, *xs = W + 6 * n,
+         *rtrue = W + 7 * n;
+  if (int rc = solver_begin(c, a, 70, 71, nm, xs)) return rc;
+  const double bnorm = host_norm(c, c.sb);
+  if (bnorm < 0) return 72;
+  const int th = 256;
+  const unsigned nb = (unsigned)std::min<size_t>(8192, (n + th * 4 - 1) / (th * 4));
+  auto precon = [&](float2* out, const float2* in) {
+    hipLaunchKernelGGL(pbc::k_precon, dim3(nb), dim3(th), 0, st, out, in, c.adia_c, n);
+  };
+  int total_it = 0, nc = 0, restart = 0;
+  double resid = 1.0;
+  bool converged = false;
+  LinComb z; z.lit(r, 0.0);
+  auto true_residual = [&]() -> int {
+    if (int rc = run_mv(c, 'N', xs, c.sy, nc)) return rc;
+    LinComb l; l.lit(c.sb, 1.0).lit(c.sy, -1.0);
+    if (lincomb(c, rtrue, l, n, st)) return 84;
+    DotSet d; d.add(rtrue, rtrue, DOT_N2);
+    if (dots(c, d, n, SL_D + 16, st)) return 84;
+    return 0;
+  };
+  while (total_it < a.maxit && !converged) {
+    // rtrue = b - A x ; r = DIAGL(rtrue) ; rtilde = r ; p = v = 0 ; rho = alpha = omega = 1
+    if (int rc = run_mv(c, 'N', xs, c.sy, nc)) return rc;
+    { LinComb l; l.lit(c.sb, 1.0).lit(c.sy, -1.0); if (lincomb(c, rtrue, l, n, st)) return 73; }
+    precon(r, rtrue);
+    { LinComb cp; cp.lit(r, 1.0); if (lincomb(c, rt, cp, n, st) || lincomb(c, p, z, n, st) || lincomb(c, v, z, n, st)) return 73; }
+    hipLaunchKernelGGL(pbc::k_restart, dim3(1), dim3(1), 0, st, c.scal);
+    for (int local = 0; local < 5 && total_it < a.maxit && !converged; ++local) {
+      DotSet d1; d1.add(rt, r, DOT_C);
+      if (dots(c, d1, n, SL_D, st)) return 74;
+      hipLaunchKernelGGL(pbc::k_beta, dim3(1), dim3(1), 0, st, c.scal);
+      { LinComb l; l.lit(r, 1.0).dev(p, pbc::BETA).dev(v, pbc::BOM); if (lincomb(c, p, l, n, st)) return 76; }
+      if (int rc = run_mv(c, 'N', p, c.sy, nc)) return rc;
+      precon(v, c.sy);
+      DotSet d2; d2.add(rt, v, DOT_C);
+      if (dots(c, d2, n, SL_D, st)) return 77;
+      hipLaunchKernelGGL(pbc::k_alpha, dim3(1), dim3(1), 0, st, c.scal);
+      { LinComb l; l.lit(r, 1.0).dev(v, pbc::NALPHA); if (lincomb(c, sv, l, n, st)) return 79; }
+      DotSet d3; d3.add(sv, sv, DOT_N2);
+      if (dots(c, d3, n, SL_D + 16, st) || fetch_scalars(c, 0, SL_D + 18, st)) return 79;
+      if (int code = (int)c.hscal[SL_STATUS]) {
+        set_error(nm, code == 75 ? "hard breakdown at step 6 (rho0*omega == 0)"
+                                 : "hard breakdown at step 10 (rtilde^H v == 0)");
+        return code;
+      }
+      if (std::sqrt(std::max(0.0, c.hscal[SL_D + 16])) < 1.19209e-7) {
+        // PIM step 12 soft breakdown: accept x += alpha p and restart
+        LinComb l; l.lit(xs, 1.0).dev(p, pbc::ALPHA);
+        if (lincomb(c, xs, l, n, st)) return 79;
+        ++total_it;
+        break;
+      }
+      if (int rc = run_mv(c, 'N', sv, c.sy, nc)) return rc;
+      precon(t, c.sy);
+      DotSet d4; d4.add(t, t, DOT_C).add(t, sv, DOT_C);
+      if (dots(c, d4, n, SL_D, st)) return 81;
+      hipLaunchKernelGGL(pbc::k_omega, dim3(1), dim3(1), 0, st, c.scal);
+      {
+        LinComb lx; lx.lit(xs, 1.0).dev(p, pbc::ALPHA).dev(sv, pbc::OMEGA);
+        LinComb lr; lr.lit(sv, 1.0).dev(t, pbc::NOMEGA);
+        if (lincomb(c, xs, lx, n, st) || lincomb(c, r, lr, n, st)) return 83;
+      }
+      ++total_it;
+      if (int rc = true_residual()) return rc;
+      if (fetch_scalars(c, 0, SL_D + 18, st)) return 84;
+      if ((int)c.hscal[SL_STATUS] == 82) { set_error(nm, "hard breakdown at step 14 (t^H t == 0)"); return 82; }
+      resid = bnorm > 0.0 ? std::sqrt(std::max(0.0, c.hscal[SL_D + 16])) / bnorm : INFINITY;
+      if (resid <= a.tol) converged = true;
+    }
+    ++restart;
+  }
+  if (!converged) {
+    if (int rc = true_residual()) return rc;
+    if (fetch_scalars(c, SL_D + 16, 2, st)) return 85;
+    resid = bnorm > 0.0 ? std::sqrt(std::max(0.0, c.hscal[SL_D + 16])) / bnorm : INFINITY;
+  }
+  if (int rc = solver_end(c, a, xs, total_it, resid, nc, 86)) return rc;
+  if (!converged) { set_error(nm, "MAXIT reached before requested tolerance"); return 87; }
+  return 0;
+}
+
+}  // namespace ddhip
+
+using namespace ddhip;
+
+#define DDH_SOLVER_ENTRY(name, fn)                                                                        \
+  extern "C" DDSCAT_HIP_API int name(const void* b, void* x, const void* adia, const void* aoff, int ndim,  \
+                                     int maxit, float tol, int* itno, float* tole, int* ncompte) {         \
+    SolverArgs a{b, x, adia, aoff, ndim, maxit, tol, itno, tole, ncompte};                                 \
+    return fn(ctx(), a);                                                                                   \
+  }
+
+DDH_SOLVER_ENTRY(ddscat_cuda_solve_pbcgs2_f32, solve_pbcgs2)
+DDH_SOLVER_ENTRY(ddscat_cuda_solve_qmrccg_f32, solve_qmrccg)
+DDH_SOLVER_ENTRY(ddscat_cuda_solve_petrkp_f32, solve_petrkp)
+DDH_SOLVER_ENTRY(ddscat_cuda_solve_gpbicg_f32, solve_gpbicg)
+DDH_SOLVER_ENTRY(ddscat_cuda_solve_pbcgst_f32, solve_pbcgst)

@@ -1,0 +1,55 @@
+// Shared pieces of the solver entry points.
+#pragma once
+#include "solver_common.hpp"
+
+namespace ddhip {
+
+// reliable-residual policy (ddscat_cuda_unified/cuda/ddscat_matvec_cuda.cu:221-222)
+constexpr int RELIABLE_PERIOD = 20;
+constexpr double RELIABLE_GAP = 1.e-3;
+
+// device scalar block layout (doubles)
+enum {
+  SL_STATUS = 0,  // first breakdown code raised by a scalar kernel (0 = none)
+  SL_RESID = 1,   // relative residual computed on the device
+  SL_FLAGS = 2,   // PBCGS2 reliable-update flags
+  SL_D = 8,       // dot-product results, complex i at SL_D + 2 i (up to 24 doubles)
+  SL_X = 64,      // solver complex scalars
+  SL_R = 160,     // solver real scalars
+};
+
+struct SolverArgs {
+  const void* b;
+  void* x;
+  const void* adia;
+  const void* aoff;
+  int ndim, maxit;
+  float tol;
+  int* itno;
+  float* tole;
+  int* ncompte;
+};
+
+struct Reliable {
+  double true_res = INFINITY, gap = INFINITY;
+  bool converged = false, restart = false;
+};
+
+// argument checks, boundary copies (full host -> compact device), clears the
+// scalar block; initial guess gathered into xs
+int solver_begin(Context& c, const SolverArgs& a, int code_arg, int code_io, const char* name, float2* xs);
+// publishes xs as the resident solution, copies it to the host full vector
+int solver_end(Context& c, const SolverArgs& a, const float2* xs, int it, double resid, int nc, int code);
+int reliable_check(Context& c, const float2* rhs, const float2* xcur, const float2* rrec, double rhsnorm,
+                   double rec_res, double tol, int& nc, Reliable& rr);
+
+int solve_pbcgs2(Context& c, const SolverArgs& a);
+int solve_qmrccg(Context& c, const SolverArgs& a);
+int solve_petrkp(Context& c, const SolverArgs& a);
+int solve_gpbicg(Context& c, const SolverArgs& a);
+int solve_pbcgst(Context& c, const SolverArgs& a);
+int solve_bicgstab4(Context& c, const SolverArgs& a);
+int solve_gpbicgstab2(Context& c, const SolverArgs& a);
+int solve_gpbicgstab4(Context& c, const SolverArgs& a);
+
+}  // namespace ddhip
