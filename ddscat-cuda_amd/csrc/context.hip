@@ -24,9 +24,16 @@ void clear_error() { g_last_error.clear(); }
 static Context g_ctx;
 Context& ctx() { return g_ctx; }
 
+// hipFree only what was allocated: a context that never reached the device
+// releases without touching the HIP runtime
+template <typename T>
+static void dfree(T*& p) {
+  if (p) hipFree(p);
+  p = nullptr;
+}
+
 void release_solver_workspace(Context& c) {
-  hipFree(c.wrk);
-  c.wrk = nullptr;
+  dfree(c.wrk);
   c.wrk_vectors = 0;
 }
 
@@ -43,13 +50,13 @@ void release_context(Context& c) {
   if (c.stream) hipStreamSynchronize(c.stream);
   matvec_release(c);
   slices_release(c);
-  hipFree(c.green);
-  hipFree(c.iocc); hipFree(c.occ); hipFree(c.rowptr);
-  hipFree(c.adia); hipFree(c.aoff); hipFree(c.adia_c); hipFree(c.aoff_c);
-  hipFree(c.xfull); hipFree(c.yfull);
-  hipFree(c.sx); hipFree(c.sy); hipFree(c.sb);
+  dfree(c.green);
+  dfree(c.iocc); dfree(c.occ); dfree(c.rowptr);
+  dfree(c.adia); dfree(c.aoff); dfree(c.adia_c); dfree(c.aoff_c);
+  dfree(c.xfull); dfree(c.yfull);
+  dfree(c.sx); dfree(c.sy); dfree(c.sb);
   release_solver_workspace(c);
-  hipFree(c.red); hipFree(c.scal);
+  dfree(c.red); dfree(c.scal);
   if (c.hscal) hipHostFree(c.hscal);
   hipStream_t keep = c.stream;
   int backend = c.backend, prec = c.precision;

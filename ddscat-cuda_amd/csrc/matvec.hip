@@ -329,10 +329,10 @@ int matvec_setup(Context& c) {
 }
 
 void matvec_release(Context& c) {
-  hipFree(c.mv.twx); hipFree(c.mv.twy); hipFree(c.mv.twz);
-  c.mv.twx = c.mv.twy = c.mv.twz = nullptr;
-  hipFree(c.a1); hipFree(c.a2);
-  c.a1 = c.a2 = nullptr;
+  for (float2** p : {&c.mv.twx, &c.mv.twy, &c.mv.twz, &c.a1, &c.a2}) {
+    if (*p) hipFree(*p);
+    *p = nullptr;
+  }
 }
 
 int matvec_launch(Context& c, int cwhat, const float2* x, float2* y, const float2* adia,

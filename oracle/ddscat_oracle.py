@@ -106,6 +106,8 @@ def green_elements(x, y, z, akd: float, idipint: int = 0) -> np.ndarray:
     y = np.asarray(y, dtype=np.float64)
     z = np.asarray(z, dtype=np.float64)
     shape = np.broadcast(x, y, z).shape
+    if shape == ():
+        return green_elements(np.atleast_1d(x), np.atleast_1d(y), np.atleast_1d(z), akd, idipint)[:, 0]
     x, y, z = (np.broadcast_to(a, shape).astype(np.float64) for a in (x, y, z))
     out = np.zeros((6,) + shape, dtype=np.complex128)
     r2 = x * x + y * y + z * z
