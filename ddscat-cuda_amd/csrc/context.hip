@@ -108,6 +108,7 @@ int finish_prepare(Context& c, const int16_t* iocc, int code_alloc, int code_occ
   }
   DDH_TRY(hipMemcpy(c.occ, occ.data(), (size_t)c.nat0 * sizeof(int), hipMemcpyHostToDevice), "occupied map H2D", code_occ);
   DDH_TRY(hipMemcpy(c.rowptr, rowptr.data(), ((size_t)nrows + 1) * sizeof(int), hipMemcpyHostToDevice), "row map H2D", code_occ);
+  if (c.backend == BACKEND_SLICES && slices_build_qmap(c)) return code_alloc;
   DDH_TRY(hipMemset(c.adia, 0, n * sizeof(float2)), "clear adia", code_alloc);
   DDH_TRY(hipMemset(c.aoff, 0, n * sizeof(float2)), "clear aoff", code_alloc);
   return 0;

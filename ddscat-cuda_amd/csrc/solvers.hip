@@ -507,8 +507,9 @@ int solve_petrkp(Context& c, const SolverArgs& a) {
   const size_t n = (size_t)3 * c.nat0;
   hipStream_t st = c.stream;
   float2* W = c.wrk;
-  float2 *ace = W, *gi = W + n, *pi = W + 2 * n, *qi = W + 3 * n, *axi = W + 4 * n, *rv = W + 5 * n,
-         *xs = W + 6 * n;
+  // work layout [ACE, GI, PI, QI, AXI, R, X] (cu:1287-1289); R = AXI - B is
+  // reduced on the fly and never stored
+  float2 *ace = W, *gi = W + n, *pi = W + 2 * n, *qi = W + 3 * n, *axi = W + 4 * n, *xs = W + 6 * n;
   if (int rc = solver_begin(c, a, 110, 111, nm, xs)) return rc;
   int nc = 0;
   DotSet db; db.add(c.sb, c.sb, DOT_N2);
@@ -771,7 +772,7 @@ int solve_pbcgst(Context& c, const SolverArgs& a) {
   auto precon = [&](float2* out, const float2* in) {
     hipLaunchKernelGGL(pbc::k_precon, dim3(nb), dim3(th), 0, st, out, in, c.adia_c, n);
   };
-  int total_it = 0, nc = 0, restart = 0;
+  int total_it = 0, nc = 0;
   double resid = 1.0;
   bool converged = false;
   LinComb z; z.lit(r, 0.0);
@@ -832,7 +833,6 @@ int solve_pbcgst(Context& c, const SolverArgs& a) {
       resid = bnorm > 0.0 ? std::sqrt(std::max(0.0, c.hscal[SL_D + 16])) / bnorm : INFINITY;
       if (resid <= a.tol) converged = true;
     }
-    ++restart;
   }
   if (!converged) {
     if (int rc = true_residual()) return rc;
