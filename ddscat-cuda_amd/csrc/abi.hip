@@ -136,6 +136,12 @@ extern "C" DDSCAT_HIP_API int ddscat_hip_set_precision(int precision) {
   return 0;
 }
 
+extern "C" DDSCAT_HIP_API int ddscat_hip_set_fft_kernels(int generic) {
+  if (generic != 0 && generic != 1) { set_error("set_fft_kernels", "expected 0 or 1"); return 1; }
+  ctx().fft_generic = generic;
+  return 0;
+}
+
 extern "C" DDSCAT_HIP_API int ddscat_hip_apply_device(int cwhat, const void* x, void* y, const void* adia,
                                                       const void* aoff, int layout, void* stream) {
   Context& c = ctx();

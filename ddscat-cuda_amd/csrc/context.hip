@@ -51,7 +51,7 @@ void release_context(Context& c) {
   matvec_release(c);
   slices_release(c);
   dfree(c.green);
-  dfree(c.iocc); dfree(c.occ); dfree(c.rowptr);
+  dfree(c.iocc); dfree(c.occ); dfree(c.rowptr); dfree(c.rowlo); dfree(c.qmap);
   dfree(c.adia); dfree(c.aoff); dfree(c.adia_c); dfree(c.aoff_c);
   dfree(c.xfull); dfree(c.yfull);
   dfree(c.sx); dfree(c.sy); dfree(c.sb);
@@ -59,11 +59,12 @@ void release_context(Context& c) {
   dfree(c.red); dfree(c.scal);
   if (c.hscal) hipHostFree(c.hscal);
   hipStream_t keep = c.stream;
-  int backend = c.backend, prec = c.precision;
+  int backend = c.backend, prec = c.precision, gen = c.fft_generic;
   c = Context{};
   c.stream = keep;
   c.backend = backend;
   c.precision = prec;
+  c.fft_generic = gen;
 }
 
 // Common tail of prepare_green / prepare: dimensions already set, Green
@@ -92,14 +93,31 @@ int finish_prepare(Context& c, const int16_t* iocc, int code_alloc, int code_occ
   std::vector<int> occ;
   occ.reserve((size_t)c.nat0);
   const int nrows = c.ny * c.nz;
-  std::vector<int> rowptr((size_t)nrows + 1, 0);
+  std::vector<int> rowptr((size_t)nrows + 1, 0), rowlo((size_t)nrows, 0);
+  bool contiguous = true;
   for (int g = 0; g < nrows; ++g) {
     rowptr[g] = (int)occ.size();
     const int16_t* row = iocc + (size_t)g * c.nx;
+    int first = -1, last = -1;
     for (int ix = 0; ix < c.nx; ++ix)
-      if (row[ix] != 0) occ.push_back(g * c.nx + ix);
+      if (row[ix] != 0) {
+        occ.push_back(g * c.nx + ix);
+        if (first < 0) first = ix;
+        last = ix;
+      }
+    rowlo[g] = first < 0 ? 0 : first;
+    if (first >= 0 && last - first + 1 != (int)occ.size() - rowptr[g]) contiguous = false;
   }
   rowptr[nrows] = (int)occ.size();
+  c.rows_contiguous = contiguous ? 1 : 0;
+  DDH_TRY(hipMalloc((void**)&c.rowlo, (size_t)nrows * sizeof(int)), "malloc row map", code_alloc);
+  DDH_TRY(hipMemcpy(c.rowlo, rowlo.data(), (size_t)nrows * sizeof(int), hipMemcpyHostToDevice), "row map H2D", code_alloc);
+  if (!contiguous || c.backend == BACKEND_SLICES) {
+    std::vector<int> qmap((size_t)c.nat, -1);
+    for (size_t q = 0; q < occ.size() && q < (size_t)c.nat0; ++q) qmap[(size_t)occ[q]] = (int)q;
+    DDH_TRY(hipMalloc((void**)&c.qmap, (size_t)c.nat * sizeof(int)), "malloc site map", code_alloc);
+    DDH_TRY(hipMemcpy(c.qmap, qmap.data(), (size_t)c.nat * sizeof(int), hipMemcpyHostToDevice), "site map H2D", code_alloc);
+  }
   if ((int)occ.size() != c.nat0) {
     char msg[160];
     std::snprintf(msg, sizeof(msg), "IOCC contains %zu occupied sites but NAT0=%d", occ.size(), c.nat0);
@@ -108,7 +126,6 @@ int finish_prepare(Context& c, const int16_t* iocc, int code_alloc, int code_occ
   }
   DDH_TRY(hipMemcpy(c.occ, occ.data(), (size_t)c.nat0 * sizeof(int), hipMemcpyHostToDevice), "occupied map H2D", code_occ);
   DDH_TRY(hipMemcpy(c.rowptr, rowptr.data(), ((size_t)nrows + 1) * sizeof(int), hipMemcpyHostToDevice), "row map H2D", code_occ);
-  if (c.backend == BACKEND_SLICES && slices_build_qmap(c)) return code_alloc;
   DDH_TRY(hipMemset(c.adia, 0, n * sizeof(float2)), "clear adia", code_alloc);
   DDH_TRY(hipMemset(c.aoff, 0, n * sizeof(float2)), "clear aoff", code_alloc);
   return 0;

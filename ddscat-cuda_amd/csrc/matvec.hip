@@ -342,7 +342,7 @@ int matvec_launch(Context& c, int cwhat, const float2* x, float2* y, const float
   const int cj = (cwhat == 'C') ? 1 : 0;
   const int compact = layout ? 1 : 0;
   if (ev) hipEventRecord(ev[0], st);
-  {
+  if (c.fft_generic || !reg_pass(c, 1, cj, compact, x, y, adia, aoff, st)) {
     dim3 grid((nrows + m.rx_fwd - 1) / m.rx_fwd, 3);
     size_t lds = (size_t)(m.gx + m.rx_fwd * m.gx) * sizeof(float2);
     hipLaunchKernelGGL(k_pass1_xfwd<PER_WIDE>, grid, dim3(MV_THREADS), lds, st, x, c.a1, c.rowptr,
@@ -350,7 +350,7 @@ int matvec_launch(Context& c, int cwhat, const float2* x, float2* y, const float
     DDH_LAUNCH_OK("MATVEC pass 1 (x FFT)", 21);
   }
   if (ev) hipEventRecord(ev[1], st);
-  {
+  if (c.fft_generic || !reg_pass(c, 2, cj, compact, x, y, adia, aoff, st)) {
     dim3 grid((m.gx + m.cy - 1) / m.cy, c.nz, 3);
     size_t lds = (size_t)(m.gy + m.cy * m.gy) * sizeof(float2);
     hipLaunchKernelGGL((k_pass_y<+1, PER_WIDE>), grid, dim3(MV_THREADS), lds, st, c.a1, c.a2, m.twy,
@@ -358,7 +358,7 @@ int matvec_launch(Context& c, int cwhat, const float2* x, float2* y, const float
     DDH_LAUNCH_OK("MATVEC pass 2 (y FFT)", 22);
   }
   if (ev) hipEventRecord(ev[2], st);
-  {
+  if (c.fft_generic || !reg_pass(c, 3, cj, compact, x, y, adia, aoff, st)) {
     dim3 grid((m.gx + m.cz - 1) / m.cz, m.gy);
     size_t lds = (size_t)(m.gz + 3 * m.cz * m.gz) * sizeof(float2);
     hipLaunchKernelGGL(k_pass3_z_green<PER_NARROW>, grid, dim3(MV_THREADS), lds, st, c.a2, c.green,
@@ -366,7 +366,7 @@ int matvec_launch(Context& c, int cwhat, const float2* x, float2* y, const float
     DDH_LAUNCH_OK("MATVEC pass 3 (z FFT + Green)", 23);
   }
   if (ev) hipEventRecord(ev[3], st);
-  {
+  if (c.fft_generic || !reg_pass(c, 4, cj, compact, x, y, adia, aoff, st)) {
     dim3 grid((m.gx + m.cy - 1) / m.cy, c.nz, 3);
     size_t lds = (size_t)(m.gy + m.cy * m.gy) * sizeof(float2);
     hipLaunchKernelGGL((k_pass_y<-1, PER_WIDE>), grid, dim3(MV_THREADS), lds, st, c.a2, c.a1, m.twy,
@@ -374,7 +374,7 @@ int matvec_launch(Context& c, int cwhat, const float2* x, float2* y, const float
     DDH_LAUNCH_OK("MATVEC pass 4 (y inverse FFT)", 24);
   }
   if (ev) hipEventRecord(ev[4], st);
-  {
+  if (c.fft_generic || !reg_pass(c, 5, cj, compact, x, y, adia, aoff, st)) {
     dim3 grid((nrows + m.rx_inv - 1) / m.rx_inv);
     size_t lds = (size_t)(m.gx + 3 * m.rx_inv * m.gx) * sizeof(float2);
     const float inv = 1.f / (float)m.ngrid;
@@ -383,6 +383,7 @@ int matvec_launch(Context& c, int cwhat, const float2* x, float2* y, const float
                        nrows, m.rx_inv, c.nat, c.nat0, compact, cj, inv);
     DDH_LAUNCH_OK("MATVEC pass 5 (x inverse FFT + finish)", 25);
   }
+  DDH_LAUNCH_OK("MATVEC register passes", 20);
   if (ev) hipEventRecord(ev[5], st);
   c.matvec_count++;
   return 0;

@@ -325,7 +325,6 @@ int slices_setup(Context& c) {
     DDH_TRY(hipEventCreateWithFlags(&s.ev_done[i], hipEventDisableTiming), "create SLICES event", 206);
   }
   DDH_TRY(hipEventCreateWithFlags(&s.ev_in, hipEventDisableTiming), "create SLICES event", 206);
-  DDH_TRY(hipMalloc((void**)&s.qmap, (size_t)c.nat * sizeof(int)), "malloc SLICES site map", 204);
   m.rx_fwd = s.rx_f; m.rx_inv = s.rx_i; m.cy = s.cy; m.cz = s.cz;
   return 0;
 }
@@ -344,7 +343,6 @@ void slices_release(Context& c) {
     if (s.ev_done[i]) hipEventDestroy(s.ev_done[i]);
   }
   if (s.ev_in) hipEventDestroy(s.ev_in);
-  if (s.qmap) hipFree(s.qmap);
   s = SliceState{};
   if (c.backend == BACKEND_SLICES) {
     for (float2** p : {&c.mv.twx, &c.mv.twy, &c.mv.twz}) {
@@ -364,7 +362,7 @@ int slices_matvec(Context& c, int cwhat, const float2* x, float2* y, const float
   {
     dim3 grid((c.nx + s.cz - 1) / s.cz, c.ny, 3);
     const size_t lds = (size_t)(m.gz + s.cz * m.gz) * sizeof(float2);
-    hipLaunchKernelGGL(k_sl_zfwd, grid, dim3(SL_THREADS), lds, st, x, s.qmap, s.zwork, m.twz, m.pz, c.nx, c.ny,
+    hipLaunchKernelGGL(k_sl_zfwd, grid, dim3(SL_THREADS), lds, st, x, c.qmap, s.zwork, m.twz, m.pz, c.nx, c.ny,
                        c.nz, c.nat, c.nat0, compact, cj, lg2(s.cz));
     DDH_LAUNCH_OK("SLICES z FFT", 21);
   }
@@ -406,21 +404,11 @@ int slices_matvec(Context& c, int cwhat, const float2* x, float2* y, const float
     dim3 grid((c.nx + s.cz - 1) / s.cz, c.ny);
     const size_t lds = (size_t)(m.gz + 3 * s.cz * m.gz) * sizeof(float2);
     hipLaunchKernelGGL(k_sl_zinv_finish, grid, dim3(SL_THREADS), lds, st, s.zwork, m.twz, m.pz, x, y, adia, aoff,
-                       compact ? nullptr : c.iocc, s.qmap, c.nx, c.ny, c.nz, c.nat, c.nat0, compact, cj,
+                       compact ? nullptr : c.iocc, c.qmap, c.nx, c.ny, c.nz, c.nat, c.nat0, compact, cj,
                        1.f / (float)m.ngrid, lg2(s.cz));
     DDH_LAUNCH_OK("SLICES z inverse FFT + finish", 29);
   }
   c.matvec_count++;
-  return 0;
-}
-
-// called from finish_prepare once the occupied-site list is on the device
-int slices_build_qmap(Context& c) {
-  SliceState& s = g_sl;
-  if (!s.qmap) return 0;
-  DDH_TRY(hipMemsetAsync(s.qmap, 0xff, (size_t)c.nat * sizeof(int), c.stream), "SLICES site map", 204);
-  hipLaunchKernelGGL(k_sl_qmap, dim3((c.nat0 + 255) / 256), dim3(256), 0, c.stream, c.occ, s.qmap, c.nat0);
-  DDH_LAUNCH_OK("SLICES site map", 204);
   return 0;
 }
 

@@ -72,7 +72,6 @@ int scatter_full(Context& c, const float2* compact, float2* full, hipStream_t st
 
 // SLICES backend (low-memory MATVEC)
 int slices_setup(Context& c);
-int slices_build_qmap(Context& c);
 void slices_release(Context& c);
 int slices_matvec(Context& c, int cwhat, const float2* x, float2* y, const float2* adia,
                   const float2* aoff, int layout, hipStream_t st);

@@ -36,6 +36,10 @@ struct Context {
   int16_t* iocc = nullptr;     // [nat]
   int* occ = nullptr;          // [nat0] occupied site j, increasing
   int* rowptr = nullptr;       // [ny*nz+1] compact range of each (iy,iz) row
+  int* rowlo = nullptr;        // [ny*nz] first occupied ix of each row
+  int* qmap = nullptr;         // [nat] full site -> compact index (-1 void); built when rows are not intervals
+  int rows_contiguous = 0;     // every row's occupied sites form one interval
+  int fft_generic = 0;         // 1: force the LDS-staged generic FFT kernels
   // full-layout operator and vector staging
   float2 *adia = nullptr, *aoff = nullptr;      // [3*nat]
   float2 *adia_c = nullptr, *aoff_c = nullptr;  // [3*nat0]
@@ -63,6 +67,9 @@ int matvec_launch(Context& c, int cwhat, const float2* x, float2* y, const float
                   const float2* aoff, int layout, hipStream_t st, hipEvent_t* ev = nullptr);
 int matvec_setup(Context& c);
 void matvec_release(Context& c);
+// register-FFT pass `which` (1..5) when its axis length is covered; 1 = launched
+int reg_pass(Context& c, int which, int cj, int compact, const float2* x, float2* y, const float2* adia,
+             const float2* aoff, hipStream_t st);
 
 // Green tensor build on device (IPBC=0), result in c.green.
 int green_build(Context& c, int idipint, double akd, const double dx[3]);

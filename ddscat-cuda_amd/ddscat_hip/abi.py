@@ -28,7 +28,7 @@ REFERENCE_SYMBOLS = (
 EXTENSION_SYMBOLS = [
     "ddscat_hip_set_backend", "ddscat_hip_set_precision", "ddscat_hip_apply_device",
     "ddscat_hip_get_green", "ddscat_hip_info", "ddscat_hip_stream", "ddscat_hip_sync",
-    "ddscat_hip_time_matvec",
+    "ddscat_hip_time_matvec", "ddscat_hip_set_fft_kernels",
 ]
 INFO_FIELDS = ("nx", "ny", "nz", "nat", "nat0", "green_count", "matvec_count", "backend", "precision",
                "gx", "gy", "gz", "rx_fwd", "cy", "cz", "rx_inv", "green_build_us")
@@ -104,6 +104,8 @@ class Library:
         L.ddscat_hip_sync.restype = i
         L.ddscat_hip_time_matvec.argtypes = [i, vp, vp, vp, vp, i, i, vp]
         L.ddscat_hip_time_matvec.restype = i
+        L.ddscat_hip_set_fft_kernels.argtypes = [i]
+        L.ddscat_hip_set_fft_kernels.restype = i
         self.nat = self.nat0 = 0
 
     # ---------------------------------------------------------------- errors
@@ -217,6 +219,10 @@ class Library:
 
     def set_precision(self, prec):
         return self._check("set_precision", self.lib.ddscat_hip_set_precision(int(prec)))
+
+    def set_fft_kernels(self, generic):
+        """0: register-resident FFTs where available (default); 1: LDS-staged everywhere."""
+        return self._check("set_fft_kernels", self.lib.ddscat_hip_set_fft_kernels(int(generic)))
 
     def apply_device(self, cwhat, x_ptr, y_ptr, adia_ptr, aoff_ptr, layout=0, stream=None):
         cw = ord(cwhat) if isinstance(cwhat, str) else int(cwhat)

@@ -112,6 +112,11 @@ DDSCAT_HIP_API int ddscat_hip_set_backend(int backend);
  * FP32 = plain single precision throughout. */
 DDSCAT_HIP_API int ddscat_hip_set_precision(int precision);
 
+/* FFT kernel family: 0 = register-resident two-level FFTs where the padded
+ * axis length allows (64, 128, 256), LDS-staged Stockham otherwise
+ * (default); 1 = LDS-staged Stockham for every axis. */
+DDSCAT_HIP_API int ddscat_hip_set_fft_kernels(int generic);
+
 /* y = A x on DEVICE pointers, stream-ordered on `stream` (NULL = library
  * stream), no synchronisation.  layout 0: full [3][NAT] vectors and
  * operators; layout 1: compact [3][NAT0] vectors and compact operators. */

@@ -139,3 +139,61 @@ def test_apply_before_prepare_and_bad_args(lib):
     x = np.zeros(24, np.complex64)
     rc = lib.lib.ddscat_cuda_apply_f32(ord("X"), x.ctypes.data, x.ctypes.data, x.ctypes.data, x.ctypes.data)
     assert rc == 11
+
+
+@pytest.mark.parametrize("dims", [(32, 64, 128), (32, 20, 64), (64, 8, 32), (128, 4, 6)])
+@pytest.mark.parametrize("cwhat", ["N", "C"])
+def test_register_fft_paths_match_oracle(lib, dims, cwhat):
+    """Axes of padded length 64/128/256 run the register-resident FFT kernels."""
+    nx, ny, nz = dims
+    rng = np.random.default_rng(nx + 7 * ny + 13 * nz + (cwhat == "C"))
+    iocc, x, adia, aoff = random_case(rng, nx, ny, nz)
+    akd = 0.21
+    lib.prepare_green(nx, ny, nz, iocc, (0.0, 0.0, akd))
+    y = lib.apply(cwhat, x, adia, aoff)
+    goct = O.green_octant(nx, ny, nz, np.float64(np.float32(akd)))
+    yref = O.matvec_fft(cwhat, x, adia, aoff, goct, nx, ny, nz, iocc)
+    err = rel_l2(y, yref)
+    print(f"register path {dims} {cwhat}: rel L2 = {err:.3e}")
+    assert err < MATVEC_TOL
+
+
+@pytest.mark.parametrize("target", ["sphere", "random"])
+def test_register_and_generic_kernels_agree_on_compact_vectors(lib, target):
+    """Both FFT kernel families, compact device layout; sphere rows are
+    intervals (index-free addressing), random rows use the site map."""
+    import torch
+    if target == "sphere":
+        tg = T.sphere(32)
+        iocc = tg.iocc
+        nx = ny = nz = 32
+    else:
+        nx, ny, nz = 32, 16, 64
+        iocc = (np.random.default_rng(4).random(nx * ny * nz) < 0.6).astype(np.int16)
+    occ = np.flatnonzero(iocc)
+    rng = np.random.default_rng(12)
+    n0 = 3 * occ.size
+    cx = lambda s=1.0: (s * (rng.normal(size=n0) + 1j * rng.normal(size=n0))).astype(np.complex64)
+    x, ad, ao = cx(), cx(), cx(0.2)
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a).view(np.float32)).cuda()
+    outs = []
+    for generic in (0, 1):
+        lib.set_fft_kernels(generic)
+        lib.prepare_green(nx, ny, nz, iocc, (0.3, 0.1, 0.0))
+        xd, ad_d, ao_d = dev(x), dev(ad), dev(ao)
+        yd = torch.empty_like(xd)
+        torch.cuda.synchronize()
+        lib.apply_device("N", xd.data_ptr(), yd.data_ptr(), ad_d.data_ptr(), ao_d.data_ptr(), layout=1)
+        lib.sync()
+        outs.append(yd.cpu().numpy().view(np.complex64).astype(np.complex128))
+    lib.set_fft_kernels(0)
+    xf = np.zeros(3 * nx * ny * nz, complex)
+    xf.reshape(3, -1)[:, occ] = x.reshape(3, -1)
+    adf = np.zeros_like(xf); adf.reshape(3, -1)[:, occ] = ad.reshape(3, -1)
+    aof = np.zeros_like(xf); aof.reshape(3, -1)[:, occ] = ao.reshape(3, -1)
+    kd = float(np.linalg.norm(np.float32([0.3, 0.1, 0.0])))
+    yref = O.matvec_fft("N", xf, adf, aof, O.green_octant(nx, ny, nz, kd), nx, ny, nz, iocc)
+    yref = yref.reshape(3, -1)[:, occ].reshape(-1)
+    for y in outs:
+        assert rel_l2(y, yref) < MATVEC_TOL
+    assert rel_l2(outs[0], outs[1]) < 1e-6
