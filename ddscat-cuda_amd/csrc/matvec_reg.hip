@@ -1,5 +1,5 @@
 // FFT3D MATVEC passes built on the register-resident two-level FFT
-// (fft_reg.hpp) for padded axis lengths 2N = R1 * R2 in {64, 128, 256, 512}.
+// (fft_reg.hpp) for padded axis lengths 2N = R1 * R2 in {64, 128, 256}.
 // Same five passes, buffers and operator as matvec.hip; the difference is
 // that a transform keeps its data in registers, crosses LDS once, and pass 3
 // does the 3x3 Green multiply in registers between its forward and inverse
@@ -136,7 +136,15 @@ __global__ __launch_bounds__(RT) void kr_pass3(float2* __restrict__ a2, const fl
   __shared__ float2 twL[L];
   __shared__ float2 buf[L * P];
   const int s = threadIdx.x % C, t = threadIdx.x / C;
-  const int kx = blockIdx.x * C + s, ky = blockIdx.y;
+  // Grid order puts Green-sharing blocks on one XCD (workgroups go round-robin
+  // over the 8 XCDs by linear id): ky runs 0, ny, 1, gy-1, 2, gy-2, ... so
+  // mirror rows are one grid row apart, and column tile b and its mirror
+  // G-1-b are h = ceil(G/2) grid columns apart.
+  const int j = blockIdx.y;
+  const int ky = j == 0 ? 0 : j == 1 ? ny : ((j & 1) ? gy - ((j - 2) / 2 + 1) : (j - 2) / 2 + 1);
+  const int G = gridDim.x, h = (G + 1) / 2;
+  const int tile = (int)blockIdx.x < h ? (int)blockIdx.x : G - 1 - ((int)blockIdx.x - h);
+  const int kx = tile * C + s;
   for (int i = threadIdx.x; i < L; i += RT) twL[i] = twg[i];
   const bool live = kx < gx;
   const size_t plane = (size_t)gy * gx, cs = (size_t)nz * plane;
@@ -157,7 +165,10 @@ __global__ __launch_bounds__(RT) void kr_pass3(float2* __restrict__ a2, const fl
   fft2_fwd<R1, R2, +1>(v1, buf, twL, t, s, P);
   __syncthreads();
   fft2_fwd<R1, R2, +1>(v2, buf, twL, t, s, P);
-  // 3x3 symmetric Green multiply with octant parity reconstruction (cu:109-115)
+  // 3x3 symmetric Green multiply with octant parity reconstruction (cu:109-115).
+  // Each octant entry serves up to eight (kx, ky, kz) mirrors; kz mirrors
+  // are in this block and the kx / ky mirror blocks run on the same XCD at
+  // about the same time (grid mapping above), so repeats hit in L2.
   const int ox = nx + 1, oy = ny + 1;
   const size_t st = (size_t)ox * oy * (nz + 1);
   const int ir = live ? min(kx, gx - kx) : 0;

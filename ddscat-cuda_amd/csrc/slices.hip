@@ -24,7 +24,6 @@ static constexpr int SL_PERW = 16;  // single-component kernels
 struct SliceState {
   float2* zwork = nullptr;
   float2* slice[2] = {nullptr, nullptr};
-  int* qmap = nullptr;  // full site -> compact index or -1
   hipStream_t side[2] = {nullptr, nullptr};
   hipEvent_t ev_in = nullptr, ev_done[2] = {nullptr, nullptr};
   int batch = 4;
@@ -264,11 +263,6 @@ __global__ __launch_bounds__(SL_THREADS) void k_sl_zinv_finish(
     if (cj) { r1.y = -r1.y; r2.y = -r2.y; r3.y = -r3.y; }
     y[o] = r1; y[o + stride] = r2; y[o + 2 * stride] = r3;
   }
-}
-
-__global__ void k_sl_qmap(const int* __restrict__ occ, int* __restrict__ qmap, int nat0) {
-  const int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q < nat0) qmap[occ[q]] = q;
 }
 
 // ------------------------------------------------------------------ host --
