@@ -58,6 +58,85 @@ def setup_workload(lib, edge=128, m=1.5 + 0.1j, xsize=10.0, seed=20261202):
     return tg, kd, bufs, (x, adia, aoff)
 
 
+def time_to_qext(lib, tg, kd, m=1.5 + 0.1j, method="qmrccg", tol=1e-5, maxit=1000, graph=True):
+    """Time to Q_ext for incidence +x with both polarizations (E || y, E || z;
+    per-polarization LDR operators), everything resident in HBM:
+      plain_fp32   - one single-RHS solve per polarization, FP32 reductions
+                     and updates (PREC_FP32), then EVALQ;
+      mixed        - the same with FP64 reductions / scalars (PREC_MIXED);
+      batched_mixed- both polarizations in one batched solve (one HIP graph
+                     replay per iteration), then the batched EVALQ.
+    Wall time from the first solve call to all Q values on the host."""
+    import numpy as np
+    import torch
+    from ddscat_hip import PREC_MIXED, PREC_FP32
+    from ddscat_hip import targets as T
+    khat = np.array([1.0, 0.0, 0.0])
+    pols = [np.array([0, 1, 0], complex), np.array([0, 0, 1], complex)]
+    occ = tg.occ
+    comp = lambda v: np.ascontiguousarray(np.asarray(v).reshape(3, -1)[:, occ].reshape(-1)).astype(np.complex64)
+    bs, ads, aos = [], [], []
+    for e0 in pols:
+        ad, ao = T.operators(tg, m, kd, khat, e0)
+        bs.append(T_incident_compact(tg, khat * kd, e0))
+        ads.append(comp(ad)); aos.append(comp(ao))
+    # flat float32 views of [B][3 NAT0] complex64 arrays; RHS r is row(t, r)
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a).reshape(-1).view(np.float32)).cuda()
+    B = len(pols)
+    bd, add, aod = dev(np.stack(bs)), dev(np.stack(ads)), dev(np.stack(aos))
+    xd = torch.zeros_like(bd)
+    n2 = bd.numel() // B
+    assert bd.dim() == 1 and n2 == 2 * 3 * tg.nat0
+    row = lambda t, r: t[r * n2:(r + 1) * n2]
+    geo = math.pi * (3.0 * tg.nat0 / (4.0 * math.pi)) ** (2.0 / 3.0)
+    ak = (khat * kd).astype(np.float32)
+    # untimed one-iteration solves: allocate the Krylov workspaces
+    lib.solve_device(method, row(bd, 0).data_ptr(), row(xd, 0).data_ptr(), row(add, 0).data_ptr(),
+                     row(aod, 0).data_ptr(), maxit=1, tol=tol, check=False)
+    lib.solve_batch(method, B, bd.data_ptr(), xd.data_ptr(), add.data_ptr(), aod.data_ptr(), maxit=1, tol=tol,
+                    graph=graph, check=False)
+    out = {}
+    for mode, prec in (("plain_fp32", PREC_FP32), ("mixed", PREC_MIXED)):
+        lib.set_precision(prec)
+        xd.zero_()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        per = []
+        for r in range(B):
+            it, tole, nc, rc = lib.solve_device(method, row(bd, r).data_ptr(), row(xd, r).data_ptr(),
+                                                row(add, r).data_ptr(), row(aod, r).data_ptr(), maxit=maxit,
+                                                tol=tol, check=False)
+            cabs, cext, _, rq = lib.evalq(ak, check=False)
+            per.append(dict(iterations=it, matvecs=nc, resid=float(tole), rc=rc or rq, qext=cext / geo,
+                            qabs=cabs / geo))
+        dt = time.perf_counter() - t0
+        out[mode] = dict(seconds=round(dt, 4), rhs=per)
+    lib.set_precision(PREC_MIXED)
+    xd.zero_()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    it, te, nc, rc = lib.solve_batch(method, B, bd.data_ptr(), xd.data_ptr(), add.data_ptr(), aod.data_ptr(),
+                                     maxit=maxit, tol=tol, graph=graph, check=False)
+    q = lib.evalq_batch(B, bd.data_ptr(), xd.data_ptr(), add.data_ptr(), aod.data_ptr(), ak, check=False)
+    dt = time.perf_counter() - t0
+    out["batched_mixed"] = dict(seconds=round(dt, 4), graph=graph, rc=rc,
+                                rhs=[dict(iterations=int(it[r]), matvecs=int(nc[r]), resid=float(te[r]),
+                                          qext=q[r, 1] / geo, qabs=q[r, 0] / geo) for r in range(B)])
+    f, bm = out["plain_fp32"], out["batched_mixed"]
+    qdiff = max(abs(a["qext"] - b["qext"]) / abs(b["qext"]) for a, b in zip(f["rhs"], bm["rhs"]))
+    return dict(method=method, tol=tol, incidence="+x, E||y and E||z", **out,
+                speedup_batched_mixed_vs_plain_fp32=round(f["seconds"] / bm["seconds"], 3),
+                speedup_mixed_vs_plain_fp32=round(f["seconds"] / out["mixed"]["seconds"], 3),
+                qext_max_rel_diff_fp32_vs_mixed=qdiff)
+
+
+def T_incident_compact(tg, kvec, e0):
+    import numpy as np
+    from ddscat_hip import targets as T
+    e = T.incident_field(tg, kvec, e0, tg.centroid_x0())
+    return np.ascontiguousarray(e.reshape(3, -1)[:, tg.occ].reshape(-1))
+
+
 def cpu_baseline(tg, kd, host, budget_s=20.0):
     """Oracle FP32 (complex64) FFT MATVEC on all host cores (scipy.fft workers)."""
     import numpy as np
@@ -114,6 +193,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--edge", type=int, default=128)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-solve", action="store_true", help="skip the time-to-Q_ext leg")
+    ap.add_argument("--solver", default="qmrccg")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--backend", choices=["fft3d", "slices"], default="fft3d")
     ap.add_argument("--pmc-calib", action="store_true", help="run one torch copy of known size (rocprof scale)")
@@ -176,6 +257,10 @@ def main():
     roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None, kernel=dom_name)
 
+    solve = None
+    if not args.no_solve:
+        solve = time_to_qext(lib, tg, kd, method=args.solver)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(tg, kd, host, budget_s=args.cpu_budget)
@@ -201,6 +286,7 @@ def main():
         "matvec_gbs": round(total_bytes / (ms_step * 1e-3) / 1e9, 1),
         "matvec_bytes": total_bytes,
         "passes": per_pass,
+        "time_to_qext": solve,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

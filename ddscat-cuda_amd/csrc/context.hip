@@ -50,6 +50,7 @@ void release_context(Context& c) {
   if (c.stream) hipStreamSynchronize(c.stream);
   matvec_release(c);
   slices_release(c);
+  batch_release();
   dfree(c.green);
   dfree(c.iocc); dfree(c.occ); dfree(c.rowptr); dfree(c.rowlo); dfree(c.qmap);
   dfree(c.adia); dfree(c.aoff); dfree(c.adia_c); dfree(c.aoff_c);

@@ -33,8 +33,12 @@ template <int PER>
 __global__ __launch_bounds__(MV_THREADS) void k_pass1_xfwd(
     const float2* __restrict__ x, float2* __restrict__ a1, const int* __restrict__ rowptr,
     const int* __restrict__ occ, const float2* __restrict__ twg, FftPlan p, int nx, int nrows,
-    int rx, int nat, int nat0, int compact, int conj_in) {
+    int rx, int nat, int nat0, int compact, int conj_in, BatchArgs ba) {
   extern __shared__ float2 sm[];
+  const int rb = blockIdx.z;
+  if (ba.active && !ba.active[rb]) return;
+  x += rb * ba.xs;
+  a1 += rb * ba.a1s;
   const int L = p.L;
   float2* tw = sm;
   float2* buf = sm + L;
@@ -79,13 +83,17 @@ __global__ __launch_bounds__(MV_THREADS) void k_pass1_xfwd(
 template <int DIR, int PER>
 __global__ __launch_bounds__(MV_THREADS) void k_pass_y(
     const float2* __restrict__ in, float2* __restrict__ out, const float2* __restrict__ twg,
-    FftPlan p, int gx, int nin, int nout, int nz, int cy_log2) {
+    FftPlan p, int gx, int nin, int nout, int nz, int cy_log2, BatchArgs ba) {
   extern __shared__ float2 sm[];
+  const int rb = blockIdx.z / 3;
+  if (ba.active && !ba.active[rb]) return;
+  in += rb * (DIR > 0 ? ba.a1s : ba.a2s);
+  out += rb * (DIR > 0 ? ba.a2s : ba.a1s);
   const int L = p.L;
   const int cy = 1 << cy_log2;
   float2* tw = sm;
   float2* buf = sm + L;
-  const int c = blockIdx.z, iz = blockIdx.y, kx0 = blockIdx.x * cy;
+  const int c = blockIdx.z % 3, iz = blockIdx.y, kx0 = blockIdx.x * cy;
   const int T = blockDim.x;
   for (int i = threadIdx.x; i < L; i += T) tw[i] = twg[i];
   const float2* src = in + ((size_t)c * nz + iz) * (size_t)nin * gx;
@@ -110,8 +118,11 @@ __global__ __launch_bounds__(MV_THREADS) void k_pass_y(
 template <int PER>
 __global__ __launch_bounds__(MV_THREADS) void k_pass3_z_green(
     float2* __restrict__ a2, const float2* __restrict__ green, const float2* __restrict__ twg,
-    FftPlan p, int nx, int ny, int nz, int gx, int gy, int cz_log2) {
+    FftPlan p, int nx, int ny, int nz, int gx, int gy, int cz_log2, BatchArgs ba) {
   extern __shared__ float2 sm[];
+  const int rb = blockIdx.z;
+  if (ba.active && !ba.active[rb]) return;
+  a2 += rb * ba.a2s;
   const int L = p.L;  // = 2 nz
   const int cz = 1 << cz_log2;
   float2* tw = sm;
@@ -200,8 +211,15 @@ __global__ __launch_bounds__(MV_THREADS) void k_pass5_xinv_finish(
     const float2* __restrict__ x, float2* __restrict__ y, const float2* __restrict__ adia,
     const float2* __restrict__ aoff, const int16_t* __restrict__ iocc,
     const int* __restrict__ rowptr, const int* __restrict__ occ, int nx, int nrows, int rx,
-    int nat, int nat0, int compact, int cj, float inv) {
+    int nat, int nat0, int compact, int cj, float inv, BatchArgs ba) {
   extern __shared__ float2 sm[];
+  const int rb = blockIdx.y;
+  if (ba.active && !ba.active[rb]) return;
+  a1 += rb * ba.a1s;
+  x += rb * ba.xs;
+  y += rb * ba.xs;
+  adia += rb * ba.os;
+  aoff += rb * ba.os;
   const int L = p.L;
   float2* tw = sm;
   float2* b0 = sm + L;
@@ -328,6 +346,9 @@ int matvec_setup(Context& c) {
   return 0;
 }
 
+size_t matvec_a1_count(const Context& c) { return (size_t)3 * c.nz * c.ny * c.mv.gx; }
+size_t matvec_a2_count(const Context& c) { return (size_t)3 * c.nz * c.mv.gy * c.mv.gx; }
+
 void matvec_release(Context& c) {
   for (float2** p : {&c.mv.twx, &c.mv.twy, &c.mv.twz, &c.a1, &c.a2}) {
     if (*p) hipFree(*p);
@@ -336,56 +357,61 @@ void matvec_release(Context& c) {
 }
 
 int matvec_launch(Context& c, int cwhat, const float2* x, float2* y, const float2* adia,
-                  const float2* aoff, int layout, hipStream_t st, hipEvent_t* ev) {
+                  const float2* aoff, int layout, hipStream_t st, hipEvent_t* ev, const MvBatch* btp) {
   const MatvecPlan& m = c.mv;
   const int nrows = c.ny * c.nz;
   const int cj = (cwhat == 'C') ? 1 : 0;
   const int compact = layout ? 1 : 0;
+  MvBatch bt = btp ? *btp : MvBatch{};
+  if (!bt.a1) bt.a1 = c.a1;
+  if (!bt.a2) bt.a2 = c.a2;
+  const unsigned nb = (unsigned)bt.nrhs;
+  float2 *a1 = bt.a1, *a2 = bt.a2;
   if (ev) hipEventRecord(ev[0], st);
-  if (c.fft_generic || !reg_pass(c, 1, cj, compact, x, y, adia, aoff, st)) {
-    dim3 grid((nrows + m.rx_fwd - 1) / m.rx_fwd, 3);
+  if (c.fft_generic || !reg_pass(c, 1, cj, compact, x, y, adia, aoff, st, bt)) {
+    dim3 grid((nrows + m.rx_fwd - 1) / m.rx_fwd, 3, nb);
     size_t lds = (size_t)(m.gx + m.rx_fwd * m.gx) * sizeof(float2);
-    hipLaunchKernelGGL(k_pass1_xfwd<PER_WIDE>, grid, dim3(MV_THREADS), lds, st, x, c.a1, c.rowptr,
-                       c.occ, m.twx, m.px, c.nx, nrows, m.rx_fwd, c.nat, c.nat0, compact, cj);
+    hipLaunchKernelGGL(k_pass1_xfwd<PER_WIDE>, grid, dim3(MV_THREADS), lds, st, x, a1, c.rowptr,
+                       c.occ, m.twx, m.px, c.nx, nrows, m.rx_fwd, c.nat, c.nat0, compact, cj, bt.k);
     DDH_LAUNCH_OK("MATVEC pass 1 (x FFT)", 21);
   }
   if (ev) hipEventRecord(ev[1], st);
-  if (c.fft_generic || !reg_pass(c, 2, cj, compact, x, y, adia, aoff, st)) {
-    dim3 grid((m.gx + m.cy - 1) / m.cy, c.nz, 3);
+  if (c.fft_generic || !reg_pass(c, 2, cj, compact, x, y, adia, aoff, st, bt)) {
+    dim3 grid((m.gx + m.cy - 1) / m.cy, c.nz, 3 * nb);
     size_t lds = (size_t)(m.gy + m.cy * m.gy) * sizeof(float2);
-    hipLaunchKernelGGL((k_pass_y<+1, PER_WIDE>), grid, dim3(MV_THREADS), lds, st, c.a1, c.a2, m.twy,
-                       m.py, m.gx, c.ny, m.gy, c.nz, ilog2(m.cy));
+    hipLaunchKernelGGL((k_pass_y<+1, PER_WIDE>), grid, dim3(MV_THREADS), lds, st, a1, a2, m.twy,
+                       m.py, m.gx, c.ny, m.gy, c.nz, ilog2(m.cy), bt.k);
     DDH_LAUNCH_OK("MATVEC pass 2 (y FFT)", 22);
   }
   if (ev) hipEventRecord(ev[2], st);
-  if (c.fft_generic || !reg_pass(c, 3, cj, compact, x, y, adia, aoff, st)) {
-    dim3 grid((m.gx + m.cz - 1) / m.cz, m.gy);
+  if (c.fft_generic || !reg_pass(c, 3, cj, compact, x, y, adia, aoff, st, bt)) {
+    dim3 grid((m.gx + m.cz - 1) / m.cz, m.gy, nb);
     size_t lds = (size_t)(m.gz + 3 * m.cz * m.gz) * sizeof(float2);
-    hipLaunchKernelGGL(k_pass3_z_green<PER_NARROW>, grid, dim3(MV_THREADS), lds, st, c.a2, c.green,
-                       m.twz, m.pz, c.nx, c.ny, c.nz, m.gx, m.gy, ilog2(m.cz));
+    hipLaunchKernelGGL(k_pass3_z_green<PER_NARROW>, grid, dim3(MV_THREADS), lds, st, a2, c.green,
+                       m.twz, m.pz, c.nx, c.ny, c.nz, m.gx, m.gy, ilog2(m.cz), bt.k);
     DDH_LAUNCH_OK("MATVEC pass 3 (z FFT + Green)", 23);
   }
   if (ev) hipEventRecord(ev[3], st);
-  if (c.fft_generic || !reg_pass(c, 4, cj, compact, x, y, adia, aoff, st)) {
-    dim3 grid((m.gx + m.cy - 1) / m.cy, c.nz, 3);
+  if (c.fft_generic || !reg_pass(c, 4, cj, compact, x, y, adia, aoff, st, bt)) {
+    dim3 grid((m.gx + m.cy - 1) / m.cy, c.nz, 3 * nb);
     size_t lds = (size_t)(m.gy + m.cy * m.gy) * sizeof(float2);
-    hipLaunchKernelGGL((k_pass_y<-1, PER_WIDE>), grid, dim3(MV_THREADS), lds, st, c.a2, c.a1, m.twy,
-                       m.py, m.gx, m.gy, c.ny, c.nz, ilog2(m.cy));
+    hipLaunchKernelGGL((k_pass_y<-1, PER_WIDE>), grid, dim3(MV_THREADS), lds, st, a2, a1, m.twy,
+                       m.py, m.gx, m.gy, c.ny, c.nz, ilog2(m.cy), bt.k);
     DDH_LAUNCH_OK("MATVEC pass 4 (y inverse FFT)", 24);
   }
   if (ev) hipEventRecord(ev[4], st);
-  if (c.fft_generic || !reg_pass(c, 5, cj, compact, x, y, adia, aoff, st)) {
-    dim3 grid((nrows + m.rx_inv - 1) / m.rx_inv);
+  if (c.fft_generic || !reg_pass(c, 5, cj, compact, x, y, adia, aoff, st, bt)) {
+    dim3 grid((nrows + m.rx_inv - 1) / m.rx_inv, nb);
     size_t lds = (size_t)(m.gx + 3 * m.rx_inv * m.gx) * sizeof(float2);
     const float inv = 1.f / (float)m.ngrid;
-    hipLaunchKernelGGL(k_pass5_xinv_finish<PER_NARROW>, grid, dim3(MV_THREADS), lds, st, c.a1, m.twx,
+    hipLaunchKernelGGL(k_pass5_xinv_finish<PER_NARROW>, grid, dim3(MV_THREADS), lds, st, a1, m.twx,
                        m.px, x, y, adia, aoff, compact ? nullptr : c.iocc, c.rowptr, c.occ, c.nx,
-                       nrows, m.rx_inv, c.nat, c.nat0, compact, cj, inv);
+                       nrows, m.rx_inv, c.nat, c.nat0, compact, cj, inv, bt.k);
     DDH_LAUNCH_OK("MATVEC pass 5 (x inverse FFT + finish)", 25);
   }
   DDH_LAUNCH_OK("MATVEC register passes", 20);
   if (ev) hipEventRecord(ev[5], st);
-  c.matvec_count++;
+  c.matvec_count += nb;
   return 0;
 }
 

@@ -38,10 +38,14 @@ __device__ __forceinline__ int compact_index(const SiteMap& m, int g, int ix, in
 template <int R1, int R2>
 __global__ __launch_bounds__(RT) void kr_pass1(const float2* __restrict__ x, float2* __restrict__ a1, SiteMap sm,
                                                const float2* __restrict__ twg, int nx, int nrows, int nat, int nat0,
-                                               int compact, int cj) {
+                                               int compact, int cj, BatchArgs ba) {
   constexpr int L = R1 * R2, C = RT / R1, P = C + 1;
   __shared__ float2 twL[L];
   __shared__ float2 buf[L * P];
+  const int rb = blockIdx.z;
+  if (ba.active && !ba.active[rb]) return;
+  x += rb * ba.xs;
+  a1 += rb * ba.a1s;
   const int t = threadIdx.x % R1, s = threadIdx.x / R1;
   const int g = blockIdx.x * C + s, c = blockIdx.y;
   for (int i = threadIdx.x; i < L; i += RT) twL[i] = twg[i];
@@ -84,12 +88,17 @@ __global__ __launch_bounds__(RT) void kr_pass1(const float2* __restrict__ x, flo
 // in [3][nz][nin][gx] -> out [3][nz][nout][gx] along the middle axis
 template <int R1, int R2, int DIR>
 __global__ __launch_bounds__(RT) void kr_passy(const float2* __restrict__ in, float2* __restrict__ out,
-                                               const float2* __restrict__ twg, int gx, int nin, int nout, int nz) {
+                                               const float2* __restrict__ twg, int gx, int nin, int nout, int nz,
+                                               BatchArgs ba) {
   constexpr int L = R1 * R2, C = RT / R1, P = C + 1;
   __shared__ float2 twL[L];
   __shared__ float2 buf[L * P];
+  const int rb = blockIdx.z / 3;
+  if (ba.active && !ba.active[rb]) return;
+  in += rb * (DIR > 0 ? ba.a1s : ba.a2s);
+  out += rb * (DIR > 0 ? ba.a2s : ba.a1s);
   const int s = threadIdx.x % C, t = threadIdx.x / C;
-  const int kx = blockIdx.x * C + s, iz = blockIdx.y, c = blockIdx.z;
+  const int kx = blockIdx.x * C + s, iz = blockIdx.y, c = blockIdx.z % 3;
   for (int i = threadIdx.x; i < L; i += RT) twL[i] = twg[i];
   const bool live = kx < gx;
   const float2* src = in + ((size_t)c * nz + iz) * (size_t)nin * gx + kx;
@@ -131,10 +140,13 @@ __global__ __launch_bounds__(RT) void kr_passy(const float2* __restrict__ in, fl
 template <int R1, int R2>
 __global__ __launch_bounds__(RT) void kr_pass3(float2* __restrict__ a2, const float2* __restrict__ green,
                                                const float2* __restrict__ twg, int nx, int ny, int nz, int gx,
-                                               int gy) {
+                                               int gy, BatchArgs ba) {
   constexpr int L = R1 * R2, C = RT / R1, P = C + 1;
   __shared__ float2 twL[L];
   __shared__ float2 buf[L * P];
+  const int rb = blockIdx.z;
+  if (ba.active && !ba.active[rb]) return;
+  a2 += rb * ba.a2s;
   const int s = threadIdx.x % C, t = threadIdx.x / C;
   // Grid order puts Green-sharing blocks on one XCD (workgroups go round-robin
   // over the 8 XCDs by linear id): ky runs 0, ny, 1, gy-1, 2, gy-2, ... so
@@ -223,10 +235,17 @@ __global__ __launch_bounds__(RT) void kr_pass5(const float2* __restrict__ a1, co
                                                const float2* __restrict__ x, float2* __restrict__ y,
                                                const float2* __restrict__ adia, const float2* __restrict__ aoff,
                                                const int16_t* __restrict__ iocc, SiteMap sm, int nx, int nrows,
-                                               int nat, int nat0, int compact, int cj, float inv) {
+                                               int nat, int nat0, int compact, int cj, float inv, BatchArgs ba) {
   constexpr int L = R1 * R2, C = RT / R1, P = C + 1;
   __shared__ float2 twL[L];
   __shared__ float2 buf[L * P];
+  const int rb = blockIdx.y;
+  if (ba.active && !ba.active[rb]) return;
+  a1 += rb * ba.a1s;
+  x += rb * ba.xs;
+  y += rb * ba.xs;
+  adia += rb * ba.os;
+  aoff += rb * ba.os;
   const int t = threadIdx.x % R1, s = threadIdx.x / R1;
   const int g = blockIdx.x * C + s;
   for (int i = threadIdx.x; i < L; i += RT) twL[i] = twg[i];
@@ -296,35 +315,35 @@ __global__ __launch_bounds__(RT) void kr_pass5(const float2* __restrict__ a1, co
 // ------------------------------------------------------------------ host --
 template <int R1, int R2>
 static int launch_reg(Context& c, int cj, int compact, const float2* x, float2* y, const float2* adia,
-                      const float2* aoff, hipStream_t st, hipEvent_t* ev, int which) {
+                      const float2* aoff, hipStream_t st, const MvBatch& bt, int which) {
   const MatvecPlan& m = c.mv;
   constexpr int C = RT / R1;
   const int nrows = c.ny * c.nz;
+  const unsigned nb = (unsigned)bt.nrhs;
   SiteMap sm{c.rowptr, c.rowlo, c.qmap, c.rows_contiguous};
   switch (which) {
     case 1:
-      hipLaunchKernelGGL((kr_pass1<R1, R2>), dim3((nrows + C - 1) / C, 3), dim3(RT), 0, st, x, c.a1, sm, m.twx,
-                         c.nx, nrows, c.nat, c.nat0, compact, cj);
+      hipLaunchKernelGGL((kr_pass1<R1, R2>), dim3((nrows + C - 1) / C, 3, nb), dim3(RT), 0, st, x, bt.a1, sm,
+                         m.twx, c.nx, nrows, c.nat, c.nat0, compact, cj, bt.k);
       break;
     case 2:
-      hipLaunchKernelGGL((kr_passy<R1, R2, +1>), dim3((m.gx + C - 1) / C, c.nz, 3), dim3(RT), 0, st, c.a1, c.a2,
-                         m.twy, m.gx, c.ny, m.gy, c.nz);
+      hipLaunchKernelGGL((kr_passy<R1, R2, +1>), dim3((m.gx + C - 1) / C, c.nz, 3 * nb), dim3(RT), 0, st, bt.a1,
+                         bt.a2, m.twy, m.gx, c.ny, m.gy, c.nz, bt.k);
       break;
     case 3:
-      hipLaunchKernelGGL((kr_pass3<R1, R2>), dim3((m.gx + C - 1) / C, m.gy), dim3(RT), 0, st, c.a2, c.green, m.twz,
-                         c.nx, c.ny, c.nz, m.gx, m.gy);
+      hipLaunchKernelGGL((kr_pass3<R1, R2>), dim3((m.gx + C - 1) / C, m.gy, nb), dim3(RT), 0, st, bt.a2, c.green,
+                         m.twz, c.nx, c.ny, c.nz, m.gx, m.gy, bt.k);
       break;
     case 4:
-      hipLaunchKernelGGL((kr_passy<R1, R2, -1>), dim3((m.gx + C - 1) / C, c.nz, 3), dim3(RT), 0, st, c.a2, c.a1,
-                         m.twy, m.gx, m.gy, c.ny, c.nz);
+      hipLaunchKernelGGL((kr_passy<R1, R2, -1>), dim3((m.gx + C - 1) / C, c.nz, 3 * nb), dim3(RT), 0, st, bt.a2,
+                         bt.a1, m.twy, m.gx, m.gy, c.ny, c.nz, bt.k);
       break;
     default:
-      hipLaunchKernelGGL((kr_pass5<R1, R2>), dim3((nrows + C - 1) / C), dim3(RT), 0, st, c.a1, m.twx, x, y, adia,
-                         aoff, compact ? nullptr : c.iocc, sm, c.nx, nrows, c.nat, c.nat0, compact, cj,
-                         1.f / (float)m.ngrid);
+      hipLaunchKernelGGL((kr_pass5<R1, R2>), dim3((nrows + C - 1) / C, nb), dim3(RT), 0, st, bt.a1, m.twx, x, y,
+                         adia, aoff, compact ? nullptr : c.iocc, sm, c.nx, nrows, c.nat, c.nat0, compact, cj,
+                         1.f / (float)m.ngrid, bt.k);
       break;
   }
-  (void)ev;
   return 0;
 }
 
@@ -338,22 +357,18 @@ int reg_shape(int L) {
   }
 }
 
-template <int R1, int R2>
-static int pass_dispatch_t(Context& c, int which, int cj, int compact, const float2* x, float2* y, const float2* adia,
-                           const float2* aoff, hipStream_t st) {
-  return launch_reg<R1, R2>(c, cj, compact, x, y, adia, aoff, st, nullptr, which);
-}
+
 
 // launches register pass `which` (1..5) if its axis length is covered; returns
 // 1 when launched, 0 when the caller should use the generic kernel
 int reg_pass(Context& c, int which, int cj, int compact, const float2* x, float2* y, const float2* adia,
-             const float2* aoff, hipStream_t st) {
+             const float2* aoff, hipStream_t st, const MvBatch& bt) {
   const MatvecPlan& m = c.mv;
   const int L = (which == 1 || which == 5) ? m.gx : (which == 3 ? m.gz : m.gy);
   switch (reg_shape(L)) {
-    case 1: pass_dispatch_t<8, 8>(c, which, cj, compact, x, y, adia, aoff, st); return 1;
-    case 2: pass_dispatch_t<8, 16>(c, which, cj, compact, x, y, adia, aoff, st); return 1;
-    case 3: pass_dispatch_t<16, 16>(c, which, cj, compact, x, y, adia, aoff, st); return 1;
+    case 1: launch_reg<8, 8>(c, cj, compact, x, y, adia, aoff, st, bt, which); return 1;
+    case 2: launch_reg<8, 16>(c, cj, compact, x, y, adia, aoff, st, bt, which); return 1;
+    case 3: launch_reg<16, 16>(c, cj, compact, x, y, adia, aoff, st, bt, which); return 1;
     default: return 0;
   }
 }

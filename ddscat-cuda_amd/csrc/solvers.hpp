@@ -28,6 +28,9 @@ struct SolverArgs {
   int* itno;
   float* tole;
   int* ncompte;
+  // 0: reference boundary (full [3][NAT] host vectors, ndim = 3 NAT);
+  // 1: compact [3][NAT0] device vectors on the library stream, ndim = 3 NAT0
+  int device = 0;
 };
 
 struct Reliable {

@@ -60,16 +60,35 @@ void release_context(Context& c);
 int  ensure_solver_workspace(Context& c, int nvec);
 void release_solver_workspace(Context& c);
 
+// Several right-hand sides through one MATVEC launch sequence: RHS r uses
+// x + r*xs, y + r*xs, adia + r*os, aoff + r*os (os = 0: shared operators) and
+// its own FFT work a1 + r*a1s, a2 + r*a2s; RHS with active[r] == 0 are
+// skipped (every workgroup of theirs exits at once).  Kernel-side view:
+struct BatchArgs {
+  size_t xs = 0, os = 0, a1s = 0, a2s = 0;
+  const int* active = nullptr;
+};
+struct MvBatch {
+  int nrhs = 1;
+  BatchArgs k;
+  float2* a1 = nullptr;  // nullptr: the context's single-RHS buffers
+  float2* a2 = nullptr;
+};
+
 // MATVEC: y = A x.  layout 0: full [3][nat] vectors (iocc mask applied);
 // layout 1: compact [3][nat0] vectors (adia_c/aoff_c).  conj_mode: 'N' or 'C'.
 // ev (optional): 6 events recorded before passes 1..5 and after pass 5.
 int matvec_launch(Context& c, int cwhat, const float2* x, float2* y, const float2* adia,
-                  const float2* aoff, int layout, hipStream_t st, hipEvent_t* ev = nullptr);
+                  const float2* aoff, int layout, hipStream_t st, hipEvent_t* ev = nullptr,
+                  const MvBatch* bt = nullptr);
 int matvec_setup(Context& c);
 void matvec_release(Context& c);
+// FFT work elements per RHS: a1 [3][nz][ny][gx], a2 [3][nz][gy][gx]
+size_t matvec_a1_count(const Context& c);
+size_t matvec_a2_count(const Context& c);
 // register-FFT pass `which` (1..5) when its axis length is covered; 1 = launched
 int reg_pass(Context& c, int which, int cj, int compact, const float2* x, float2* y, const float2* adia,
-             const float2* aoff, hipStream_t st);
+             const float2* aoff, hipStream_t st, const MvBatch& bt);
 
 // Green tensor build on device (IPBC=0), result in c.green.
 int green_build(Context& c, int idipint, double akd, const double dx[3]);

@@ -28,8 +28,13 @@ REFERENCE_SYMBOLS = (
 EXTENSION_SYMBOLS = [
     "ddscat_hip_set_backend", "ddscat_hip_set_precision", "ddscat_hip_apply_device",
     "ddscat_hip_get_green", "ddscat_hip_info", "ddscat_hip_stream", "ddscat_hip_sync",
-    "ddscat_hip_time_matvec", "ddscat_hip_set_fft_kernels",
+    "ddscat_hip_time_matvec", "ddscat_hip_set_fft_kernels", "ddscat_hip_solve_device",
+    "ddscat_hip_solve_batch", "ddscat_hip_apply_batch", "ddscat_hip_evalq_batch",
+    "ddscat_hip_cabs_adjoint",
 ]
+# method ids of ddscat_hip_solve_device
+SOLVER_IDS = {"pbcgs2": 0, "qmrccg": 1, "petrkp": 2, "gpbicg": 3, "pbcgst": 4, "bicgstab4": 5,
+              "gpbicgstab2": 6, "gpbicgstab4": 7}
 INFO_FIELDS = ("nx", "ny", "nz", "nat", "nat0", "green_count", "matvec_count", "backend", "precision",
                "gx", "gy", "gz", "rx_fwd", "cy", "cz", "rx_inv", "green_build_us")
 
@@ -106,6 +111,16 @@ class Library:
         L.ddscat_hip_time_matvec.restype = i
         L.ddscat_hip_set_fft_kernels.argtypes = [i]
         L.ddscat_hip_set_fft_kernels.restype = i
+        L.ddscat_hip_solve_device.argtypes = [i, vp, vp, vp, vp, i, i, f, vp, vp, vp]
+        L.ddscat_hip_solve_device.restype = i
+        L.ddscat_hip_solve_batch.argtypes = [i, i, vp, vp, vp, vp, i, i, f, vp, vp, vp, i]
+        L.ddscat_hip_solve_batch.restype = i
+        L.ddscat_hip_apply_batch.argtypes = [i, i, vp, vp, vp, vp, i, vp]
+        L.ddscat_hip_apply_batch.restype = i
+        L.ddscat_hip_evalq_batch.argtypes = [i, vp, vp, vp, vp, i, vp, f, vp]
+        L.ddscat_hip_evalq_batch.restype = i
+        L.ddscat_hip_cabs_adjoint.argtypes = [i, i, vp, vp, vp, i, vp, f, i, f, vp, vp, vp, vp]
+        L.ddscat_hip_cabs_adjoint.restype = i
         self.nat = self.nat0 = 0
 
     # ---------------------------------------------------------------- errors
@@ -166,6 +181,62 @@ class Library:
         if check:
             self._check(f"solve_{name}", rc)
         return x, it.value, te.value, nc.value, rc
+
+    def solve_device(self, name, b_ptr, x_ptr, adia_ptr, aoff_ptr, maxit=300, tol=1e-5, check=True):
+        """Solver on compact [3][NAT0] device vectors (x in/out).  Returns (itno, tole, ncompte, rc)."""
+        it, nc = C.c_int(0), C.c_int(0)
+        te = C.c_float(0.0)
+        rc = self.lib.ddscat_hip_solve_device(SOLVER_IDS[name], C.c_void_p(b_ptr), C.c_void_p(x_ptr),
+                                              C.c_void_p(adia_ptr), C.c_void_p(aoff_ptr), 3 * self.nat0,
+                                              int(maxit), float(tol), C.byref(it), C.byref(te), C.byref(nc))
+        if check:
+            self._check(f"solve_device_{name}", rc)
+        return it.value, te.value, nc.value, rc
+
+    def apply_batch(self, cwhat, nrhs, x_ptr, y_ptr, adia_ptr, aoff_ptr, op_per_rhs=True, stream=None):
+        cw = ord(cwhat) if isinstance(cwhat, str) else int(cwhat)
+        return self._check("apply_batch", self.lib.ddscat_hip_apply_batch(
+            cw, int(nrhs), C.c_void_p(x_ptr), C.c_void_p(y_ptr), C.c_void_p(adia_ptr), C.c_void_p(aoff_ptr),
+            1 if op_per_rhs else 0, C.c_void_p(stream) if stream else None))
+
+    def evalq_batch(self, nrhs, e_ptr, p_ptr, adia_ptr, aoff_ptr, ak, e02=1.0, op_per_rhs=True, check=True):
+        """Cabs, Cext, Cpha per RHS ([nrhs][3] float64) from device vectors."""
+        ak = np.ascontiguousarray(np.broadcast_to(np.asarray(ak, np.float32).reshape(-1, 3), (nrhs, 3)))
+        out = np.zeros((nrhs, 3), np.float64)
+        rc = self.lib.ddscat_hip_evalq_batch(int(nrhs), C.c_void_p(e_ptr), C.c_void_p(p_ptr), C.c_void_p(adia_ptr),
+                                             C.c_void_p(aoff_ptr), 1 if op_per_rhs else 0, _p(ak), float(e02),
+                                             _p(out))
+        if check:
+            self._check("evalq_batch", rc)
+        return out
+
+    def cabs_adjoint(self, name, nrhs, p_ptr, adia_ptr, aoff_ptr, ak, g_adia_ptr, g_aoff_ptr=None, e02=1.0,
+                     op_per_rhs=True, maxit=300, tol=1e-5, check=True):
+        """Writes dCabs/d(adia) (and d(aoff)) as complex64 device arrays; returns (itno, tole, rc)."""
+        ak = np.ascontiguousarray(np.broadcast_to(np.asarray(ak, np.float32).reshape(-1, 3), (nrhs, 3)))
+        it = np.zeros(nrhs, np.int32)
+        te = np.zeros(nrhs, np.float32)
+        rc = self.lib.ddscat_hip_cabs_adjoint(SOLVER_IDS[name], int(nrhs), C.c_void_p(p_ptr), C.c_void_p(adia_ptr),
+                                              C.c_void_p(aoff_ptr), 1 if op_per_rhs else 0, _p(ak), float(e02),
+                                              int(maxit), float(tol), C.c_void_p(g_adia_ptr),
+                                              C.c_void_p(g_aoff_ptr) if g_aoff_ptr else None, _p(it), _p(te))
+        if check:
+            self._check("cabs_adjoint", rc)
+        return it, te, rc
+
+    def solve_batch(self, name, nrhs, b_ptr, x_ptr, adia_ptr, aoff_ptr, op_per_rhs=True, maxit=300, tol=1e-5,
+                    graph=True, check=True):
+        """Batched solve on compact device vectors [nrhs][3][NAT0] (x in/out).
+        Returns (itno[nrhs], tole[nrhs], ncompte[nrhs], rc)."""
+        it = np.zeros(nrhs, np.int32)
+        nc = np.zeros(nrhs, np.int32)
+        te = np.zeros(nrhs, np.float32)
+        rc = self.lib.ddscat_hip_solve_batch(SOLVER_IDS[name], int(nrhs), C.c_void_p(b_ptr), C.c_void_p(x_ptr),
+                                             C.c_void_p(adia_ptr), C.c_void_p(aoff_ptr), 1 if op_per_rhs else 0,
+                                             int(maxit), float(tol), _p(it), _p(te), _p(nc), 1 if graph else 0)
+        if check:
+            self._check(f"solve_batch_{name}", rc)
+        return it, te, nc, rc
 
     # --------------------------------------------------------------- evalq
     def evalq(self, ak, e02=1.0, use_resident=True, e_c=None, p_c=None, adia_c=None, aoff_c=None, check=True):

@@ -123,6 +123,62 @@ DDSCAT_HIP_API int ddscat_hip_set_fft_kernels(int generic);
 DDSCAT_HIP_API int ddscat_hip_apply_device(int cwhat, const void *x, void *y, const void *adia,
                                            const void *aoff, int layout, void *stream);
 
+/* Solver methods for ddscat_hip_solve_device. */
+enum {
+  DDSCAT_HIP_PBCGS2 = 0, DDSCAT_HIP_QMRCCG = 1, DDSCAT_HIP_PETRKP = 2, DDSCAT_HIP_GPBICG = 3,
+  DDSCAT_HIP_PBCGST = 4, DDSCAT_HIP_BICGSTAB4 = 5, DDSCAT_HIP_GPBICGSTAB2 = 6, DDSCAT_HIP_GPBICGSTAB4 = 7
+};
+
+/* The ddscat_cuda_solve_*_f32 solvers on compact [3][NAT0] DEVICE vectors
+ * (b, x in/out, adia, aoff; ndim = 3*NAT0), with the same iteration,
+ * convergence and return-code behaviour as the host entry points; blocks
+ * until the solve ends.  The solution stays resident for EVALQ / SCAT. */
+DDSCAT_HIP_API int ddscat_hip_solve_device(int method, const void *b, void *x, const void *adia,
+                                           const void *aoff, int ndim, int maxit, float tol, int *itno,
+                                           float *tole, int *ncompte);
+
+/* nrhs MATVECs in one launch sequence on compact DEVICE vectors x, y
+ * [nrhs][3][NAT0]; adia, aoff per RHS (op_per_rhs = 1) or shared (0).
+ * Stream-ordered on `stream` (NULL = library stream), FFT3D backend. */
+DDSCAT_HIP_API int ddscat_hip_apply_batch(int cwhat, int nrhs, const void *x, void *y, const void *adia,
+                                          const void *aoff, int op_per_rhs, void *stream);
+
+/* Batched solve of nrhs right-hand sides of the prepared target (FFT3D
+ * backend; method DDSCAT_HIP_QMRCCG or DDSCAT_HIP_PBCGS2).  b, x (initial guess in, solution out):
+ * compact DEVICE vectors [nrhs][3][NAT0]; adia, aoff: [nrhs][3][NAT0] when
+ * op_per_rhs = 1 (per-RHS LDR operators), one shared [3][NAT0] when 0.
+ * itno, tole, ncompte: host arrays [nrhs] (per-RHS iterations, achieved
+ * residual, MATVEC count).  Every RHS follows the single-RHS algorithm and
+ * reliable-residual policy; finished RHS stop while the rest continue.
+ * use_graph = 1 replays one captured HIP graph per iteration.  Returns 0, or
+ * the breakdown code of the first RHS that broke down (the others are
+ * still solved); 1 invalid argument, 2 method/backend not batched, 29 memory. */
+DDSCAT_HIP_API int ddscat_hip_solve_batch(int method, int nrhs, const void *b, void *x, const void *adia,
+                                          const void *aoff, int op_per_rhs, int maxit, float tol, int *itno,
+                                          float *tole, int *ncompte, int use_graph);
+
+/* EVALQ for nrhs solutions on DEVICE: e (incident field), p (polarization)
+ * [nrhs][3][NAT0], adia / aoff per RHS (op_per_rhs = 1) or shared (0);
+ * ak [nrhs][3] host floats; out [nrhs][3] host doubles = Cabs, Cext, Cpha
+ * per RHS with the formulas of ddscat_cuda_evalq_f32.  Codes 300-307. */
+DDSCAT_HIP_API int ddscat_hip_evalq_batch(int nrhs, const void *e, const void *p, const void *adia,
+                                          const void *aoff, int op_per_rhs, const float *ak, float e02,
+                                          double *out);
+
+/* Adjoint sensitivity of Cabs (the EVALQ absorption cross section) to the
+ * operators, for nrhs solutions p [nrhs][3][NAT0] on DEVICE with their adia /
+ * aoff (per RHS or shared) and ak [nrhs][3]:
+ *   dCabs = sum Re(g_adia * d adia) + sum Re(g_aoff * d aoff)
+ * g_adia (and g_aoff unless NULL) are written as complex64 [nrhs][3][NAT0]
+ * DEVICE arrays.  One batched adjoint solve A mu = W conj(p) (method
+ * DDSCAT_HIP_QMRCCG or DDSCAT_HIP_PBCGS2, tolerance tol) with the forward
+ * MATVEC (A is complex symmetric); itno / tole [nrhs] report it.  Codes: 1
+ * invalid argument, 2 method/backend, 3 launch, 29 memory, else the solver's. */
+DDSCAT_HIP_API int ddscat_hip_cabs_adjoint(int method, int nrhs, const void *p, const void *adia,
+                                           const void *aoff, int op_per_rhs, const float *ak, float e02,
+                                           int maxit, float tol, void *g_adia, void *g_aoff, int *itno,
+                                           float *tole);
+
 /* Copy the resident spectral Green tensor (ddscat_hip_info field 5 complex64
  * values) to host memory. */
 DDSCAT_HIP_API int ddscat_hip_get_green(void *out);
