@@ -130,6 +130,79 @@ def time_to_qext(lib, tg, kd, m=1.5 + 0.1j, method="qmrccg", tol=1e-5, maxit=100
                 qext_max_rel_diff_fp32_vs_mixed=qdiff)
 
 
+def batched_small_target(lib, n_orient=16, method="pbcgs2", tol=1e-5, maxit=300, seed=20261202):
+    """Sphere of diameter 32 dipoles (17,256 dipoles), m = 1.33+0.01i, x = 2,
+    n_orient incidence directions x 2 polarizations, PBCGS2 at tol 1e-5:
+    the small-target workload where single-RHS solves are launch-bound.
+    plain_fp32: one single-RHS device solve per RHS (PREC_FP32);
+    batched_mixed: all RHS in one batched solve (PREC_MIXED, graph replay).
+    Each is timed from the first solve call to all Q_ext on the host."""
+    import numpy as np
+    import torch
+    from ddscat_hip import PREC_MIXED, PREC_FP32
+    from ddscat_hip import targets as T
+    tg = T.sphere(32)
+    kd = T.kd_for_size_parameter(tg, 2.0)
+    rng = np.random.default_rng(seed)
+    dirs = rng.normal(size=(n_orient, 3))
+    dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
+    occ = tg.occ
+    comp = lambda v: np.ascontiguousarray(np.asarray(v).reshape(3, -1)[:, occ].reshape(-1)).astype(np.complex64)
+    bs, ads, aos, aks = [], [], [], []
+    for khat in dirs:
+        for e0 in T.polarization_pair(khat):
+            ad, ao = T.operators(tg, 1.33 + 0.01j, kd, khat, e0)
+            bs.append(comp(T.incident_field(tg, khat * kd, e0, tg.centroid_x0())))
+            ads.append(comp(ad)); aos.append(comp(ao)); aks.append(khat * kd)
+    B = len(bs)
+    aks = np.asarray(aks, np.float32)
+    lib.prepare_green(tg.nx, tg.ny, tg.nz, tg.iocc, aks[0])
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a).reshape(-1).view(np.float32)).cuda()
+    bd, add, aod = dev(np.stack(bs)), dev(np.stack(ads)), dev(np.stack(aos))
+    xd = torch.zeros_like(bd)
+    n2 = bd.numel() // B
+    assert bd.dim() == 1 and n2 == 2 * 3 * tg.nat0
+    row = lambda t, r: t[r * n2:(r + 1) * n2]
+    geo = math.pi * (3.0 * tg.nat0 / (4.0 * math.pi)) ** (2.0 / 3.0)
+    # warm-up (workspace allocation, graph code paths)
+    lib.solve_device(method, row(bd, 0).data_ptr(), row(xd, 0).data_ptr(), row(add, 0).data_ptr(),
+                     row(aod, 0).data_ptr(), maxit=2, tol=tol, check=False)
+    lib.solve_batch(method, B, bd.data_ptr(), xd.data_ptr(), add.data_ptr(), aod.data_ptr(), maxit=2, tol=tol,
+                    check=False)
+    lib.set_precision(PREC_FP32)
+    xd.zero_()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    q_single, it_single = [], []
+    for r in range(B):
+        it, tole, nc, rc = lib.solve_device(method, row(bd, r).data_ptr(), row(xd, r).data_ptr(),
+                                            row(add, r).data_ptr(), row(aod, r).data_ptr(), maxit=maxit, tol=tol,
+                                            check=False)
+        cabs, cext, _, _ = lib.evalq(aks[r], check=False)
+        q_single.append(cext / geo)
+        it_single.append(it)
+    t_single = time.perf_counter() - t0
+    lib.set_precision(PREC_MIXED)
+    xd.zero_()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    it, te, nc, rc = lib.solve_batch(method, B, bd.data_ptr(), xd.data_ptr(), add.data_ptr(), aod.data_ptr(),
+                                     maxit=maxit, tol=tol, check=False)
+    q = lib.evalq_batch(B, bd.data_ptr(), xd.data_ptr(), add.data_ptr(), aod.data_ptr(), aks, check=False)
+    t_batch = time.perf_counter() - t0
+    qb = q[:, 1] / geo
+    qs = np.asarray(q_single)
+    return dict(workload=f"sphere D=32 ({tg.nat0} dipoles), m=1.33+0.01i, x=2, {n_orient} orientations x 2 "
+                         f"polarizations = {B} RHS, {method.upper()} tol {tol}",
+                plain_fp32=dict(seconds=round(t_single, 4), rhs_per_s=round(B / t_single, 2),
+                                mean_iterations=float(np.mean(it_single))),
+                batched_mixed=dict(seconds=round(t_batch, 4), rhs_per_s=round(B / t_batch, 2),
+                                   mean_iterations=float(np.mean(it)), rc=int(rc)),
+                speedup_batched_mixed_vs_plain_fp32=round(t_single / t_batch, 3),
+                qext_max_rel_diff=float(np.max(np.abs(qb - qs) / np.abs(qs))),
+                qext_mean=float(np.mean(qb)))
+
+
 def T_incident_compact(tg, kvec, e0):
     import numpy as np
     from ddscat_hip import targets as T
@@ -184,6 +257,20 @@ def _green_octant_fast(nx, ny, nz, akd):
         a = np.einsum("kl,lyx->kyx", tmat(nz, par[2]), a, optimize=True)
         out[m] = (-1 if m in (1, 2, 4) else 1) * a
     return out
+
+
+TRAFFIC_FILE = ROOT / "profiles" / "matvec_fft3d_block128_traffic.json"
+
+
+def measured_traffic(args, dom):
+    """HBM bytes per launch of the dominant pass (FETCH_SIZE + WRITE_SIZE,
+    calibrated on a copy of known size) from the committed rocprofv3 counter
+    summary of this workload; None for workloads it does not cover."""
+    if args.backend != "fft3d" or args.edge != 128 or not TRAFFIC_FILE.exists():
+        return None
+    passes = list(json.loads(TRAFFIC_FILE.read_text())["passes"].values())
+    p = passes[dom]
+    return p["hbm_read_bytes"] + p["hbm_write_bytes"]
 
 
 def main():
@@ -255,9 +342,9 @@ def main():
         dom_bytes, dom_ms, dom_name = total_bytes, passes[5], "slices_matvec (all kernels)"
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
     roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None, kernel=dom_name)
+                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=measured_traffic(args, dom), kernel=dom_name)
 
-    solve = None
+    solve = small = None
     if not args.no_solve:
         solve = time_to_qext(lib, tg, kd, method=args.solver)
 
@@ -289,6 +376,9 @@ def main():
         "time_to_qext": solve,
         "cpu_baseline": cpu,
     }
+    if not args.no_solve:
+        # small-target batched leg last: it re-prepares the library for its own target
+        out["batched_sphere32"] = batched_small_target(lib)
     print(json.dumps(out))
     lib.release()
 
