@@ -203,6 +203,90 @@ def batched_small_target(lib, n_orient=16, method="pbcgs2", tol=1e-5, maxit=300,
                 qext_mean=float(np.mean(qb)))
 
 
+def random_rotations(rng, n):
+    """n rotation matrices uniform on SO(3) (unit quaternions)."""
+    import numpy as np
+    q = rng.normal(size=(n, 4))
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    w, x, y, z = q.T
+    return np.stack([np.stack([1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)], -1),
+                     np.stack([2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)], -1),
+                     np.stack([2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)], -1)], 1)
+
+
+def batched_targets(lib, n_targets=48, n_orient=8, kd=0.1, method="pbcgs2", tol=1e-5, maxit=500,
+                    single_sample=3, seed=20261202):
+    """Uneven batch: n_targets seeded unions of 1-4 ellipsoids (semi-axes
+    U[6,40] dipoles), m = U[1.2,2.0] + i U[0,0.3] per target, n_orient
+    orientations uniform on SO(3) x 2 polarizations per target, PBCGS2 at
+    tol 1e-5, mixed precision.  Per target: prepare (device Green build),
+    device RHS setup (incident fields + LDR operators for all its RHS), one
+    batched solve (graph replay), batched EVALQ.  The plain-FP32 single-RHS
+    rate is measured on the first `single_sample` targets."""
+    import numpy as np
+    import torch
+    from ddscat_hip import PREC_MIXED, PREC_FP32
+    from ddscat_hip import targets as T
+    rng = np.random.default_rng(seed)
+    tgts = []
+    for _ in range(n_targets):
+        tg = T.ellipsoid_union(rng)
+        m = complex(rng.uniform(1.2, 2.0), rng.uniform(0.0, 0.3))
+        tgts.append((tg, m, random_rotations(rng, n_orient)))
+    rhs_total, iters, t_batch, t_single, rhs_single = 0, [], 0.0, 0.0, 0
+    sizes = []
+    for ti, (tg, m, rots) in enumerate(tgts):
+        aks, e0s = [], []
+        for R in rots:
+            khat, ey, ez = R[:, 0], R[:, 1], R[:, 2]
+            for e0 in (ey, ez):
+                aks.append(khat * kd); e0s.append(e0.astype(np.complex64))
+        aks = np.asarray(aks, np.float32)
+        B = len(aks)
+        n = 3 * tg.nat0
+        eps = torch.full((tg.nat0,), m * m, dtype=torch.complex64, device="cuda")
+        b = torch.empty((B, n), dtype=torch.complex64, device="cuda")
+        ad, ao, x = torch.empty_like(b), torch.empty_like(b), torch.zeros_like(b)
+        x0 = tg.centroid_x0()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        lib.prepare_green(tg.nx, tg.ny, tg.nz, tg.iocc, aks[0])
+        lib.setup_rhs(aks, np.asarray(e0s), x0, eps.data_ptr(), b.data_ptr(), ad.data_ptr(), ao.data_ptr())
+        lib.set_precision(PREC_MIXED)
+        it, te, nc, rc = lib.solve_batch(method, B, b.data_ptr(), x.data_ptr(), ad.data_ptr(), ao.data_ptr(),
+                                         maxit=maxit, tol=tol, check=False)
+        lib.evalq_batch(B, b.data_ptr(), x.data_ptr(), ad.data_ptr(), ao.data_ptr(), aks, check=False)
+        t_batch += time.perf_counter() - t0
+        rhs_total += B
+        iters.extend(int(v) for v in it)
+        sizes.append((tg.nx, tg.ny, tg.nz, tg.nat0))
+        if ti < single_sample:
+            lib.set_precision(PREC_FP32)
+            x.zero_()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            lib.prepare_green(tg.nx, tg.ny, tg.nz, tg.iocc, aks[0])
+            for r in range(B):
+                lib.solve_device(method, b[r].data_ptr(), x[r].data_ptr(), ad[r].data_ptr(), ao[r].data_ptr(),
+                                 maxit=maxit, tol=tol, check=False)
+                lib.evalq(aks[r], check=False)
+            t_single += time.perf_counter() - t0
+            rhs_single += B
+            lib.set_precision(PREC_MIXED)
+        del b, ad, ao, x, eps
+    rate_b = rhs_total / t_batch
+    rate_s = rhs_single / t_single if t_single > 0 else None
+    return dict(workload=f"{n_targets} seeded ellipsoid unions (grids {min(s[0] for s in sizes)}-"
+                         f"{max(max(s[:3]) for s in sizes)} per axis, {sum(s[3] for s in sizes)} dipoles), "
+                         f"{n_orient} SO(3) orientations x 2 polarizations = {rhs_total} RHS, kd={kd}, "
+                         f"{method.upper()} tol {tol}",
+                batched_mixed=dict(seconds=round(t_batch, 3), rhs_per_s=round(rate_b, 1),
+                                   mean_iterations=float(np.mean(iters)), max_iterations=int(np.max(iters))),
+                plain_fp32_single=dict(sample_targets=single_sample, rhs=rhs_single, seconds=round(t_single, 3),
+                                       rhs_per_s=round(rate_s, 1) if rate_s else None),
+                speedup_batched_mixed_vs_plain_fp32=round(rate_b / rate_s, 3) if rate_s else None)
+
+
 def T_incident_compact(tg, kvec, e0):
     import numpy as np
     from ddscat_hip import targets as T
@@ -282,6 +366,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-solve", action="store_true", help="skip the time-to-Q_ext leg")
     ap.add_argument("--solver", default="qmrccg")
+    ap.add_argument("--targets", type=int, default=48, help="targets in the uneven batched leg")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--backend", choices=["fft3d", "slices"], default="fft3d")
     ap.add_argument("--pmc-calib", action="store_true", help="run one torch copy of known size (rocprof scale)")
@@ -379,6 +464,7 @@ def main():
     if not args.no_solve:
         # small-target batched leg last: it re-prepares the library for its own target
         out["batched_sphere32"] = batched_small_target(lib)
+        out["batched_targets"] = batched_targets(lib, n_targets=args.targets)
     print(json.dumps(out))
     lib.release()
 

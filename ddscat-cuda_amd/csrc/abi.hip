@@ -96,7 +96,17 @@ extern "C" DDSCAT_HIP_API int ddscat_cuda_prepare_f32(int nx, int ny, int nz, in
   }
   int rc = finish_prepare(c, iocc, 2, 6);
   if (rc) { release_context(c); return rc == 6 ? 6 : (rc >= 200 ? rc : 2); }
-  if (hipMemcpy(c.green, green, c.green_count * sizeof(float2), hipMemcpyHostToDevice) != hipSuccess) {
+  // CXZC arrives component-major; the resident copy is point-interleaved
+  float2* tmp = nullptr;
+  if (hipMalloc((void**)&tmp, c.green_count * sizeof(float2)) != hipSuccess) {
+    set_error("prepare", "malloc green staging failed");
+    release_context(c);
+    return 2;
+  }
+  const bool ok = hipMemcpy(tmp, green, c.green_count * sizeof(float2), hipMemcpyHostToDevice) == hipSuccess &&
+                  green_interleave(c, tmp, c.green, c.stream) == 0 && hipStreamSynchronize(c.stream) == hipSuccess;
+  hipFree(tmp);
+  if (!ok) {
     set_error("prepare", "green H2D failed");
     release_context(c);
     return 4;
@@ -159,8 +169,12 @@ extern "C" DDSCAT_HIP_API int ddscat_hip_apply_device(int cwhat, const void* x, 
 extern "C" DDSCAT_HIP_API int ddscat_hip_get_green(void* out) {
   Context& c = ctx();
   if (!c.ready || !out) { set_error("get_green", "not prepared"); return 10; }
-  DDH_TRY(hipStreamSynchronize(c.stream), "get_green sync", 12);
-  DDH_TRY(hipMemcpy(out, c.green, c.green_count * sizeof(float2), hipMemcpyDeviceToHost), "get_green D2H", 12);
+  float2* tmp = nullptr;
+  DDH_TRY(hipMalloc((void**)&tmp, c.green_count * sizeof(float2)), "get_green staging", 12);
+  const bool ok = green_deinterleave(c, c.green, tmp, c.stream) == 0 && hipStreamSynchronize(c.stream) == hipSuccess &&
+                  hipMemcpy(out, tmp, c.green_count * sizeof(float2), hipMemcpyDeviceToHost) == hipSuccess;
+  hipFree(tmp);
+  if (!ok) { set_error("get_green", "D2H failed"); return 12; }
   return 0;
 }
 

@@ -152,7 +152,6 @@ __global__ __launch_bounds__(MV_THREADS) void k_pass3_z_green(
   fft_run<+1, true, PER>(b2, tw, p, cz, cz, 1);
   // symmetric 3x3 multiply with the trimmed-octant reconstruction (cu:109-115)
   const int ox = nx + 1, oy = ny + 1;
-  const size_t st = (size_t)ox * oy * (nz + 1);
   const int jr = min(ky, gy - ky);
   const float sy = ky <= ny ? 1.f : -1.f;
   for (int idx = threadIdx.x; idx < L * cz; idx += T) {
@@ -162,13 +161,13 @@ __global__ __launch_bounds__(MV_THREADS) void k_pass3_z_green(
     const int ir = min(kx, gx - kx), kr = min(kz, L - kz);
     const float sx = kx <= nx ? 1.f : -1.f;
     const float sz = kz <= nz ? 1.f : -1.f;
-    const size_t b = (size_t)ir + (size_t)ox * ((size_t)jr + (size_t)oy * kr);
-    const float2 g11 = green[b];
-    const float2 g12 = cscale(green[b + st], sx * sy);
-    const float2 g13 = cscale(green[b + 2 * st], sx * sz);
-    const float2 g22 = green[b + 3 * st];
-    const float2 g23 = cscale(green[b + 4 * st], sy * sz);
-    const float2 g33 = green[b + 5 * st];
+    const Green6 G = green_at(green, (size_t)ir + (size_t)ox * ((size_t)jr + (size_t)oy * kr));
+    const float2 g11 = G.xx;
+    const float2 g12 = cscale(G.xy, sx * sy);
+    const float2 g13 = cscale(G.xz, sx * sz);
+    const float2 g22 = G.yy;
+    const float2 g23 = cscale(G.yz, sy * sz);
+    const float2 g33 = G.zz;
     const float2 p0 = b0[idx], p1 = b1[idx], p2 = b2[idx];
     b0[idx] = cadd(cadd(cmul(g11, p0), cmul(g12, p1)), cmul(g13, p2));
     b1[idx] = cadd(cadd(cmul(g12, p0), cmul(g22, p1)), cmul(g23, p2));

@@ -182,12 +182,11 @@ __global__ __launch_bounds__(RT) void kr_pass3(float2* __restrict__ a2, const fl
   // are in this block and the kx / ky mirror blocks run on the same XCD at
   // about the same time (grid mapping above), so repeats hit in L2.
   const int ox = nx + 1, oy = ny + 1;
-  const size_t st = (size_t)ox * oy * (nz + 1);
   const int ir = live ? min(kx, gx - kx) : 0;
   const int jr = min(ky, gy - ky);
   const float sxy = ((kx <= nx) == (ky <= ny)) ? 1.f : -1.f;
   const float sx = kx <= nx ? 1.f : -1.f, sy = ky <= ny ? 1.f : -1.f;
-  const float2* gb = green + (size_t)ir + (size_t)ox * jr;
+  const size_t pb = (size_t)ir + (size_t)ox * jr;
 #pragma unroll
   for (int i = 0; i < R2 / R1; ++i) {
 #pragma unroll
@@ -195,13 +194,13 @@ __global__ __launch_bounds__(RT) void kr_pass3(float2* __restrict__ a2, const fl
       const int kz = (t + R1 * i) + R2 * k1;
       const int kr = min(kz, L - kz);
       const float sz = kz <= nz ? 1.f : -1.f;
-      const float2* gp = gb + (size_t)ox * oy * kr;
-      const float2 g11 = gp[0];
-      const float2 g12 = cscale(gp[st], sxy);
-      const float2 g13 = cscale(gp[2 * st], sx * sz);
-      const float2 g22 = gp[3 * st];
-      const float2 g23 = cscale(gp[4 * st], sy * sz);
-      const float2 g33 = gp[5 * st];
+      const Green6 G = green_at(green, pb + (size_t)ox * oy * kr);
+      const float2 g11 = G.xx;
+      const float2 g12 = cscale(G.xy, sxy);
+      const float2 g13 = cscale(G.xz, sx * sz);
+      const float2 g22 = G.yy;
+      const float2 g23 = cscale(G.yz, sy * sz);
+      const float2 g33 = G.zz;
       const int e = i * R1 + k1;
       const float2 p0 = v0[e], p1 = v1[e], p2 = v2[e];
       v0[e] = cadd(cadd(cmul(g11, p0), cmul(g12, p1)), cmul(g13, p2));

@@ -93,4 +93,22 @@ int reg_pass(Context& c, int which, int cj, int compact, const float2* x, float2
 // Green tensor build on device (IPBC=0), result in c.green.
 int green_build(Context& c, int idipint, double akd, const double dx[3]);
 
+// The resident spectral Green tensor c.green is stored point-interleaved:
+// the six components (xx, xy, xz, yy, yz, zz) of octant point
+// p = ir + (NX+1)(jr + (NY+1) kr) sit at green[6p .. 6p+5], so a MATVEC reads
+// them as three 16-byte loads from one 48-byte run.  The reference layout
+// ([6][NZ+1][NY+1][NX+1], CXZC) is converted at the ABI edges.
+struct Green6 {
+  float2 xx, xy, xz, yy, yz, zz;
+};
+__device__ __forceinline__ Green6 green_at(const float2* __restrict__ g, size_t p) {
+  const float4* q = reinterpret_cast<const float4*>(g) + 3 * p;
+  const float4 a = q[0], b = q[1], c = q[2];
+  return Green6{make_float2(a.x, a.y), make_float2(a.z, a.w), make_float2(b.x, b.y),
+                make_float2(b.z, b.w), make_float2(c.x, c.y), make_float2(c.z, c.w)};
+}
+// component-major [6][npts] <-> point-interleaved [npts][6], device to device
+int green_interleave(Context& c, const float2* comp_major, float2* interleaved, hipStream_t st);
+int green_deinterleave(Context& c, const float2* interleaved, float2* comp_major, hipStream_t st);
+
 }  // namespace ddhip

@@ -30,7 +30,7 @@ EXTENSION_SYMBOLS = [
     "ddscat_hip_get_green", "ddscat_hip_info", "ddscat_hip_stream", "ddscat_hip_sync",
     "ddscat_hip_time_matvec", "ddscat_hip_set_fft_kernels", "ddscat_hip_solve_device",
     "ddscat_hip_solve_batch", "ddscat_hip_apply_batch", "ddscat_hip_evalq_batch",
-    "ddscat_hip_cabs_adjoint",
+    "ddscat_hip_cabs_adjoint", "ddscat_hip_setup_rhs",
 ]
 # method ids of ddscat_hip_solve_device
 SOLVER_IDS = {"pbcgs2": 0, "qmrccg": 1, "petrkp": 2, "gpbicg": 3, "pbcgst": 4, "bicgstab4": 5,
@@ -121,6 +121,8 @@ class Library:
         L.ddscat_hip_evalq_batch.restype = i
         L.ddscat_hip_cabs_adjoint.argtypes = [i, i, vp, vp, vp, i, vp, f, i, f, vp, vp, vp, vp]
         L.ddscat_hip_cabs_adjoint.restype = i
+        L.ddscat_hip_setup_rhs.argtypes = [i, vp, vp, vp, vp, vp, i, vp, vp, vp]
+        L.ddscat_hip_setup_rhs.restype = i
         self.nat = self.nat0 = 0
 
     # ---------------------------------------------------------------- errors
@@ -223,6 +225,18 @@ class Library:
         if check:
             self._check("cabs_adjoint", rc)
         return it, te, rc
+
+    def setup_rhs(self, ak, e0, x0, eps_ptr, b_ptr, adia_ptr, aoff_ptr, dx=(1.0, 1.0, 1.0), eps_per_comp=False):
+        """Device incident fields and LDR operators for RHS r = (ak[r], e0[r]) into
+        compact [nrhs][3][NAT0] device arrays (eps: complex64 device array)."""
+        ak = np.ascontiguousarray(np.asarray(ak, np.float32).reshape(-1, 3))
+        nrhs = ak.shape[0]
+        e0 = np.ascontiguousarray(np.asarray(e0, np.complex64).reshape(nrhs, 3))
+        x0 = np.ascontiguousarray(np.asarray(x0, np.float32).reshape(3))
+        dx = np.ascontiguousarray(np.asarray(dx, np.float32).reshape(3))
+        return self._check("setup_rhs", self.lib.ddscat_hip_setup_rhs(
+            nrhs, _p(ak), _p(e0), _p(x0), _p(dx), C.c_void_p(eps_ptr), 1 if eps_per_comp else 0,
+            C.c_void_p(b_ptr), C.c_void_p(adia_ptr), C.c_void_p(aoff_ptr)))
 
     def solve_batch(self, name, nrhs, b_ptr, x_ptr, adia_ptr, aoff_ptr, op_per_rhs=True, maxit=300, tol=1e-5,
                     graph=True, check=True):

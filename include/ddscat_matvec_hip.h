@@ -179,6 +179,20 @@ DDSCAT_HIP_API int ddscat_hip_cabs_adjoint(int method, int nrhs, const void *p, 
                                            int maxit, float tol, void *g_adia, void *g_aoff, int *itno,
                                            float *tole);
 
+/* Per-RHS inputs of a batched solve built on the DEVICE for the prepared
+ * target (what DDSCAT's EVALE and ALPHADIAG/LATTDR produce on the host):
+ * for r < nrhs with k d = ak[3r..3r+2] and polarization e0[6r..6r+5]
+ * (3 interleaved complex64),
+ *   b[r]    = e0 exp(i k.x_q), x_q = ((ix+1) + x0) dx per axis (FP64 phase),
+ *   adia[r] = 1/alpha of the lattice dispersion relation for eps(q),
+ *   aoff[r] = 0,
+ * all compact [nrhs][3][NAT0] DEVICE arrays.  eps: complex64 DEVICE array,
+ * [NAT0] (isotropic, eps_per_comp = 0) or [3][NAT0] (eps_per_comp = 1).
+ * Blocks until done.  Codes: 1 invalid argument, 2 copy, 3 launch, 29 memory. */
+DDSCAT_HIP_API int ddscat_hip_setup_rhs(int nrhs, const float *ak, const float *e0, const float *x0,
+                                        const float *dx, const void *eps, int eps_per_comp, void *b,
+                                        void *adia, void *aoff);
+
 /* Copy the resident spectral Green tensor (ddscat_hip_info field 5 complex64
  * values) to host memory. */
 DDSCAT_HIP_API int ddscat_hip_get_green(void *out);

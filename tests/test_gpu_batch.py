@@ -229,3 +229,36 @@ def test_evalq_batch_matches_single_evalq(lib, case):
         print(f"RHS {r}: batch {out[r]} single {ref}")
         assert np.allclose(out[r], ref, rtol=1e-9, atol=0.0)
         assert cext > 0 and cabs > 0
+
+
+def test_setup_rhs_matches_host_operators(lib):
+    """Device incident fields and LDR operators equal the host harness's
+    (targets.incident_field / targets.operators) for random incidences."""
+    import torch
+    tg = T.sphere(16)
+    kd = T.kd_for_size_parameter(tg, 2.0)
+    lib.prepare_green(tg.nx, tg.ny, tg.nz, tg.iocc, (kd, 0.0, 0.0))
+    rng = np.random.default_rng(21)
+    dirs = rng.normal(size=(3, 3))
+    dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
+    m = 1.6 + 0.05j
+    aks, e0s, refs = [], [], []
+    occ = tg.occ
+    comp = lambda v: np.asarray(v).reshape(3, -1)[:, occ].reshape(-1)
+    x0 = tg.centroid_x0()
+    for khat in dirs:
+        for e0 in T.polarization_pair(khat):
+            aks.append(khat * kd); e0s.append(e0)
+            ad, _ = T.operators(tg, m, kd, khat, e0)
+            refs.append((comp(T.incident_field(tg, khat * kd, e0, x0)), comp(ad)))
+    B, n = len(aks), 3 * tg.nat0
+    eps = torch.full((tg.nat0,), complex(m) ** 2, dtype=torch.complex64, device="cuda")
+    b = torch.empty((B, n), dtype=torch.complex64, device="cuda")
+    ad = torch.empty_like(b); ao = torch.empty_like(b)
+    lib.setup_rhs(np.asarray(aks), np.asarray(e0s), x0, eps.data_ptr(), b.data_ptr(), ad.data_ptr(), ao.data_ptr())
+    b, ad, ao = b.cpu().numpy(), ad.cpu().numpy(), ao.cpu().numpy()
+    for r, (bref, adref) in enumerate(refs):
+        eb = np.linalg.norm(b[r] - bref) / np.linalg.norm(bref)
+        ea = np.linalg.norm(ad[r] - adref) / np.linalg.norm(adref)
+        assert eb < 2e-6 and ea < 1e-6, (r, eb, ea)
+    assert not np.any(ao)
