@@ -233,8 +233,9 @@ def batched_targets(lib, n_targets=48, n_orient=8, kd=0.1, method="pbcgs2", tol=
         tg = T.ellipsoid_union(rng)
         m = complex(rng.uniform(1.2, 2.0), rng.uniform(0.0, 0.3))
         tgts.append((tg, m, random_rotations(rng, n_orient)))
-    rhs_total, iters, t_batch, t_single, rhs_single = 0, [], 0.0, 0.0, 0
-    sizes = []
+    rhs_total, iters, sizes = 0, [], []
+    stage = dict(prepare=0.0, setup=0.0, solve=0.0, evalq=0.0)
+    t_batch_sample, t_single, rhs_single = 0.0, 0.0, 0
     for ti, (tg, m, rots) in enumerate(tgts):
         aks, e0s = [], []
         for R in rots:
@@ -248,43 +249,50 @@ def batched_targets(lib, n_targets=48, n_orient=8, kd=0.1, method="pbcgs2", tol=
         b = torch.empty((B, n), dtype=torch.complex64, device="cuda")
         ad, ao, x = torch.empty_like(b), torch.empty_like(b), torch.zeros_like(b)
         x0 = tg.centroid_x0()
+        lib.set_precision(PREC_MIXED)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         lib.prepare_green(tg.nx, tg.ny, tg.nz, tg.iocc, aks[0])
+        t1 = time.perf_counter()
         lib.setup_rhs(aks, np.asarray(e0s), x0, eps.data_ptr(), b.data_ptr(), ad.data_ptr(), ao.data_ptr())
-        lib.set_precision(PREC_MIXED)
+        t2 = time.perf_counter()
         it, te, nc, rc = lib.solve_batch(method, B, b.data_ptr(), x.data_ptr(), ad.data_ptr(), ao.data_ptr(),
                                          maxit=maxit, tol=tol, check=False)
+        t3 = time.perf_counter()
         lib.evalq_batch(B, b.data_ptr(), x.data_ptr(), ad.data_ptr(), ao.data_ptr(), aks, check=False)
-        t_batch += time.perf_counter() - t0
+        t4 = time.perf_counter()
+        for k, v in zip(stage, (t1 - t0, t2 - t1, t3 - t2, t4 - t3)):
+            stage[k] += v
         rhs_total += B
         iters.extend(int(v) for v in it)
         sizes.append((tg.nx, tg.ny, tg.nz, tg.nat0))
         if ti < single_sample:
+            # same target, same prepared state: B single-RHS plain-FP32 solves + EVALQ
+            t_batch_sample += (t3 - t2) + (t4 - t3)
             lib.set_precision(PREC_FP32)
             x.zero_()
             torch.cuda.synchronize()
             t0 = time.perf_counter()
-            lib.prepare_green(tg.nx, tg.ny, tg.nz, tg.iocc, aks[0])
             for r in range(B):
                 lib.solve_device(method, b[r].data_ptr(), x[r].data_ptr(), ad[r].data_ptr(), ao[r].data_ptr(),
                                  maxit=maxit, tol=tol, check=False)
                 lib.evalq(aks[r], check=False)
             t_single += time.perf_counter() - t0
             rhs_single += B
-            lib.set_precision(PREC_MIXED)
         del b, ad, ao, x, eps
-    rate_b = rhs_total / t_batch
-    rate_s = rhs_single / t_single if t_single > 0 else None
-    return dict(workload=f"{n_targets} seeded ellipsoid unions (grids {min(s[0] for s in sizes)}-"
+    t_batch = sum(stage.values())
+    return dict(workload=f"{n_targets} seeded ellipsoid unions (grids {min(min(s[:3]) for s in sizes)}-"
                          f"{max(max(s[:3]) for s in sizes)} per axis, {sum(s[3] for s in sizes)} dipoles), "
                          f"{n_orient} SO(3) orientations x 2 polarizations = {rhs_total} RHS, kd={kd}, "
                          f"{method.upper()} tol {tol}",
-                batched_mixed=dict(seconds=round(t_batch, 3), rhs_per_s=round(rate_b, 1),
+                batched_mixed=dict(seconds=round(t_batch, 3), rhs_per_s=round(rhs_total / t_batch, 1),
+                                   stage_seconds={k: round(v, 3) for k, v in stage.items()},
                                    mean_iterations=float(np.mean(iters)), max_iterations=int(np.max(iters))),
-                plain_fp32_single=dict(sample_targets=single_sample, rhs=rhs_single, seconds=round(t_single, 3),
-                                       rhs_per_s=round(rate_s, 1) if rate_s else None),
-                speedup_batched_mixed_vs_plain_fp32=round(rate_b / rate_s, 3) if rate_s else None)
+                solve_comparison=dict(sample_targets=single_sample, rhs=rhs_single,
+                                      plain_fp32_single_seconds=round(t_single, 3),
+                                      batched_mixed_seconds=round(t_batch_sample, 3),
+                                      speedup_batched_mixed_vs_plain_fp32=round(t_single / t_batch_sample, 3)
+                                      if t_batch_sample > 0 else None))
 
 
 def T_incident_compact(tg, kvec, e0):
@@ -455,6 +463,7 @@ def main():
                                f"{args.backend.upper()} MATVEC on compact vectors",
                    "backend": args.backend, "nat0": tg.nat0},
         "roofline": roofline,
+        "fft_kernels": {f"pass{p}": info.get(f"kernel_pass{p}") for p in range(1, 6)} | {"autotuned": info.get("autotuned")},
         "matvec_gbs": round(total_bytes / (ms_step * 1e-3) / 1e9, 1),
         "matvec_bytes": total_bytes,
         "passes": per_pass,

@@ -152,6 +152,12 @@ extern "C" DDSCAT_HIP_API int ddscat_hip_set_fft_kernels(int generic) {
   return 0;
 }
 
+extern "C" DDSCAT_HIP_API int ddscat_hip_set_autotune(int on) {
+  if (on != 0 && on != 1) { set_error("set_autotune", "expected 0 or 1"); return 1; }
+  ctx().autotune = on;
+  return 0;
+}
+
 extern "C" DDSCAT_HIP_API int ddscat_hip_apply_device(int cwhat, const void* x, void* y, const void* adia,
                                                       const void* aoff, int layout, void* stream) {
   Context& c = ctx();
@@ -183,7 +189,8 @@ extern "C" DDSCAT_HIP_API int ddscat_hip_info(long long* out, int n) {
   if (!out) return 1;
   const long long v[] = {c.nx, c.ny, c.nz, c.nat, c.nat0, (long long)c.green_count, (long long)c.matvec_count,
                          c.backend, c.precision, c.mv.gx, c.mv.gy, c.mv.gz, c.mv.rx_fwd, c.mv.cy, c.mv.cz,
-                         c.mv.rx_inv, (long long)(c.green_build_ms * 1000.0)};
+                         c.mv.rx_inv, (long long)(c.green_build_ms * 1000.0), c.mv.kind[1], c.mv.kind[2],
+                         c.mv.kind[3], c.mv.kind[4], c.mv.kind[5], c.mv.tuned};
   const int m = (int)(sizeof(v) / sizeof(v[0]));
   for (int i = 0; i < n && i < m; ++i) out[i] = v[i];
   return 0;
@@ -199,6 +206,44 @@ extern "C" DDSCAT_HIP_API int ddscat_hip_sync(void) {
   Context& c = ctx();
   if (!c.stream) return 0;
   DDH_TRY(hipStreamSynchronize(c.stream), "sync", 1);
+  return 0;
+}
+
+namespace {
+// out = e_inc - y (E_inc + E_sca with y = -E_sca), or -y without e_inc
+__global__ void k_nearfield_finish(const float2* __restrict__ y, const float2* __restrict__ einc,
+                                   float2* __restrict__ out, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float2 v = y[i];
+  out[i] = einc ? make_float2(einc[i].x - v.x, einc[i].y - v.y) : make_float2(-v.x, -v.y);
+}
+}  // namespace
+
+// Near field on the whole lattice (DDSCAT NRFLD): with the local operator
+// switched off, the MATVEC returns y = -E_sca where E_sca,j = sum_k c_jk P_k is
+// the field the dipoles radiate at site j (c = -A_jk, cu:625-626), and pass 5
+// without an IOCC mask evaluates it on empty sites too.
+extern "C" DDSCAT_HIP_API int ddscat_hip_nearfield(const void* p_compact, const void* einc_full, void* out_full) {
+  Context& c = ctx();
+  if (!c.ready) { set_error("nearfield", "not prepared"); return 10; }
+  if (!p_compact || !out_full) { set_error("nearfield", "invalid argument"); return 11; }
+  if (c.backend != BACKEND_FFT3D) { set_error("nearfield", "near field runs on the FFT3D backend"); return 11; }
+  clear_error();
+  hipStream_t st = c.stream;
+  const size_t n = (size_t)3 * c.nat;
+  DDH_TRY(hipMemsetAsync(c.adia, 0, n * sizeof(float2), st), "nearfield", 12);
+  DDH_TRY(hipMemsetAsync(c.aoff, 0, n * sizeof(float2), st), "nearfield", 12);
+  if (scatter_full(c, (const float2*)p_compact, c.xfull, st)) return 12;
+  int16_t* keep = c.iocc;
+  c.iocc = nullptr;  // full-layout finish without the occupancy mask
+  const int rc = matvec_launch(c, 'N', c.xfull, c.yfull, c.adia, c.aoff, 0, st);
+  c.iocc = keep;
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_nearfield_finish, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, c.yfull,
+                     (const float2*)einc_full, (float2*)out_full, n);
+  DDH_LAUNCH_OK("near-field finish", 13);
+  DDH_TRY(hipStreamSynchronize(st), "nearfield", 13);
   return 0;
 }
 

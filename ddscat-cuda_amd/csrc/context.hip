@@ -60,12 +60,13 @@ void release_context(Context& c) {
   dfree(c.red); dfree(c.scal);
   if (c.hscal) hipHostFree(c.hscal);
   hipStream_t keep = c.stream;
-  int backend = c.backend, prec = c.precision, gen = c.fft_generic;
+  int backend = c.backend, prec = c.precision, gen = c.fft_generic, tune = c.autotune;
   c = Context{};
   c.stream = keep;
   c.backend = backend;
   c.precision = prec;
   c.fft_generic = gen;
+  c.autotune = tune;
 }
 
 // Common tail of prepare_green / prepare: dimensions already set, Green
@@ -129,6 +130,7 @@ int finish_prepare(Context& c, const int16_t* iocc, int code_alloc, int code_occ
   DDH_TRY(hipMemcpy(c.rowptr, rowptr.data(), ((size_t)nrows + 1) * sizeof(int), hipMemcpyHostToDevice), "row map H2D", code_occ);
   DDH_TRY(hipMemset(c.adia, 0, n * sizeof(float2)), "clear adia", code_alloc);
   DDH_TRY(hipMemset(c.aoff, 0, n * sizeof(float2)), "clear aoff", code_alloc);
+  if (c.backend == BACKEND_FFT3D && matvec_autotune(c)) return code_alloc;
   return 0;
 }
 

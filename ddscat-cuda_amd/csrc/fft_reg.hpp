@@ -32,11 +32,76 @@ __host__ __device__ constexpr float cos32(int m) {
 }
 __host__ __device__ constexpr float sin32(int m) { return cos32(m - 8); }
 
-// multiply by e^{DIR 2 pi i m / N} for N dividing 32, m compile-time after unrolling
+// cos / sin of 2 pi m / N for any N, evaluated at compile time: the angle is
+// reduced to [-pi/4, pi/4] by octant and expanded in Taylor series to double
+// precision, so a table entry is exact to float rounding.
+constexpr double kTwoPi = 6.283185307179586476925286766559;
+constexpr double taylor_sin(double x) {
+  double t = x, s = x;
+  for (int k = 1; k < 12; ++k) {
+    t *= -x * x / ((2.0 * k) * (2.0 * k + 1.0));
+    s += t;
+  }
+  return s;
+}
+constexpr double taylor_cos(double x) {
+  double t = 1.0, s = 1.0;
+  for (int k = 1; k < 12; ++k) {
+    t *= -x * x / ((2.0 * k - 1.0) * (2.0 * k));
+    s += t;
+  }
+  return s;
+}
+// (cos, sin) of 2 pi m / N via the octant o = round(8 m / N) and r = 2 pi m / N - o pi / 4
+constexpr double cos_frac(long m, long N, bool want_sin) {
+  m %= N;
+  if (m < 0) m += N;
+  const long o = (8 * m + N / 2) / N;  // nearest eighth of a turn
+  const double r = kTwoPi * ((double)m / (double)N) - (double)o * (kTwoPi / 8.0);
+  const double c = taylor_cos(r), s = taylor_sin(r);
+  const double h = 0.70710678118654752440;
+  // rotate (c, s) by o * 45 degrees
+  double cr = c, sr = s;
+  switch (o & 7) {
+    case 0: cr = c; sr = s; break;
+    case 1: cr = h * (c - s); sr = h * (c + s); break;
+    case 2: cr = -s; sr = c; break;
+    case 3: cr = -h * (c + s); sr = h * (c - s); break;
+    case 4: cr = -c; sr = -s; break;
+    case 5: cr = h * (s - c); sr = -h * (c + s); break;
+    case 6: cr = s; sr = -c; break;
+    default: cr = h * (c + s); sr = h * (s - c); break;
+  }
+  return want_sin ? sr : cr;
+}
+template <int N>
+struct TwTable {
+  float c[N], s[N];
+  constexpr TwTable() : c(), s() {
+    for (int m = 0; m < N; ++m) {
+      c[m] = (float)cos_frac(m, N, false);
+      s[m] = (float)cos_frac(m, N, true);
+    }
+  }
+};
+template <int N>
+struct Tw {
+  static constexpr TwTable<N> t{};
+};
+
+// multiply by e^{DIR 2 pi i m / N}, m compile-time after unrolling
 template <int N, int DIR>
 __device__ __forceinline__ float2 twc(float2 a, int m) {
-  const int q = m * (32 / N);
-  const float c = cos32(q), s = DIR * sin32(q);
+  float c, s;
+  if constexpr (32 % N == 0) {
+    const int q = m * (32 / N);
+    c = cos32(q);
+    s = DIR * sin32(q);
+  } else {
+    const int q = m % N;
+    c = Tw<N>::t.c[q];
+    s = DIR * Tw<N>::t.s[q];
+  }
   return make_float2(a.x * c - a.y * s, a.x * s + a.y * c);
 }
 
@@ -77,29 +142,51 @@ __device__ __forceinline__ void dft_reg(float2* a) {
     dft4<DIR>(a);
   } else if constexpr (R == 8) {
     dft8<DIR>(a);
+  } else if constexpr (R == 3) {
+    dft3<DIR>(a);
+  } else if constexpr (R == 5) {
+    dft5<DIR>(a);
   } else if constexpr (R == 16) {
     dft_comp<4, 4, DIR>(a);
   } else if constexpr (R == 32) {
     dft_comp<4, 8, DIR>(a);
+  } else if constexpr (R % 8 == 0 && R > 8) {
+    dft_comp<R / 8, 8, DIR>(a);
+  } else if constexpr (R % 4 == 0 && R > 4) {
+    dft_comp<R / 4, 4, DIR>(a);
+  } else if constexpr (R % 5 == 0 && R > 5) {
+    dft_comp<R / 5, 5, DIR>(a);
+  } else if constexpr (R % 3 == 0 && R > 3) {
+    dft_comp<R / 3, 3, DIR>(a);
+  } else if constexpr (R % 2 == 0 && R > 2) {
+    dft_comp<R / 2, 2, DIR>(a);
   } else {
     static_assert(R == 1, "unsupported register DFT size");
   }
 }
 
-// LDS exchange layout for C sequences: element (row, s) at row * P + s,
-// row = k2 * R1 + n1, P = C + 1 (pad against bank conflicts)
+// Work split of a length L = R1 * R2 transform over R1 threads.  The first
+// step is one R2-point DFT per thread; the second step's R2 DFTs of length R1
+// go round-robin to the threads: thread t owns k2 = t + R1 i for i < G while
+// k2 < R2 (all of them when R1 divides R2).  A thread's register array holds
+// VN values.  LDS exchange layout for C sequences: element (row, s) at
+// row * P + s, row = k2 * R1 + n1, P = C + 1 (pad against bank conflicts).
 template <int R1, int R2>
 struct Reg2 {
   static constexpr int L = R1 * R2;
-  static constexpr int G = R2 / R1;  // k2 groups per thread in the second step
-  static_assert(R2 % R1 == 0, "R2 must be a multiple of R1");
+  static constexpr int G = (R2 + R1 - 1) / R1;
+  static constexpr int VN = R2 > G * R1 ? R2 : G * R1;
 };
+template <int R1, int R2>
+__device__ __forceinline__ bool reg2_owns(int t, int i) {
+  return Reg2<R1, R2>::G * R1 == R2 || t + R1 * i < R2;
+}
 
-// forward: v[n2] = x[t + R1 n2] in, v[i * R1 + k1] = X[(t + R1 i) + R2 k1] out
+// forward: v[n2] = x[t + R1 n2] (n2 < R2) in; v[i R1 + k1] = X[(t + R1 i) + R2 k1]
+// out for the groups i the thread owns
 template <int R1, int R2, int DIR>
 __device__ __forceinline__ void fft2_fwd(float2* v, float2* __restrict__ lds, const float2* __restrict__ twL, int t,
                                          int s, int P) {
-  constexpr int L = R1 * R2;
   dft_reg<R2, DIR>(v);
 #pragma unroll
   for (int k2 = 0; k2 < R2; ++k2) {
@@ -109,7 +196,8 @@ __device__ __forceinline__ void fft2_fwd(float2* v, float2* __restrict__ lds, co
   }
   __syncthreads();
 #pragma unroll
-  for (int i = 0; i < R2 / R1; ++i) {
+  for (int i = 0; i < Reg2<R1, R2>::G; ++i) {
+    if (!reg2_owns<R1, R2>(t, i)) continue;
     const int k2 = t + R1 * i;
     float2 u[R1];
 #pragma unroll
@@ -118,16 +206,16 @@ __device__ __forceinline__ void fft2_fwd(float2* v, float2* __restrict__ lds, co
 #pragma unroll
     for (int k1 = 0; k1 < R1; ++k1) v[i * R1 + k1] = u[k1];
   }
-  (void)L;
 }
 
-// inverse of the forward distribution: v[i * R1 + k1] = Y[(t + R1 i) + R2 k1]
-// in, v[n2] = y[t + R1 n2] out (direction DIR, normally -1)
+// inverse of the forward distribution: v[i R1 + k1] = Y[(t + R1 i) + R2 k1] in
+// (owned groups), v[n2] = y[t + R1 n2] out (direction DIR, normally -1)
 template <int R1, int R2, int DIR>
 __device__ __forceinline__ void fft2_inv(float2* v, float2* __restrict__ lds, const float2* __restrict__ twL, int t,
                                          int s, int P) {
 #pragma unroll
-  for (int i = 0; i < R2 / R1; ++i) {
+  for (int i = 0; i < Reg2<R1, R2>::G; ++i) {
+    if (!reg2_owns<R1, R2>(t, i)) continue;
     const int k2 = t + R1 * i;
     float2 u[R1];
 #pragma unroll
@@ -144,6 +232,47 @@ __device__ __forceinline__ void fft2_inv(float2* v, float2* __restrict__ lds, co
 #pragma unroll
   for (int k2 = 0; k2 < R2; ++k2) v[k2] = lds[(k2 * R1 + t) * P + s];
   dft_reg<R2, DIR>(v);
+}
+
+// Split-exchange form of fft2_fwd: the real and the imaginary parts cross
+// LDS in turn, so the exchange buffer holds L*P floats instead of L*P
+// float2.  Half the LDS per workgroup doubles the workgroups a CU can hold
+// for kernels whose occupancy LDS limits (pass 1); two more barriers.
+template <int R1, int R2, int DIR>
+__device__ __forceinline__ void fft2_fwd_s(float2* v, float* __restrict__ lds, const float2* __restrict__ twL, int t,
+                                           int s, int P) {
+  constexpr int G = Reg2<R1, R2>::G;
+  dft_reg<R2, DIR>(v);
+#pragma unroll
+  for (int k2 = 0; k2 < R2; ++k2) {
+    float2 w = twL[t * k2];
+    if (DIR < 0) w.y = -w.y;
+    v[k2] = cmul(v[k2], w);
+  }
+  float re[G * R1];
+#pragma unroll
+  for (int k2 = 0; k2 < R2; ++k2) lds[(k2 * R1 + t) * P + s] = v[k2].x;
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < G; ++i) {
+    if (!reg2_owns<R1, R2>(t, i)) continue;
+#pragma unroll
+    for (int n1 = 0; n1 < R1; ++n1) re[i * R1 + n1] = lds[((t + R1 * i) * R1 + n1) * P + s];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k2 = 0; k2 < R2; ++k2) lds[(k2 * R1 + t) * P + s] = v[k2].y;
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < G; ++i) {
+    if (!reg2_owns<R1, R2>(t, i)) continue;
+    float2 u[R1];
+#pragma unroll
+    for (int n1 = 0; n1 < R1; ++n1) u[n1] = make_float2(re[i * R1 + n1], lds[((t + R1 * i) * R1 + n1) * P + s]);
+    dft_reg<R1, DIR>(u);
+#pragma unroll
+    for (int k1 = 0; k1 < R1; ++k1) v[i * R1 + k1] = u[k1];
+  }
 }
 
 }  // namespace ddhip

@@ -21,6 +21,8 @@
 #include <vector>
 #include <cmath>
 #include <algorithm>
+#include <array>
+#include <map>
 
 namespace ddhip {
 
@@ -355,62 +357,145 @@ void matvec_release(Context& c) {
   }
 }
 
-int matvec_launch(Context& c, int cwhat, const float2* x, float2* y, const float2* adia,
-                  const float2* aoff, int layout, hipStream_t st, hipEvent_t* ev, const MvBatch* btp) {
+// One MATVEC pass with kernel choice `kind` (-1: LDS-staged, >= 0: register shape).
+static int launch_pass(Context& c, int which, int kind, int cj, int compact, const float2* x, float2* y,
+                       const float2* adia, const float2* aoff, hipStream_t st, const MvBatch& bt) {
   const MatvecPlan& m = c.mv;
   const int nrows = c.ny * c.nz;
+  const unsigned nb = (unsigned)bt.nrhs;
+  float2 *a1 = bt.a1, *a2 = bt.a2;
+  if (!c.fft_generic && kind >= 0 && reg_pass(c, which, kind, cj, compact, x, y, adia, aoff, st, bt)) return 0;
+  switch (which) {
+    case 1: {
+      dim3 grid((nrows + m.rx_fwd - 1) / m.rx_fwd, 3, nb);
+      size_t lds = (size_t)(m.gx + m.rx_fwd * m.gx) * sizeof(float2);
+      hipLaunchKernelGGL(k_pass1_xfwd<PER_WIDE>, grid, dim3(MV_THREADS), lds, st, x, a1, c.rowptr, c.occ, m.twx,
+                         m.px, c.nx, nrows, m.rx_fwd, c.nat, c.nat0, compact, cj, bt.k);
+      DDH_LAUNCH_OK("MATVEC pass 1 (x FFT)", 21);
+      break;
+    }
+    case 2: {
+      dim3 grid((m.gx + m.cy - 1) / m.cy, c.nz, 3 * nb);
+      size_t lds = (size_t)(m.gy + m.cy * m.gy) * sizeof(float2);
+      hipLaunchKernelGGL((k_pass_y<+1, PER_WIDE>), grid, dim3(MV_THREADS), lds, st, a1, a2, m.twy, m.py, m.gx,
+                         c.ny, m.gy, c.nz, ilog2(m.cy), bt.k);
+      DDH_LAUNCH_OK("MATVEC pass 2 (y FFT)", 22);
+      break;
+    }
+    case 3: {
+      dim3 grid((m.gx + m.cz - 1) / m.cz, m.gy, nb);
+      size_t lds = (size_t)(m.gz + 3 * m.cz * m.gz) * sizeof(float2);
+      hipLaunchKernelGGL(k_pass3_z_green<PER_NARROW>, grid, dim3(MV_THREADS), lds, st, a2, c.green, m.twz, m.pz,
+                         c.nx, c.ny, c.nz, m.gx, m.gy, ilog2(m.cz), bt.k);
+      DDH_LAUNCH_OK("MATVEC pass 3 (z FFT + Green)", 23);
+      break;
+    }
+    case 4: {
+      dim3 grid((m.gx + m.cy - 1) / m.cy, c.nz, 3 * nb);
+      size_t lds = (size_t)(m.gy + m.cy * m.gy) * sizeof(float2);
+      hipLaunchKernelGGL((k_pass_y<-1, PER_WIDE>), grid, dim3(MV_THREADS), lds, st, a2, a1, m.twy, m.py, m.gx,
+                         m.gy, c.ny, c.nz, ilog2(m.cy), bt.k);
+      DDH_LAUNCH_OK("MATVEC pass 4 (y inverse FFT)", 24);
+      break;
+    }
+    default: {
+      dim3 grid((nrows + m.rx_inv - 1) / m.rx_inv, nb);
+      size_t lds = (size_t)(m.gx + 3 * m.rx_inv * m.gx) * sizeof(float2);
+      const float inv = 1.f / (float)m.ngrid;
+      hipLaunchKernelGGL(k_pass5_xinv_finish<PER_NARROW>, grid, dim3(MV_THREADS), lds, st, a1, m.twx, m.px, x, y,
+                         adia, aoff, compact ? nullptr : c.iocc, c.rowptr, c.occ, c.nx, nrows, m.rx_inv, c.nat,
+                         c.nat0, compact, cj, inv, bt.k);
+      DDH_LAUNCH_OK("MATVEC pass 5 (x inverse FFT + finish)", 25);
+      break;
+    }
+  }
+  return 0;
+}
+
+int matvec_launch(Context& c, int cwhat, const float2* x, float2* y, const float2* adia,
+                  const float2* aoff, int layout, hipStream_t st, hipEvent_t* ev, const MvBatch* btp) {
   const int cj = (cwhat == 'C') ? 1 : 0;
   const int compact = layout ? 1 : 0;
   MvBatch bt = btp ? *btp : MvBatch{};
   if (!bt.a1) bt.a1 = c.a1;
   if (!bt.a2) bt.a2 = c.a2;
-  const unsigned nb = (unsigned)bt.nrhs;
-  float2 *a1 = bt.a1, *a2 = bt.a2;
-  if (ev) hipEventRecord(ev[0], st);
-  if (c.fft_generic || !reg_pass(c, 1, cj, compact, x, y, adia, aoff, st, bt)) {
-    dim3 grid((nrows + m.rx_fwd - 1) / m.rx_fwd, 3, nb);
-    size_t lds = (size_t)(m.gx + m.rx_fwd * m.gx) * sizeof(float2);
-    hipLaunchKernelGGL(k_pass1_xfwd<PER_WIDE>, grid, dim3(MV_THREADS), lds, st, x, a1, c.rowptr,
-                       c.occ, m.twx, m.px, c.nx, nrows, m.rx_fwd, c.nat, c.nat0, compact, cj, bt.k);
-    DDH_LAUNCH_OK("MATVEC pass 1 (x FFT)", 21);
-  }
-  if (ev) hipEventRecord(ev[1], st);
-  if (c.fft_generic || !reg_pass(c, 2, cj, compact, x, y, adia, aoff, st, bt)) {
-    dim3 grid((m.gx + m.cy - 1) / m.cy, c.nz, 3 * nb);
-    size_t lds = (size_t)(m.gy + m.cy * m.gy) * sizeof(float2);
-    hipLaunchKernelGGL((k_pass_y<+1, PER_WIDE>), grid, dim3(MV_THREADS), lds, st, a1, a2, m.twy,
-                       m.py, m.gx, c.ny, m.gy, c.nz, ilog2(m.cy), bt.k);
-    DDH_LAUNCH_OK("MATVEC pass 2 (y FFT)", 22);
-  }
-  if (ev) hipEventRecord(ev[2], st);
-  if (c.fft_generic || !reg_pass(c, 3, cj, compact, x, y, adia, aoff, st, bt)) {
-    dim3 grid((m.gx + m.cz - 1) / m.cz, m.gy, nb);
-    size_t lds = (size_t)(m.gz + 3 * m.cz * m.gz) * sizeof(float2);
-    hipLaunchKernelGGL(k_pass3_z_green<PER_NARROW>, grid, dim3(MV_THREADS), lds, st, a2, c.green,
-                       m.twz, m.pz, c.nx, c.ny, c.nz, m.gx, m.gy, ilog2(m.cz), bt.k);
-    DDH_LAUNCH_OK("MATVEC pass 3 (z FFT + Green)", 23);
-  }
-  if (ev) hipEventRecord(ev[3], st);
-  if (c.fft_generic || !reg_pass(c, 4, cj, compact, x, y, adia, aoff, st, bt)) {
-    dim3 grid((m.gx + m.cy - 1) / m.cy, c.nz, 3 * nb);
-    size_t lds = (size_t)(m.gy + m.cy * m.gy) * sizeof(float2);
-    hipLaunchKernelGGL((k_pass_y<-1, PER_WIDE>), grid, dim3(MV_THREADS), lds, st, a2, a1, m.twy,
-                       m.py, m.gx, m.gy, c.ny, c.nz, ilog2(m.cy), bt.k);
-    DDH_LAUNCH_OK("MATVEC pass 4 (y inverse FFT)", 24);
-  }
-  if (ev) hipEventRecord(ev[4], st);
-  if (c.fft_generic || !reg_pass(c, 5, cj, compact, x, y, adia, aoff, st, bt)) {
-    dim3 grid((nrows + m.rx_inv - 1) / m.rx_inv, nb);
-    size_t lds = (size_t)(m.gx + 3 * m.rx_inv * m.gx) * sizeof(float2);
-    const float inv = 1.f / (float)m.ngrid;
-    hipLaunchKernelGGL(k_pass5_xinv_finish<PER_NARROW>, grid, dim3(MV_THREADS), lds, st, a1, m.twx,
-                       m.px, x, y, adia, aoff, compact ? nullptr : c.iocc, c.rowptr, c.occ, c.nx,
-                       nrows, m.rx_inv, c.nat, c.nat0, compact, cj, inv, bt.k);
-    DDH_LAUNCH_OK("MATVEC pass 5 (x inverse FFT + finish)", 25);
+  for (int p = 1; p <= 5; ++p) {
+    if (ev) hipEventRecord(ev[p - 1], st);
+    if (int rc = launch_pass(c, p, c.mv.kind[p], cj, compact, x, y, adia, aoff, st, bt)) return rc;
   }
   DDH_LAUNCH_OK("MATVEC register passes", 20);
   if (ev) hipEventRecord(ev[5], st);
-  c.matvec_count += nb;
+  c.matvec_count += (unsigned)bt.nrhs;
+  return 0;
+}
+
+// Autotuner: for each pass, every register shape of its axis length and the
+// LDS-staged kernel run a few times on the prepared grid (compact vectors,
+// stream events); the fastest is kept in c.mv.kind.  Choices are cached per
+// grid for the life of the process, so re-preparing the same grid (for
+// instance every target orientation of a DDSCAT run) does not re-time.
+namespace {
+struct TuneKey {
+  int nx, ny, nz;
+  bool operator<(const TuneKey& o) const {
+    return nx != o.nx ? nx < o.nx : ny != o.ny ? ny < o.ny : nz < o.nz;
+  }
+};
+std::map<TuneKey, std::array<int, 6>>& tune_cache() {
+  static std::map<TuneKey, std::array<int, 6>> m;
+  return m;
+}
+}  // namespace
+
+int matvec_autotune(Context& c) {
+  MatvecPlan& m = c.mv;
+  const int lens[6] = {0, m.gx, m.gy, m.gz, m.gy, m.gx};
+  for (int p = 1; p <= 5; ++p) m.kind[p] = reg_candidates(lens[p]) > 0 ? 0 : -1;
+  m.tuned = 0;
+  if (!c.autotune || c.fft_generic || !c.sx || !c.sy || !c.adia_c || !c.aoff_c) return 0;
+  const TuneKey key{c.nx, c.ny, c.nz};
+  auto it = tune_cache().find(key);
+  if (it != tune_cache().end()) {
+    for (int p = 1; p <= 5; ++p) m.kind[p] = it->second[p];
+    m.tuned = 1;
+    return 0;
+  }
+  hipStream_t st = c.stream;
+  const size_t cb = (size_t)3 * c.nat0 * sizeof(float2);
+  DDH_TRY(hipMemsetAsync(c.sx, 0, cb, st), "autotune", 20);
+  DDH_TRY(hipMemsetAsync(c.adia_c, 0, cb, st), "autotune", 20);
+  DDH_TRY(hipMemsetAsync(c.aoff_c, 0, cb, st), "autotune", 20);
+  hipEvent_t e0, e1;
+  DDH_TRY(hipEventCreate(&e0), "autotune", 20);
+  if (hipEventCreate(&e1) != hipSuccess) { hipEventDestroy(e0); set_error("autotune", "event"); return 20; }
+  MvBatch bt;
+  bt.a1 = c.a1;
+  bt.a2 = c.a2;
+  int rc = 0;
+  for (int p = 1; p <= 5 && !rc; ++p) {
+    const int ncand = reg_candidates(lens[p]);
+    if (ncand == 0) continue;
+    float best = 1e30f;
+    int best_kind = m.kind[p];
+    for (int k = -1; k < ncand && !rc; ++k) {
+      for (int w = 0; w < 2 && !rc; ++w) rc = launch_pass(c, p, k, 0, 1, c.sx, c.sy, c.adia_c, c.aoff_c, st, bt);
+      hipEventRecord(e0, st);
+      for (int r = 0; r < 5 && !rc; ++r) rc = launch_pass(c, p, k, 0, 1, c.sx, c.sy, c.adia_c, c.aoff_c, st, bt);
+      hipEventRecord(e1, st);
+      if (rc || hipEventSynchronize(e1) != hipSuccess) { rc = rc ? rc : 20; break; }
+      float ms = 0.f;
+      hipEventElapsedTime(&ms, e0, e1);
+      if (ms < best) { best = ms; best_kind = k; }
+    }
+    m.kind[p] = best_kind;
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (rc) return rc;
+  std::array<int, 6> k6;
+  for (int p = 0; p < 6; ++p) k6[p] = m.kind[p];
+  tune_cache()[key] = k6;
+  m.tuned = 1;
   return 0;
 }
 

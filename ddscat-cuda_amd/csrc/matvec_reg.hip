@@ -1,5 +1,5 @@
 // FFT3D MATVEC passes built on the register-resident two-level FFT
-// (fft_reg.hpp) for padded axis lengths 2N = R1 * R2 in {64, 128, 256}.
+// (fft_reg.hpp) for the padded axis lengths 2N listed in kShapes below.
 // Same five passes, buffers and operator as matvec.hip; the difference is
 // that a transform keeps its data in registers, crosses LDS once, and pass 3
 // does the 3x3 Green multiply in registers between its forward and inverse
@@ -15,7 +15,6 @@
 
 namespace ddhip {
 
-static constexpr int RT = 256;  // threads per workgroup
 
 struct SiteMap {
   const int* rowptr;
@@ -35,13 +34,13 @@ __device__ __forceinline__ int compact_index(const SiteMap& m, int g, int ix, in
 }
 
 // ------------------------------------------------------------------ pass 1
-template <int R1, int R2>
-__global__ __launch_bounds__(RT) void kr_pass1(const float2* __restrict__ x, float2* __restrict__ a1, SiteMap sm,
+template <int R1, int R2, int C>
+__global__ __launch_bounds__(R1 * C) void kr_pass1(const float2* __restrict__ x, float2* __restrict__ a1, SiteMap sm,
                                                const float2* __restrict__ twg, int nx, int nrows, int nat, int nat0,
                                                int compact, int cj, BatchArgs ba) {
-  constexpr int L = R1 * R2, C = RT / R1, P = C + 1;
+  constexpr int L = R1 * R2, RT = R1 * C, P = C + 1, G = Reg2<R1, R2>::G, VN = Reg2<R1, R2>::VN;
   __shared__ float2 twL[L];
-  __shared__ float2 buf[L * P];
+  __shared__ float buf[L * P];  // split exchange
   const int rb = blockIdx.z;
   if (ba.active && !ba.active[rb]) return;
   x += rb * ba.xs;
@@ -49,7 +48,7 @@ __global__ __launch_bounds__(RT) void kr_pass1(const float2* __restrict__ x, flo
   const int t = threadIdx.x % R1, s = threadIdx.x / R1;
   const int g = blockIdx.x * C + s, c = blockIdx.y;
   for (int i = threadIdx.x; i < L; i += RT) twL[i] = twg[i];
-  float2 v[R2];
+  float2 v[VN];
   const bool live = g < nrows;
   int lo = 0, q0 = 0, cnt = 0;
   if (compact && sm.contiguous && live) {
@@ -74,23 +73,24 @@ __global__ __launch_bounds__(RT) void kr_pass1(const float2* __restrict__ x, flo
     v[n2] = val;
   }
   __syncthreads();
-  fft2_fwd<R1, R2, +1>(v, buf, twL, t, s, P);
+  fft2_fwd_s<R1, R2, +1>(v, buf, twL, t, s, P);
   if (live) {
     float2* out = a1 + ((size_t)c * nrows + g) * L;
 #pragma unroll
-    for (int i = 0; i < R2 / R1; ++i)
+    for (int i = 0; i < G; ++i)
 #pragma unroll
-      for (int k1 = 0; k1 < R1; ++k1) out[(t + R1 * i) + R2 * k1] = v[i * R1 + k1];
+      for (int k1 = 0; k1 < R1; ++k1)
+        if (reg2_owns<R1, R2>(t, i)) out[(t + R1 * i) + R2 * k1] = v[i * R1 + k1];
   }
 }
 
 // ------------------------------------------------------------ passes 2, 4
 // in [3][nz][nin][gx] -> out [3][nz][nout][gx] along the middle axis
-template <int R1, int R2, int DIR>
-__global__ __launch_bounds__(RT) void kr_passy(const float2* __restrict__ in, float2* __restrict__ out,
+template <int R1, int R2, int C, int DIR>
+__global__ __launch_bounds__(R1 * C) void kr_passy(const float2* __restrict__ in, float2* __restrict__ out,
                                                const float2* __restrict__ twg, int gx, int nin, int nout, int nz,
                                                BatchArgs ba) {
-  constexpr int L = R1 * R2, C = RT / R1, P = C + 1;
+  constexpr int L = R1 * R2, RT = R1 * C, P = C + 1, G = Reg2<R1, R2>::G, VN = Reg2<R1, R2>::VN;
   __shared__ float2 twL[L];
   __shared__ float2 buf[L * P];
   const int rb = blockIdx.z / 3;
@@ -103,7 +103,7 @@ __global__ __launch_bounds__(RT) void kr_passy(const float2* __restrict__ in, fl
   const bool live = kx < gx;
   const float2* src = in + ((size_t)c * nz + iz) * (size_t)nin * gx + kx;
   float2* dst = out + ((size_t)c * nz + iz) * (size_t)nout * gx + kx;
-  float2 v[R2];
+  float2 v[VN];
   if (DIR > 0) {
 #pragma unroll
     for (int n2 = 0; n2 < R2; ++n2) {
@@ -114,16 +114,18 @@ __global__ __launch_bounds__(RT) void kr_passy(const float2* __restrict__ in, fl
     fft2_fwd<R1, R2, +1>(v, buf, twL, t, s, P);
     if (live) {
 #pragma unroll
-      for (int i = 0; i < R2 / R1; ++i)
+      for (int i = 0; i < G; ++i)
 #pragma unroll
-        for (int k1 = 0; k1 < R1; ++k1) dst[(size_t)((t + R1 * i) + R2 * k1) * gx] = v[i * R1 + k1];
+        for (int k1 = 0; k1 < R1; ++k1)
+          if (reg2_owns<R1, R2>(t, i)) dst[(size_t)((t + R1 * i) + R2 * k1) * gx] = v[i * R1 + k1];
     }
   } else {
 #pragma unroll
-    for (int i = 0; i < R2 / R1; ++i)
+    for (int i = 0; i < G; ++i)
 #pragma unroll
       for (int k1 = 0; k1 < R1; ++k1)
-        v[i * R1 + k1] = live ? src[(size_t)((t + R1 * i) + R2 * k1) * gx] : make_float2(0.f, 0.f);
+        v[i * R1 + k1] = (live && reg2_owns<R1, R2>(t, i)) ? src[(size_t)((t + R1 * i) + R2 * k1) * gx]
+                                                           : make_float2(0.f, 0.f);
     __syncthreads();
     fft2_inv<R1, R2, -1>(v, buf, twL, t, s, P);
     if (live) {
@@ -137,11 +139,11 @@ __global__ __launch_bounds__(RT) void kr_passy(const float2* __restrict__ in, fl
 }
 
 // ------------------------------------------------------------------ pass 3
-template <int R1, int R2>
-__global__ __launch_bounds__(RT) void kr_pass3(float2* __restrict__ a2, const float2* __restrict__ green,
+template <int R1, int R2, int C>
+__global__ __launch_bounds__(R1 * C) void kr_pass3(float2* __restrict__ a2, const float2* __restrict__ green,
                                                const float2* __restrict__ twg, int nx, int ny, int nz, int gx,
                                                int gy, BatchArgs ba) {
-  constexpr int L = R1 * R2, C = RT / R1, P = C + 1;
+  constexpr int L = R1 * R2, RT = R1 * C, P = C + 1, G = Reg2<R1, R2>::G, VN = Reg2<R1, R2>::VN;
   __shared__ float2 twL[L];
   __shared__ float2 buf[L * P];
   const int rb = blockIdx.z;
@@ -151,17 +153,17 @@ __global__ __launch_bounds__(RT) void kr_pass3(float2* __restrict__ a2, const fl
   // Grid order puts Green-sharing blocks on one XCD (workgroups go round-robin
   // over the 8 XCDs by linear id): ky runs 0, ny, 1, gy-1, 2, gy-2, ... so
   // mirror rows are one grid row apart, and column tile b and its mirror
-  // G-1-b are h = ceil(G/2) grid columns apart.
+  // NT-1-b are h = ceil(NT/2) grid columns apart (NT = column tiles).
   const int j = blockIdx.y;
   const int ky = j == 0 ? 0 : j == 1 ? ny : ((j & 1) ? gy - ((j - 2) / 2 + 1) : (j - 2) / 2 + 1);
-  const int G = gridDim.x, h = (G + 1) / 2;
-  const int tile = (int)blockIdx.x < h ? (int)blockIdx.x : G - 1 - ((int)blockIdx.x - h);
+  const int NT = gridDim.x, h = (NT + 1) / 2;
+  const int tile = (int)blockIdx.x < h ? (int)blockIdx.x : NT - 1 - ((int)blockIdx.x - h);
   const int kx = tile * C + s;
   for (int i = threadIdx.x; i < L; i += RT) twL[i] = twg[i];
   const bool live = kx < gx;
   const size_t plane = (size_t)gy * gx, cs = (size_t)nz * plane;
   float2* col = a2 + (size_t)ky * gx + kx;
-  float2 v0[R2], v1[R2], v2[R2];
+  float2 v0[VN], v1[VN], v2[VN];
 #pragma unroll
   for (int n2 = 0; n2 < R2; ++n2) {
     const int n = t + R1 * n2;
@@ -188,7 +190,8 @@ __global__ __launch_bounds__(RT) void kr_pass3(float2* __restrict__ a2, const fl
   const float sx = kx <= nx ? 1.f : -1.f, sy = ky <= ny ? 1.f : -1.f;
   const size_t pb = (size_t)ir + (size_t)ox * jr;
 #pragma unroll
-  for (int i = 0; i < R2 / R1; ++i) {
+  for (int i = 0; i < G; ++i) {
+    if (!reg2_owns<R1, R2>(t, i)) continue;
 #pragma unroll
     for (int k1 = 0; k1 < R1; ++k1) {
       const int kz = (t + R1 * i) + R2 * k1;
@@ -229,13 +232,13 @@ __global__ __launch_bounds__(RT) void kr_pass3(float2* __restrict__ a2, const fl
 }
 
 // ------------------------------------------------------------------ pass 5
-template <int R1, int R2>
-__global__ __launch_bounds__(RT) void kr_pass5(const float2* __restrict__ a1, const float2* __restrict__ twg,
+template <int R1, int R2, int C>
+__global__ __launch_bounds__(R1 * C) void kr_pass5(const float2* __restrict__ a1, const float2* __restrict__ twg,
                                                const float2* __restrict__ x, float2* __restrict__ y,
                                                const float2* __restrict__ adia, const float2* __restrict__ aoff,
                                                const int16_t* __restrict__ iocc, SiteMap sm, int nx, int nrows,
                                                int nat, int nat0, int compact, int cj, float inv, BatchArgs ba) {
-  constexpr int L = R1 * R2, C = RT / R1, P = C + 1;
+  constexpr int L = R1 * R2, RT = R1 * C, P = C + 1, G = Reg2<R1, R2>::G, VN = Reg2<R1, R2>::VN;
   __shared__ float2 twL[L];
   __shared__ float2 buf[L * P];
   const int rb = blockIdx.y;
@@ -251,16 +254,17 @@ __global__ __launch_bounds__(RT) void kr_pass5(const float2* __restrict__ a1, co
   const bool live = g < nrows;
   const size_t cs = (size_t)nrows * L;
   const float2* src = a1 + (size_t)g * L;
-  float2 v0[R2], v1[R2], v2[R2];
+  float2 v0[VN], v1[VN], v2[VN];
 #pragma unroll
-  for (int i = 0; i < R2 / R1; ++i)
+  for (int i = 0; i < G; ++i)
 #pragma unroll
     for (int k1 = 0; k1 < R1; ++k1) {
       const int k = (t + R1 * i) + R2 * k1;
       const int e = i * R1 + k1;
-      v0[e] = live ? src[k] : make_float2(0.f, 0.f);
-      v1[e] = live ? src[k + cs] : make_float2(0.f, 0.f);
-      v2[e] = live ? src[k + 2 * cs] : make_float2(0.f, 0.f);
+      const bool ok = live && reg2_owns<R1, R2>(t, i);
+      v0[e] = ok ? src[k] : make_float2(0.f, 0.f);
+      v1[e] = ok ? src[k + cs] : make_float2(0.f, 0.f);
+      v2[e] = ok ? src[k + 2 * cs] : make_float2(0.f, 0.f);
     }
   __syncthreads();
   fft2_inv<R1, R2, -1>(v0, buf, twL, t, s, P);
@@ -312,64 +316,86 @@ __global__ __launch_bounds__(RT) void kr_pass5(const float2* __restrict__ a1, co
 }
 
 // ------------------------------------------------------------------ host --
-template <int R1, int R2>
-static int launch_reg(Context& c, int cj, int compact, const float2* x, float2* y, const float2* adia,
-                      const float2* aoff, hipStream_t st, const MvBatch& bt, int which) {
+template <int R1, int R2, int C>
+static void launch_reg(Context& c, int cj, int compact, const float2* x, float2* y, const float2* adia,
+                       const float2* aoff, hipStream_t st, const MvBatch& bt, int which) {
   const MatvecPlan& m = c.mv;
-  constexpr int C = RT / R1;
+  constexpr int RT = R1 * C;
   const int nrows = c.ny * c.nz;
   const unsigned nb = (unsigned)bt.nrhs;
   SiteMap sm{c.rowptr, c.rowlo, c.qmap, c.rows_contiguous};
   switch (which) {
     case 1:
-      hipLaunchKernelGGL((kr_pass1<R1, R2>), dim3((nrows + C - 1) / C, 3, nb), dim3(RT), 0, st, x, bt.a1, sm,
+      hipLaunchKernelGGL((kr_pass1<R1, R2, C>), dim3((nrows + C - 1) / C, 3, nb), dim3(RT), 0, st, x, bt.a1, sm,
                          m.twx, c.nx, nrows, c.nat, c.nat0, compact, cj, bt.k);
       break;
     case 2:
-      hipLaunchKernelGGL((kr_passy<R1, R2, +1>), dim3((m.gx + C - 1) / C, c.nz, 3 * nb), dim3(RT), 0, st, bt.a1,
-                         bt.a2, m.twy, m.gx, c.ny, m.gy, c.nz, bt.k);
+      hipLaunchKernelGGL((kr_passy<R1, R2, C, +1>), dim3((m.gx + C - 1) / C, c.nz, 3 * nb), dim3(RT), 0, st,
+                         bt.a1, bt.a2, m.twy, m.gx, c.ny, m.gy, c.nz, bt.k);
       break;
     case 3:
-      hipLaunchKernelGGL((kr_pass3<R1, R2>), dim3((m.gx + C - 1) / C, m.gy, nb), dim3(RT), 0, st, bt.a2, c.green,
-                         m.twz, c.nx, c.ny, c.nz, m.gx, m.gy, bt.k);
+      hipLaunchKernelGGL((kr_pass3<R1, R2, C>), dim3((m.gx + C - 1) / C, m.gy, nb), dim3(RT), 0, st, bt.a2,
+                         c.green, m.twz, c.nx, c.ny, c.nz, m.gx, m.gy, bt.k);
       break;
     case 4:
-      hipLaunchKernelGGL((kr_passy<R1, R2, -1>), dim3((m.gx + C - 1) / C, c.nz, 3 * nb), dim3(RT), 0, st, bt.a2,
-                         bt.a1, m.twy, m.gx, m.gy, c.ny, c.nz, bt.k);
+      hipLaunchKernelGGL((kr_passy<R1, R2, C, -1>), dim3((m.gx + C - 1) / C, c.nz, 3 * nb), dim3(RT), 0, st,
+                         bt.a2, bt.a1, m.twy, m.gx, m.gy, c.ny, c.nz, bt.k);
       break;
     default:
-      hipLaunchKernelGGL((kr_pass5<R1, R2>), dim3((nrows + C - 1) / C, nb), dim3(RT), 0, st, bt.a1, m.twx, x, y,
-                         adia, aoff, compact ? nullptr : c.iocc, sm, c.nx, nrows, c.nat, c.nat0, compact, cj,
+      hipLaunchKernelGGL((kr_pass5<R1, R2, C>), dim3((nrows + C - 1) / C, nb), dim3(RT), 0, st, bt.a1, m.twx, x,
+                         y, adia, aoff, compact ? nullptr : c.iocc, sm, c.nx, nrows, c.nat, c.nat0, compact, cj,
                          1.f / (float)m.ngrid, bt.k);
       break;
   }
-  return 0;
 }
 
-// axis length -> register FFT shape; 0 if not covered
-int reg_shape(int L) {
-  switch (L) {
-    case 64: return 1;
-    case 128: return 2;
-    case 256: return 3;
-    default: return 0;
-  }
+using RegLaunch = void (*)(Context&, int, int, const float2*, float2*, const float2*, const float2*, hipStream_t,
+                           const MvBatch&, int);
+struct RegShape {
+  int L, r1, r2, c;
+  RegLaunch fn;
+};
+#define DDH_SHAPE(L_, R1_, R2_, C_) RegShape{L_, R1_, R2_, C_, &launch_reg<R1_, R2_, C_>}
+// One register shape per padded axis length 2N (L = R1 R2, R1 threads per
+// sequence, C sequences per workgroup of R1 C threads).  Shapes keep a
+// thread's register array at <= 16 complex values so the three-component
+// passes (3 and 5) stay free of spills; lengths without such a split (162,
+// 200, 216, primes of 7 ...) use the LDS-staged kernels.
+// The first shape of a length is its default; further ones are autotuner
+// candidates.
+static const RegShape kShapes[] = {
+    DDH_SHAPE(32, 4, 8, 64),    DDH_SHAPE(48, 8, 6, 32),    DDH_SHAPE(60, 10, 6, 32),  DDH_SHAPE(64, 8, 8, 32),
+    DDH_SHAPE(64, 4, 16, 64),   DDH_SHAPE(72, 12, 6, 16),   DDH_SHAPE(80, 10, 8, 32),  DDH_SHAPE(96, 12, 8, 16),
+    DDH_SHAPE(96, 8, 12, 32),   DDH_SHAPE(100, 10, 10, 32), DDH_SHAPE(108, 12, 9, 16), DDH_SHAPE(120, 12, 10, 16),
+    DDH_SHAPE(120, 8, 15, 32),  DDH_SHAPE(128, 8, 16, 32),  DDH_SHAPE(128, 16, 8, 16), DDH_SHAPE(144, 12, 12, 16),
+    DDH_SHAPE(144, 16, 9, 16),  DDH_SHAPE(150, 15, 10, 16), DDH_SHAPE(160, 16, 10, 16), DDH_SHAPE(180, 15, 12, 16),
+    DDH_SHAPE(192, 16, 12, 16), DDH_SHAPE(240, 16, 15, 16), DDH_SHAPE(256, 16, 16, 16),
+};
+#undef DDH_SHAPE
+
+// shape `cand` (0 = default) of length L, or nullptr
+static const RegShape* find_shape(int L, int cand) {
+  for (const RegShape& r : kShapes)
+    if (r.L == L && cand-- == 0) return &r;
+  return nullptr;
 }
 
+int reg_candidates(int L) {
+  int n = 0;
+  for (const RegShape& r : kShapes) n += r.L == L;
+  return n;
+}
 
-
-// launches register pass `which` (1..5) if its axis length is covered; returns
-// 1 when launched, 0 when the caller should use the generic kernel
-int reg_pass(Context& c, int which, int cj, int compact, const float2* x, float2* y, const float2* adia,
+// launches register pass `which` (1..5) with shape `cand` of its axis length;
+// returns 1 when launched, 0 when the caller should use the generic kernel
+int reg_pass(Context& c, int which, int cand, int cj, int compact, const float2* x, float2* y, const float2* adia,
              const float2* aoff, hipStream_t st, const MvBatch& bt) {
   const MatvecPlan& m = c.mv;
   const int L = (which == 1 || which == 5) ? m.gx : (which == 3 ? m.gz : m.gy);
-  switch (reg_shape(L)) {
-    case 1: launch_reg<8, 8>(c, cj, compact, x, y, adia, aoff, st, bt, which); return 1;
-    case 2: launch_reg<8, 16>(c, cj, compact, x, y, adia, aoff, st, bt, which); return 1;
-    case 3: launch_reg<16, 16>(c, cj, compact, x, y, adia, aoff, st, bt, which); return 1;
-    default: return 0;
-  }
+  const RegShape* r = cand >= 0 ? find_shape(L, cand) : nullptr;
+  if (!r) return 0;
+  r->fn(c, cj, compact, x, y, adia, aoff, st, bt, which);
+  return 1;
 }
 
 }  // namespace ddhip

@@ -18,6 +18,10 @@ struct MatvecPlan {
   float2 *twx = nullptr, *twy = nullptr, *twz = nullptr;  // device twiddle tables
   // tile parameters
   int rx_fwd = 1, rx_inv = 1, cy = 1, cz = 1;
+  // FFT kernel per pass 1..5: index into the register shapes of that pass's
+  // axis length, or -1 for the LDS-staged kernel (chosen by the autotuner)
+  int kind[6] = {-1, -1, -1, -1, -1, -1};
+  int tuned = 0;
 };
 
 struct Context {
@@ -40,6 +44,7 @@ struct Context {
   int* qmap = nullptr;         // [nat] full site -> compact index (-1 void); built when rows are not intervals
   int rows_contiguous = 0;     // every row's occupied sites form one interval
   int fft_generic = 0;         // 1: force the LDS-staged generic FFT kernels
+  int autotune = 1;            // 1: time the FFT kernel candidates per pass at prepare
   // full-layout operator and vector staging
   float2 *adia = nullptr, *aoff = nullptr;      // [3*nat]
   float2 *adia_c = nullptr, *aoff_c = nullptr;  // [3*nat0]
@@ -86,9 +91,13 @@ void matvec_release(Context& c);
 // FFT work elements per RHS: a1 [3][nz][ny][gx], a2 [3][nz][gy][gx]
 size_t matvec_a1_count(const Context& c);
 size_t matvec_a2_count(const Context& c);
-// register-FFT pass `which` (1..5) when its axis length is covered; 1 = launched
-int reg_pass(Context& c, int which, int cj, int compact, const float2* x, float2* y, const float2* adia,
+// register-FFT shapes available for a padded axis length (0: none)
+int reg_candidates(int L);
+// register-FFT pass `which` (1..5) with shape `cand` of its axis length; 1 = launched
+int reg_pass(Context& c, int which, int cand, int cj, int compact, const float2* x, float2* y, const float2* adia,
              const float2* aoff, hipStream_t st, const MvBatch& bt);
+// times every FFT kernel candidate of each pass and stores the fastest in c.mv.kind
+int matvec_autotune(Context& c);
 
 // Green tensor build on device (IPBC=0), result in c.green.
 int green_build(Context& c, int idipint, double akd, const double dx[3]);

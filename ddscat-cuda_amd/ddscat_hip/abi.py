@@ -30,13 +30,15 @@ EXTENSION_SYMBOLS = [
     "ddscat_hip_get_green", "ddscat_hip_info", "ddscat_hip_stream", "ddscat_hip_sync",
     "ddscat_hip_time_matvec", "ddscat_hip_set_fft_kernels", "ddscat_hip_solve_device",
     "ddscat_hip_solve_batch", "ddscat_hip_apply_batch", "ddscat_hip_evalq_batch",
-    "ddscat_hip_cabs_adjoint", "ddscat_hip_setup_rhs",
+    "ddscat_hip_cabs_adjoint", "ddscat_hip_setup_rhs", "ddscat_hip_set_autotune",
+    "ddscat_hip_nearfield",
 ]
 # method ids of ddscat_hip_solve_device
 SOLVER_IDS = {"pbcgs2": 0, "qmrccg": 1, "petrkp": 2, "gpbicg": 3, "pbcgst": 4, "bicgstab4": 5,
               "gpbicgstab2": 6, "gpbicgstab4": 7}
 INFO_FIELDS = ("nx", "ny", "nz", "nat", "nat0", "green_count", "matvec_count", "backend", "precision",
-               "gx", "gy", "gz", "rx_fwd", "cy", "cz", "rx_inv", "green_build_us")
+               "gx", "gy", "gz", "rx_fwd", "cy", "cz", "rx_inv", "green_build_us",
+               "kernel_pass1", "kernel_pass2", "kernel_pass3", "kernel_pass4", "kernel_pass5", "autotuned")
 
 BACKEND_FFT3D, BACKEND_SLICES = 0, 1
 PREC_MIXED, PREC_FP32 = 0, 1
@@ -123,6 +125,10 @@ class Library:
         L.ddscat_hip_cabs_adjoint.restype = i
         L.ddscat_hip_setup_rhs.argtypes = [i, vp, vp, vp, vp, vp, i, vp, vp, vp]
         L.ddscat_hip_setup_rhs.restype = i
+        L.ddscat_hip_set_autotune.argtypes = [i]
+        L.ddscat_hip_set_autotune.restype = i
+        L.ddscat_hip_nearfield.argtypes = [vp, vp, vp]
+        L.ddscat_hip_nearfield.restype = i
         self.nat = self.nat0 = 0
 
     # ---------------------------------------------------------------- errors
@@ -308,6 +314,14 @@ class Library:
     def set_fft_kernels(self, generic):
         """0: register-resident FFTs where available (default); 1: LDS-staged everywhere."""
         return self._check("set_fft_kernels", self.lib.ddscat_hip_set_fft_kernels(int(generic)))
+
+    def nearfield(self, p_ptr, out_ptr, einc_ptr=None):
+        """E_inc + E_sca (or E_sca) on every lattice site, full [3][NAT] device array."""
+        return self._check("nearfield", self.lib.ddscat_hip_nearfield(
+            C.c_void_p(p_ptr), C.c_void_p(einc_ptr) if einc_ptr else None, C.c_void_p(out_ptr)))
+
+    def set_autotune(self, on):
+        return self._check("set_autotune", self.lib.ddscat_hip_set_autotune(1 if on else 0))
 
     def apply_device(self, cwhat, x_ptr, y_ptr, adia_ptr, aoff_ptr, layout=0, stream=None):
         cw = ord(cwhat) if isinstance(cwhat, str) else int(cwhat)

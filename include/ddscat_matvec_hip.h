@@ -117,6 +117,12 @@ DDSCAT_HIP_API int ddscat_hip_set_precision(int precision);
  * (default); 1 = LDS-staged Stockham for every axis. */
 DDSCAT_HIP_API int ddscat_hip_set_fft_kernels(int generic);
 
+/* FFT kernel autotuning at prepare (FFT3D backend): 1 (default) times every
+ * candidate kernel of each MATVEC pass (register shapes of the pass's axis
+ * length and the LDS-staged kernel) and keeps the fastest, cached per grid
+ * for the process; 0 uses each length's default register shape. */
+DDSCAT_HIP_API int ddscat_hip_set_autotune(int on);
+
 /* y = A x on DEVICE pointers, stream-ordered on `stream` (NULL = library
  * stream), no synchronisation.  layout 0: full [3][NAT] vectors and
  * operators; layout 1: compact [3][NAT0] vectors and compact operators. */
@@ -193,12 +199,22 @@ DDSCAT_HIP_API int ddscat_hip_setup_rhs(int nrhs, const float *ak, const float *
                                         const float *dx, const void *eps, int eps_per_comp, void *b,
                                         void *adia, void *aoff);
 
+/* Near field on every lattice site of the prepared grid (DDSCAT NRFLD):
+ * out_full [3][NAT] (DEVICE) = einc_full + E_sca, or E_sca alone when
+ * einc_full is NULL, where E_sca,j = sum_k G_jk P_k is radiated by the compact
+ * polarization p_compact [3][NAT0] (DEVICE); computed with the MATVEC's FFT
+ * convolution and no occupancy mask.  FFT3D backend; blocks until done.
+ * Codes 10 not prepared, 11 argument / backend, 12-13 launch or copy. */
+DDSCAT_HIP_API int ddscat_hip_nearfield(const void *p_compact, const void *einc_full, void *out_full);
+
 /* Copy the resident spectral Green tensor (ddscat_hip_info field 5 complex64
  * values) to host memory. */
 DDSCAT_HIP_API int ddscat_hip_get_green(void *out);
 
 /* Fill up to n int64 fields: nx ny nz nat nat0 green_count matvec_count
- * backend precision gx gy gz rx_fwd cy cz rx_inv green_build_us. */
+ * backend precision gx gy gz rx_fwd cy cz rx_inv green_build_us, then the FFT
+ * kernel of passes 1..5 (-1 LDS-staged, k >= 0 register shape k of the axis
+ * length) and whether the autotuner chose them. */
 DDSCAT_HIP_API int ddscat_hip_info(long long *out, int n);
 
 /* Run `reps` MATVECs on device pointers with HIP events between the five
