@@ -182,9 +182,28 @@ __device__ __forceinline__ bool reg2_owns(int t, int i) {
   return Reg2<R1, R2>::G * R1 == R2 || t + R1 * i < R2;
 }
 
+// LDS exchange index of element (k2, n1) of sequence s.
+//  LAY 0 (column passes, lanes run over s): (k2 R1 + n1) P + s, P = C + 1.
+//  LAY 1 (row passes, lanes run over n1 when writing and over k2 when
+//  reading): s P + k2 (R1 + 1) + n1 with P >= R2 (R1 + 1): consecutive n1 and
+//  k2 strides of R1 + 1 both hit distinct banks.
+template <int LAY, int R1>
+__device__ __forceinline__ int lidx(int k2, int n1, int s, int P) {
+  return LAY == 0 ? (k2 * R1 + n1) * P + s : s * P + k2 * (R1 + 1) + n1;
+}
+// per-sequence stride for LAY 1: R2 (R1 + 1) rounded up so that two
+// sequences of one 32-lane half-wave fall on disjoint 4-byte banks (odd multiple of 16)
+template <int R1, int R2>
+__host__ __device__ constexpr int row_stride() {
+  const int base = R2 * (R1 + 1);
+  int s = (base + 15) / 16 * 16;
+  if ((s / 16) % 2 == 0) s += 16;
+  return s;
+}
+
 // forward: v[n2] = x[t + R1 n2] (n2 < R2) in; v[i R1 + k1] = X[(t + R1 i) + R2 k1]
 // out for the groups i the thread owns
-template <int R1, int R2, int DIR>
+template <int R1, int R2, int DIR, int LAY = 0>
 __device__ __forceinline__ void fft2_fwd(float2* v, float2* __restrict__ lds, const float2* __restrict__ twL, int t,
                                          int s, int P) {
   dft_reg<R2, DIR>(v);
@@ -192,7 +211,7 @@ __device__ __forceinline__ void fft2_fwd(float2* v, float2* __restrict__ lds, co
   for (int k2 = 0; k2 < R2; ++k2) {
     float2 w = twL[t * k2];
     if (DIR < 0) w.y = -w.y;
-    lds[(k2 * R1 + t) * P + s] = cmul(v[k2], w);
+    lds[lidx<LAY, R1>(k2, t, s, P)] = cmul(v[k2], w);
   }
   __syncthreads();
 #pragma unroll
@@ -201,7 +220,7 @@ __device__ __forceinline__ void fft2_fwd(float2* v, float2* __restrict__ lds, co
     const int k2 = t + R1 * i;
     float2 u[R1];
 #pragma unroll
-    for (int n1 = 0; n1 < R1; ++n1) u[n1] = lds[(k2 * R1 + n1) * P + s];
+    for (int n1 = 0; n1 < R1; ++n1) u[n1] = lds[lidx<LAY, R1>(k2, n1, s, P)];
     dft_reg<R1, DIR>(u);
 #pragma unroll
     for (int k1 = 0; k1 < R1; ++k1) v[i * R1 + k1] = u[k1];
@@ -210,7 +229,7 @@ __device__ __forceinline__ void fft2_fwd(float2* v, float2* __restrict__ lds, co
 
 // inverse of the forward distribution: v[i R1 + k1] = Y[(t + R1 i) + R2 k1] in
 // (owned groups), v[n2] = y[t + R1 n2] out (direction DIR, normally -1)
-template <int R1, int R2, int DIR>
+template <int R1, int R2, int DIR, int LAY = 0>
 __device__ __forceinline__ void fft2_inv(float2* v, float2* __restrict__ lds, const float2* __restrict__ twL, int t,
                                          int s, int P) {
 #pragma unroll
@@ -225,12 +244,12 @@ __device__ __forceinline__ void fft2_inv(float2* v, float2* __restrict__ lds, co
     for (int n1 = 0; n1 < R1; ++n1) {
       float2 w = twL[n1 * k2];
       if (DIR < 0) w.y = -w.y;
-      lds[(k2 * R1 + n1) * P + s] = cmul(u[n1], w);
+      lds[lidx<LAY, R1>(k2, n1, s, P)] = cmul(u[n1], w);
     }
   }
   __syncthreads();
 #pragma unroll
-  for (int k2 = 0; k2 < R2; ++k2) v[k2] = lds[(k2 * R1 + t) * P + s];
+  for (int k2 = 0; k2 < R2; ++k2) v[k2] = lds[lidx<LAY, R1>(k2, t, s, P)];
   dft_reg<R2, DIR>(v);
 }
 
@@ -238,7 +257,7 @@ __device__ __forceinline__ void fft2_inv(float2* v, float2* __restrict__ lds, co
 // LDS in turn, so the exchange buffer holds L*P floats instead of L*P
 // float2.  Half the LDS per workgroup doubles the workgroups a CU can hold
 // for kernels whose occupancy LDS limits (pass 1); two more barriers.
-template <int R1, int R2, int DIR>
+template <int R1, int R2, int DIR, int LAY = 0>
 __device__ __forceinline__ void fft2_fwd_s(float2* v, float* __restrict__ lds, const float2* __restrict__ twL, int t,
                                            int s, int P) {
   constexpr int G = Reg2<R1, R2>::G;
@@ -251,28 +270,114 @@ __device__ __forceinline__ void fft2_fwd_s(float2* v, float* __restrict__ lds, c
   }
   float re[G * R1];
 #pragma unroll
-  for (int k2 = 0; k2 < R2; ++k2) lds[(k2 * R1 + t) * P + s] = v[k2].x;
+  for (int k2 = 0; k2 < R2; ++k2) lds[lidx<LAY, R1>(k2, t, s, P)] = v[k2].x;
   __syncthreads();
 #pragma unroll
   for (int i = 0; i < G; ++i) {
     if (!reg2_owns<R1, R2>(t, i)) continue;
 #pragma unroll
-    for (int n1 = 0; n1 < R1; ++n1) re[i * R1 + n1] = lds[((t + R1 * i) * R1 + n1) * P + s];
+    for (int n1 = 0; n1 < R1; ++n1) re[i * R1 + n1] = lds[lidx<LAY, R1>(t + R1 * i, n1, s, P)];
   }
   __syncthreads();
 #pragma unroll
-  for (int k2 = 0; k2 < R2; ++k2) lds[(k2 * R1 + t) * P + s] = v[k2].y;
+  for (int k2 = 0; k2 < R2; ++k2) lds[lidx<LAY, R1>(k2, t, s, P)] = v[k2].y;
   __syncthreads();
 #pragma unroll
   for (int i = 0; i < G; ++i) {
     if (!reg2_owns<R1, R2>(t, i)) continue;
     float2 u[R1];
 #pragma unroll
-    for (int n1 = 0; n1 < R1; ++n1) u[n1] = make_float2(re[i * R1 + n1], lds[((t + R1 * i) * R1 + n1) * P + s]);
+    for (int n1 = 0; n1 < R1; ++n1) u[n1] = make_float2(re[i * R1 + n1], lds[lidx<LAY, R1>(t + R1 * i, n1, s, P)]);
     dft_reg<R1, DIR>(u);
 #pragma unroll
     for (int k1 = 0; k1 < R1; ++k1) v[i * R1 + k1] = u[k1];
   }
+}
+
+// ---- lane-pair split DFTs: a Q-point DFT (Q even) held by two adjacent
+// lanes (h = lane & 1), Q/2 values each; one radix-2 step crosses the pair
+// through a lane shuffle.
+// DIT form: a[j] = x[h + 2j] in, a[j] = X[(Q/2) h + j] out.
+template <int Q, int DIR>
+__device__ __forceinline__ void dit_pair(float2* a, int h) {
+  constexpr int H = Q / 2;
+  dft_reg<H, DIR>(a);
+  const float sg = h ? -1.f : 1.f;
+#pragma unroll
+  for (int m = 0; m < H; ++m) {
+    const float2 tw = twc<Q, DIR>(a[m], m);
+    const float2 z = h ? tw : a[m];  // lane 0: Y0[m], lane 1: w^m Y1[m]
+    const float2 b = make_float2(__shfl_xor(z.x, 1), __shfl_xor(z.y, 1));
+    a[m] = make_float2(b.x + sg * z.x, b.y + sg * z.y);  // Y0 + w Y1 | Y0 - w Y1
+  }
+}
+// DIF form: a[j] = x[(Q/2) h + j] in, a[j] = X[h + 2j] out.
+template <int Q, int DIR>
+__device__ __forceinline__ void dif_pair(float2* a, int h) {
+  constexpr int H = Q / 2;
+#pragma unroll
+  for (int j = 0; j < H; ++j) {
+    const float2 b = make_float2(__shfl_xor(a[j].x, 1), __shfl_xor(a[j].y, 1));
+    const float2 sum = make_float2(a[j].x + b.x, a[j].y + b.y);         // lane 0: x[j] + x[j+H]
+    const float2 dif = twc<Q, DIR>(make_float2(b.x - a[j].x, b.y - a[j].y), j);  // lane 1: (x[j] - x[j+H]) w^j
+    a[j] = h ? dif : sum;
+  }
+  dft_reg<H, DIR>(a);
+}
+
+// LDS index for the lane-pair transform of L = Q^2 points over C columns:
+// element (k2, n1) of column s; the two halves of k2 sit side by side in a
+// padded row of P2 slots so that a lane pair (h = 0, 1) hits disjoint banks.
+template <int Q>
+__device__ __forceinline__ int pidx(int k2, int n1, int s, int C, int P2) {
+  constexpr int H = Q / 2;
+  return ((k2 % H) * Q + n1) * P2 + (k2 / H) * C + s;
+}
+__host__ __device__ constexpr int pair_stride(int C) {
+  int p = (2 * C + 15) / 16 * 16;
+  if ((p / 16) % 2 == 0) p += 16;
+  return p;
+}
+
+// forward two-level transform with lane pairs (virtual thread t of Q per
+// column, lane half h): v[j] = x[t + Q (h + 2j)] in,
+// v[j] = X[t + Q ((Q/2) h + j)] out
+template <int Q, int DIR>
+__device__ __forceinline__ void fft2p_fwd(float2* v, float2* __restrict__ lds, const float2* __restrict__ twL, int t,
+                                          int h, int s, int C, int P2) {
+  constexpr int H = Q / 2;
+  dit_pair<Q, DIR>(v, h);
+#pragma unroll
+  for (int j = 0; j < H; ++j) {
+    const int k2 = H * h + j;
+    float2 w = twL[t * k2];
+    if (DIR < 0) w.y = -w.y;
+    lds[pidx<Q>(k2, t, s, C, P2)] = cmul(v[j], w);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < H; ++j) v[j] = lds[pidx<Q>(t, h + 2 * j, s, C, P2)];
+  dit_pair<Q, DIR>(v, h);
+}
+
+// inverse of fft2p_fwd's distribution: v[j] = Y[t + Q ((Q/2) h + j)] in,
+// v[j] = y[t + Q (h + 2j)] out
+template <int Q, int DIR>
+__device__ __forceinline__ void fft2p_inv(float2* v, float2* __restrict__ lds, const float2* __restrict__ twL, int t,
+                                          int h, int s, int C, int P2) {
+  constexpr int H = Q / 2;
+  dif_pair<Q, DIR>(v, h);
+#pragma unroll
+  for (int j = 0; j < H; ++j) {
+    const int n1 = h + 2 * j;
+    float2 w = twL[n1 * t];
+    if (DIR < 0) w.y = -w.y;
+    lds[pidx<Q>(t, n1, s, C, P2)] = cmul(v[j], w);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < H; ++j) v[j] = lds[pidx<Q>(H * h + j, t, s, C, P2)];
+  dif_pair<Q, DIR>(v, h);
 }
 
 }  // namespace ddhip

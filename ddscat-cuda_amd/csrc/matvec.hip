@@ -85,7 +85,7 @@ __global__ __launch_bounds__(MV_THREADS) void k_pass1_xfwd(
 template <int DIR, int PER>
 __global__ __launch_bounds__(MV_THREADS) void k_pass_y(
     const float2* __restrict__ in, float2* __restrict__ out, const float2* __restrict__ twg,
-    FftPlan p, int gx, int nin, int nout, int nz, int cy_log2, BatchArgs ba) {
+    FftPlan p, int gx, int nin, int nout, int nz, int cy_log2, size_t ins, size_t outs, BatchArgs ba) {
   extern __shared__ float2 sm[];
   const int rb = blockIdx.z / 3;
   if (ba.active && !ba.active[rb]) return;
@@ -98,7 +98,7 @@ __global__ __launch_bounds__(MV_THREADS) void k_pass_y(
   const int c = blockIdx.z % 3, iz = blockIdx.y, kx0 = blockIdx.x * cy;
   const int T = blockDim.x;
   for (int i = threadIdx.x; i < L; i += T) tw[i] = twg[i];
-  const float2* src = in + ((size_t)c * nz + iz) * (size_t)nin * gx;
+  const float2* src = in + ((size_t)c * nz + iz) * ins;
   for (int idx = threadIdx.x; idx < L * cy; idx += T) {
     const int e = idx >> cy_log2, col = idx & (cy - 1);
     const int kx = kx0 + col;
@@ -108,7 +108,7 @@ __global__ __launch_bounds__(MV_THREADS) void k_pass_y(
   }
   __syncthreads();
   fft_run<DIR, true, PER>(buf, tw, p, cy, cy, 1);
-  float2* dst = out + ((size_t)c * nz + iz) * (size_t)nout * gx;
+  float2* dst = out + ((size_t)c * nz + iz) * outs;
   for (int idx = threadIdx.x; idx < nout * cy; idx += T) {
     const int e = idx >> cy_log2, col = idx & (cy - 1);
     const int kx = kx0 + col;
@@ -120,7 +120,7 @@ __global__ __launch_bounds__(MV_THREADS) void k_pass_y(
 template <int PER>
 __global__ __launch_bounds__(MV_THREADS) void k_pass3_z_green(
     float2* __restrict__ a2, const float2* __restrict__ green, const float2* __restrict__ twg,
-    FftPlan p, int nx, int ny, int nz, int gx, int gy, int cz_log2, BatchArgs ba) {
+    FftPlan p, int nx, int ny, int nz, int gx, int gy, int cz_log2, size_t plane, BatchArgs ba) {
   extern __shared__ float2 sm[];
   const int rb = blockIdx.z;
   if (ba.active && !ba.active[rb]) return;
@@ -133,7 +133,6 @@ __global__ __launch_bounds__(MV_THREADS) void k_pass3_z_green(
   float2* b2 = b1 + L * cz;
   const int ky = blockIdx.y, kx0 = blockIdx.x * cz;
   const int T = blockDim.x;
-  const size_t plane = (size_t)gy * gx;
   const size_t cstride = (size_t)nz * plane;
   for (int i = threadIdx.x; i < L; i += T) tw[i] = twg[i];
   for (int idx = threadIdx.x; idx < L * cz; idx += T) {
@@ -341,14 +340,16 @@ int matvec_setup(Context& c) {
   m.cy = std::min(16, pow2floor(std::max(1, wide / m.gy)));
   m.cz = std::min(16, pow2floor(std::max(1, narrow / m.gz)));
   if (upload_twiddles(m.gx, &m.twx) || upload_twiddles(m.gy, &m.twy) || upload_twiddles(m.gz, &m.twz)) return 206;
-  const size_t n1 = (size_t)3 * c.nz * c.ny * m.gx, n2 = (size_t)3 * c.nz * m.gy * m.gx;
+  m.p1 = (size_t)c.ny * m.gx;
+  m.p2 = (size_t)m.gy * m.gx;
+  const size_t n1 = (size_t)3 * c.nz * m.p1, n2 = (size_t)3 * c.nz * m.p2;
   DDH_TRY(hipMalloc((void**)&c.a1, n1 * sizeof(float2)), "malloc FFT work a1", 204);
   DDH_TRY(hipMalloc((void**)&c.a2, n2 * sizeof(float2)), "malloc FFT work a2", 204);
   return 0;
 }
 
 size_t matvec_a1_count(const Context& c) { return (size_t)3 * c.nz * c.ny * c.mv.gx; }
-size_t matvec_a2_count(const Context& c) { return (size_t)3 * c.nz * c.mv.gy * c.mv.gx; }
+size_t matvec_a2_count(const Context& c) { return (size_t)3 * c.nz * c.mv.p2; }
 
 void matvec_release(Context& c) {
   for (float2** p : {&c.mv.twx, &c.mv.twy, &c.mv.twz, &c.a1, &c.a2}) {
@@ -378,7 +379,7 @@ static int launch_pass(Context& c, int which, int kind, int cj, int compact, con
       dim3 grid((m.gx + m.cy - 1) / m.cy, c.nz, 3 * nb);
       size_t lds = (size_t)(m.gy + m.cy * m.gy) * sizeof(float2);
       hipLaunchKernelGGL((k_pass_y<+1, PER_WIDE>), grid, dim3(MV_THREADS), lds, st, a1, a2, m.twy, m.py, m.gx,
-                         c.ny, m.gy, c.nz, ilog2(m.cy), bt.k);
+                         c.ny, m.gy, c.nz, ilog2(m.cy), m.p1, m.p2, bt.k);
       DDH_LAUNCH_OK("MATVEC pass 2 (y FFT)", 22);
       break;
     }
@@ -386,7 +387,7 @@ static int launch_pass(Context& c, int which, int kind, int cj, int compact, con
       dim3 grid((m.gx + m.cz - 1) / m.cz, m.gy, nb);
       size_t lds = (size_t)(m.gz + 3 * m.cz * m.gz) * sizeof(float2);
       hipLaunchKernelGGL(k_pass3_z_green<PER_NARROW>, grid, dim3(MV_THREADS), lds, st, a2, c.green, m.twz, m.pz,
-                         c.nx, c.ny, c.nz, m.gx, m.gy, ilog2(m.cz), bt.k);
+                         c.nx, c.ny, c.nz, m.gx, m.gy, ilog2(m.cz), m.p2, bt.k);
       DDH_LAUNCH_OK("MATVEC pass 3 (z FFT + Green)", 23);
       break;
     }
@@ -394,7 +395,7 @@ static int launch_pass(Context& c, int which, int kind, int cj, int compact, con
       dim3 grid((m.gx + m.cy - 1) / m.cy, c.nz, 3 * nb);
       size_t lds = (size_t)(m.gy + m.cy * m.gy) * sizeof(float2);
       hipLaunchKernelGGL((k_pass_y<-1, PER_WIDE>), grid, dim3(MV_THREADS), lds, st, a2, a1, m.twy, m.py, m.gx,
-                         m.gy, c.ny, c.nz, ilog2(m.cy), bt.k);
+                         m.gy, c.ny, c.nz, ilog2(m.cy), m.p2, m.p1, bt.k);
       DDH_LAUNCH_OK("MATVEC pass 4 (y inverse FFT)", 24);
       break;
     }

@@ -38,9 +38,9 @@ template <int R1, int R2, int C>
 __global__ __launch_bounds__(R1 * C) void kr_pass1(const float2* __restrict__ x, float2* __restrict__ a1, SiteMap sm,
                                                const float2* __restrict__ twg, int nx, int nrows, int nat, int nat0,
                                                int compact, int cj, BatchArgs ba) {
-  constexpr int L = R1 * R2, RT = R1 * C, P = C + 1, G = Reg2<R1, R2>::G, VN = Reg2<R1, R2>::VN;
+  constexpr int L = R1 * R2, RT = R1 * C, P = row_stride<R1, R2>(), G = Reg2<R1, R2>::G, VN = Reg2<R1, R2>::VN;
   __shared__ float2 twL[L];
-  __shared__ float buf[L * P];  // split exchange
+  __shared__ float buf[C * P];  // split exchange, row layout
   const int rb = blockIdx.z;
   if (ba.active && !ba.active[rb]) return;
   x += rb * ba.xs;
@@ -73,7 +73,7 @@ __global__ __launch_bounds__(R1 * C) void kr_pass1(const float2* __restrict__ x,
     v[n2] = val;
   }
   __syncthreads();
-  fft2_fwd_s<R1, R2, +1>(v, buf, twL, t, s, P);
+  fft2_fwd_s<R1, R2, +1, 1>(v, buf, twL, t, s, P);
   if (live) {
     float2* out = a1 + ((size_t)c * nrows + g) * L;
 #pragma unroll
@@ -89,7 +89,7 @@ __global__ __launch_bounds__(R1 * C) void kr_pass1(const float2* __restrict__ x,
 template <int R1, int R2, int C, int DIR>
 __global__ __launch_bounds__(R1 * C) void kr_passy(const float2* __restrict__ in, float2* __restrict__ out,
                                                const float2* __restrict__ twg, int gx, int nin, int nout, int nz,
-                                               BatchArgs ba) {
+                                               size_t ins, size_t outs, BatchArgs ba) {
   constexpr int L = R1 * R2, RT = R1 * C, P = C + 1, G = Reg2<R1, R2>::G, VN = Reg2<R1, R2>::VN;
   __shared__ float2 twL[L];
   __shared__ float2 buf[L * P];
@@ -101,8 +101,8 @@ __global__ __launch_bounds__(R1 * C) void kr_passy(const float2* __restrict__ in
   const int kx = blockIdx.x * C + s, iz = blockIdx.y, c = blockIdx.z % 3;
   for (int i = threadIdx.x; i < L; i += RT) twL[i] = twg[i];
   const bool live = kx < gx;
-  const float2* src = in + ((size_t)c * nz + iz) * (size_t)nin * gx + kx;
-  float2* dst = out + ((size_t)c * nz + iz) * (size_t)nout * gx + kx;
+  const float2* src = in + ((size_t)c * nz + iz) * ins + kx;
+  float2* dst = out + ((size_t)c * nz + iz) * outs + kx;
   float2 v[VN];
   if (DIR > 0) {
 #pragma unroll
@@ -142,7 +142,7 @@ __global__ __launch_bounds__(R1 * C) void kr_passy(const float2* __restrict__ in
 template <int R1, int R2, int C>
 __global__ __launch_bounds__(R1 * C) void kr_pass3(float2* __restrict__ a2, const float2* __restrict__ green,
                                                const float2* __restrict__ twg, int nx, int ny, int nz, int gx,
-                                               int gy, BatchArgs ba) {
+                                               int gy, size_t plane, BatchArgs ba) {
   constexpr int L = R1 * R2, RT = R1 * C, P = C + 1, G = Reg2<R1, R2>::G, VN = Reg2<R1, R2>::VN;
   __shared__ float2 twL[L];
   __shared__ float2 buf[L * P];
@@ -161,7 +161,7 @@ __global__ __launch_bounds__(R1 * C) void kr_pass3(float2* __restrict__ a2, cons
   const int kx = tile * C + s;
   for (int i = threadIdx.x; i < L; i += RT) twL[i] = twg[i];
   const bool live = kx < gx;
-  const size_t plane = (size_t)gy * gx, cs = (size_t)nz * plane;
+  const size_t cs = (size_t)nz * plane;
   float2* col = a2 + (size_t)ky * gx + kx;
   float2 v0[VN], v1[VN], v2[VN];
 #pragma unroll
@@ -231,6 +231,185 @@ __global__ __launch_bounds__(R1 * C) void kr_pass3(float2* __restrict__ a2, cons
   }
 }
 
+// Pass 3 with the mirror rows ky and gy - ky in the two halves of one
+// workgroup: both halves read the same Green octant entries at the same
+// time, so one of each pair of loads is served by the CU's L1.
+template <int R1, int R2, int C>
+__global__ __launch_bounds__(2 * R1 * C) void kr_pass3y(float2* __restrict__ a2, const float2* __restrict__ green,
+                                               const float2* __restrict__ twg, int nx, int ny, int nz, int gx,
+                                               int gy, size_t plane, BatchArgs ba) {
+  constexpr int L = R1 * R2, RT = 2 * R1 * C, P = C + 1, G = Reg2<R1, R2>::G, VN = Reg2<R1, R2>::VN;
+  __shared__ float2 twL[L];
+  __shared__ float2 bufs[2 * L * P];
+  const int rb = blockIdx.z;
+  if (ba.active && !ba.active[rb]) return;
+  a2 += rb * ba.a2s;
+  const int hp = threadIdx.x / (R1 * C), lt = threadIdx.x % (R1 * C);
+  const int s = lt % C, t = lt / C;
+  float2* buf = bufs + hp * L * P;
+  // Grid order puts Green-sharing blocks on one XCD (workgroups go round-robin
+  // over the 8 XCDs by linear id): ky runs 0, ny, 1, gy-1, 2, gy-2, ... so
+  // mirror rows are one grid row apart, and column tile b and its mirror
+  // NT-1-b are h = ceil(NT/2) grid columns apart (NT = column tiles).
+  // grid row j holds ky = j and gy - j (j = 0: the self-mirror rows 0 and ny)
+  const int j = blockIdx.y;
+  const int ky = j == 0 ? (hp ? ny : 0) : (hp ? gy - j : j);
+  const int NT = gridDim.x, h = (NT + 1) / 2;
+  const int tile = (int)blockIdx.x < h ? (int)blockIdx.x : NT - 1 - ((int)blockIdx.x - h);
+  const int kx = tile * C + s;
+  for (int i = threadIdx.x; i < L; i += RT) twL[i] = twg[i];
+  const bool live = kx < gx;
+  const size_t cs = (size_t)nz * plane;
+  float2* col = a2 + (size_t)ky * gx + kx;
+  float2 v0[VN], v1[VN], v2[VN];
+#pragma unroll
+  for (int n2 = 0; n2 < R2; ++n2) {
+    const int n = t + R1 * n2;
+    const bool ok = live && n < nz;
+    const size_t o = (size_t)n * plane;
+    v0[n2] = ok ? col[o] : make_float2(0.f, 0.f);
+    v1[n2] = ok ? col[o + cs] : make_float2(0.f, 0.f);
+    v2[n2] = ok ? col[o + 2 * cs] : make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  fft2_fwd<R1, R2, +1>(v0, buf, twL, t, s, P);
+  __syncthreads();
+  fft2_fwd<R1, R2, +1>(v1, buf, twL, t, s, P);
+  __syncthreads();
+  fft2_fwd<R1, R2, +1>(v2, buf, twL, t, s, P);
+  // 3x3 symmetric Green multiply with octant parity reconstruction (cu:109-115).
+  // Each octant entry serves up to eight (kx, ky, kz) mirrors; kz mirrors
+  // are in this block and the kx / ky mirror blocks run on the same XCD at
+  // about the same time (grid mapping above), so repeats hit in L2.
+  const int ox = nx + 1, oy = ny + 1;
+  const int ir = live ? min(kx, gx - kx) : 0;
+  const int jr = min(ky, gy - ky);
+  const float sxy = ((kx <= nx) == (ky <= ny)) ? 1.f : -1.f;
+  const float sx = kx <= nx ? 1.f : -1.f, sy = ky <= ny ? 1.f : -1.f;
+  const size_t pb = (size_t)ir + (size_t)ox * jr;
+#pragma unroll
+  for (int i = 0; i < G; ++i) {
+    if (!reg2_owns<R1, R2>(t, i)) continue;
+#pragma unroll
+    for (int k1 = 0; k1 < R1; ++k1) {
+      const int kz = (t + R1 * i) + R2 * k1;
+      const int kr = min(kz, L - kz);
+      const float sz = kz <= nz ? 1.f : -1.f;
+      const Green6 G = green_at(green, pb + (size_t)ox * oy * kr);
+      const float2 g11 = G.xx;
+      const float2 g12 = cscale(G.xy, sxy);
+      const float2 g13 = cscale(G.xz, sx * sz);
+      const float2 g22 = G.yy;
+      const float2 g23 = cscale(G.yz, sy * sz);
+      const float2 g33 = G.zz;
+      const int e = i * R1 + k1;
+      const float2 p0 = v0[e], p1 = v1[e], p2 = v2[e];
+      v0[e] = cadd(cadd(cmul(g11, p0), cmul(g12, p1)), cmul(g13, p2));
+      v1[e] = cadd(cadd(cmul(g12, p0), cmul(g22, p1)), cmul(g23, p2));
+      v2[e] = cadd(cadd(cmul(g13, p0), cmul(g23, p1)), cmul(g33, p2));
+    }
+  }
+  __syncthreads();
+  fft2_inv<R1, R2, -1>(v0, buf, twL, t, s, P);
+  __syncthreads();
+  fft2_inv<R1, R2, -1>(v1, buf, twL, t, s, P);
+  __syncthreads();
+  fft2_inv<R1, R2, -1>(v2, buf, twL, t, s, P);
+  if (live) {
+#pragma unroll
+    for (int n2 = 0; n2 < R2; ++n2) {
+      const int n = t + R1 * n2;
+      if (n < nz) {
+        const size_t o = (size_t)n * plane;
+        col[o] = v0[n2];
+        col[o + cs] = v1[n2];
+        col[o + 2 * cs] = v2[n2];
+      }
+    }
+  }
+}
+
+// Pass 3 with lane pairs (L = Q^2): each column's Q virtual threads are lane
+// pairs holding Q/2 values per component, half the registers of kr_pass3, so
+// more waves fit a SIMD to hide the load / Green-read latency.
+template <int Q, int C>
+__global__ __launch_bounds__(2 * Q * C) void kr_pass3p(float2* __restrict__ a2, const float2* __restrict__ green,
+                                                    const float2* __restrict__ twg, int nx, int ny, int nz, int gx,
+                                                    int gy, size_t plane, BatchArgs ba) {
+  constexpr int L = Q * Q, H = Q / 2, RT = 2 * Q * C, P2 = pair_stride(C);
+  __shared__ float2 twL[L];
+  __shared__ float2 buf[H * Q * P2];
+  const int rb = blockIdx.z;
+  if (ba.active && !ba.active[rb]) return;
+  a2 += rb * ba.a2s;
+  const int h = threadIdx.x & 1, s = (threadIdx.x >> 1) % C, t = (threadIdx.x >> 1) / C;
+  // XCD-aware grid order as in kr_pass3
+  const int j = blockIdx.y;
+  const int ky = j == 0 ? 0 : j == 1 ? ny : ((j & 1) ? gy - ((j - 2) / 2 + 1) : (j - 2) / 2 + 1);
+  const int NT = gridDim.x, hx = (NT + 1) / 2;
+  const int tile = (int)blockIdx.x < hx ? (int)blockIdx.x : NT - 1 - ((int)blockIdx.x - hx);
+  const int kx = tile * C + s;
+  for (int i = threadIdx.x; i < L; i += RT) twL[i] = twg[i];
+  const bool live = kx < gx;
+  const size_t cs = (size_t)nz * plane;
+  float2* col = a2 + (size_t)ky * gx + kx;
+  float2 v0[H], v1[H], v2[H];
+#pragma unroll
+  for (int i = 0; i < H; ++i) {
+    const int n = t + Q * (h + 2 * i);
+    const bool ok = live && n < nz;
+    const size_t o = (size_t)n * plane;
+    v0[i] = ok ? col[o] : make_float2(0.f, 0.f);
+    v1[i] = ok ? col[o + cs] : make_float2(0.f, 0.f);
+    v2[i] = ok ? col[o + 2 * cs] : make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  fft2p_fwd<Q, +1>(v0, buf, twL, t, h, s, C, P2);
+  __syncthreads();
+  fft2p_fwd<Q, +1>(v1, buf, twL, t, h, s, C, P2);
+  __syncthreads();
+  fft2p_fwd<Q, +1>(v2, buf, twL, t, h, s, C, P2);
+  // 3x3 symmetric Green multiply with octant parity reconstruction (cu:109-115)
+  const int ox = nx + 1, oy = ny + 1;
+  const int ir = live ? min(kx, gx - kx) : 0;
+  const int jr = min(ky, gy - ky);
+  const float sxy = ((kx <= nx) == (ky <= ny)) ? 1.f : -1.f;
+  const float sx = kx <= nx ? 1.f : -1.f, sy = ky <= ny ? 1.f : -1.f;
+  const size_t pb = (size_t)ir + (size_t)ox * jr;
+#pragma unroll
+  for (int i = 0; i < H; ++i) {
+    const int kz = t + Q * (H * h + i);
+    const int kr = min(kz, L - kz);
+    const float sz = kz <= nz ? 1.f : -1.f;
+    const Green6 G = green_at(green, pb + (size_t)ox * oy * kr);
+    const float2 g12 = cscale(G.xy, sxy);
+    const float2 g13 = cscale(G.xz, sx * sz);
+    const float2 g23 = cscale(G.yz, sy * sz);
+    const float2 p0 = v0[i], p1 = v1[i], p2 = v2[i];
+    v0[i] = cadd(cadd(cmul(G.xx, p0), cmul(g12, p1)), cmul(g13, p2));
+    v1[i] = cadd(cadd(cmul(g12, p0), cmul(G.yy, p1)), cmul(g23, p2));
+    v2[i] = cadd(cadd(cmul(g13, p0), cmul(g23, p1)), cmul(G.zz, p2));
+  }
+  __syncthreads();
+  fft2p_inv<Q, -1>(v0, buf, twL, t, h, s, C, P2);
+  __syncthreads();
+  fft2p_inv<Q, -1>(v1, buf, twL, t, h, s, C, P2);
+  __syncthreads();
+  fft2p_inv<Q, -1>(v2, buf, twL, t, h, s, C, P2);
+  if (live) {
+#pragma unroll
+    for (int i = 0; i < H; ++i) {
+      const int n = t + Q * (h + 2 * i);
+      if (n < nz) {
+        const size_t o = (size_t)n * plane;
+        col[o] = v0[i];
+        col[o + cs] = v1[i];
+        col[o + 2 * cs] = v2[i];
+      }
+    }
+  }
+}
+
 // ------------------------------------------------------------------ pass 5
 template <int R1, int R2, int C>
 __global__ __launch_bounds__(R1 * C) void kr_pass5(const float2* __restrict__ a1, const float2* __restrict__ twg,
@@ -238,9 +417,9 @@ __global__ __launch_bounds__(R1 * C) void kr_pass5(const float2* __restrict__ a1
                                                const float2* __restrict__ adia, const float2* __restrict__ aoff,
                                                const int16_t* __restrict__ iocc, SiteMap sm, int nx, int nrows,
                                                int nat, int nat0, int compact, int cj, float inv, BatchArgs ba) {
-  constexpr int L = R1 * R2, RT = R1 * C, P = C + 1, G = Reg2<R1, R2>::G, VN = Reg2<R1, R2>::VN;
+  constexpr int L = R1 * R2, RT = R1 * C, P = R2 * (R1 + 1), G = Reg2<R1, R2>::G, VN = Reg2<R1, R2>::VN;
   __shared__ float2 twL[L];
-  __shared__ float2 buf[L * P];
+  __shared__ float2 buf[C * P];  // row layout
   const int rb = blockIdx.y;
   if (ba.active && !ba.active[rb]) return;
   a1 += rb * ba.a1s;
@@ -267,11 +446,11 @@ __global__ __launch_bounds__(R1 * C) void kr_pass5(const float2* __restrict__ a1
       v2[e] = ok ? src[k + 2 * cs] : make_float2(0.f, 0.f);
     }
   __syncthreads();
-  fft2_inv<R1, R2, -1>(v0, buf, twL, t, s, P);
+  fft2_inv<R1, R2, -1, 1>(v0, buf, twL, t, s, P);
   __syncthreads();
-  fft2_inv<R1, R2, -1>(v1, buf, twL, t, s, P);
+  fft2_inv<R1, R2, -1, 1>(v1, buf, twL, t, s, P);
   __syncthreads();
-  fft2_inv<R1, R2, -1>(v2, buf, twL, t, s, P);
+  fft2_inv<R1, R2, -1, 1>(v2, buf, twL, t, s, P);
   if (!live) return;
   int lo = 0, q0 = 0, cnt = 0;
   if (compact && sm.contiguous) {
@@ -331,15 +510,15 @@ static void launch_reg(Context& c, int cj, int compact, const float2* x, float2*
       break;
     case 2:
       hipLaunchKernelGGL((kr_passy<R1, R2, C, +1>), dim3((m.gx + C - 1) / C, c.nz, 3 * nb), dim3(RT), 0, st,
-                         bt.a1, bt.a2, m.twy, m.gx, c.ny, m.gy, c.nz, bt.k);
+                         bt.a1, bt.a2, m.twy, m.gx, c.ny, m.gy, c.nz, m.p1, m.p2, bt.k);
       break;
     case 3:
       hipLaunchKernelGGL((kr_pass3<R1, R2, C>), dim3((m.gx + C - 1) / C, m.gy, nb), dim3(RT), 0, st, bt.a2,
-                         c.green, m.twz, c.nx, c.ny, c.nz, m.gx, m.gy, bt.k);
+                         c.green, m.twz, c.nx, c.ny, c.nz, m.gx, m.gy, m.p2, bt.k);
       break;
     case 4:
       hipLaunchKernelGGL((kr_passy<R1, R2, C, -1>), dim3((m.gx + C - 1) / C, c.nz, 3 * nb), dim3(RT), 0, st,
-                         bt.a2, bt.a1, m.twy, m.gx, m.gy, c.ny, c.nz, bt.k);
+                         bt.a2, bt.a1, m.twy, m.gx, m.gy, c.ny, c.nz, m.p2, m.p1, bt.k);
       break;
     default:
       hipLaunchKernelGGL((kr_pass5<R1, R2, C>), dim3((nrows + C - 1) / C, nb), dim3(RT), 0, st, bt.a1, m.twx, x,
@@ -347,6 +526,32 @@ static void launch_reg(Context& c, int cj, int compact, const float2* x, float2*
                          1.f / (float)m.ngrid, bt.k);
       break;
   }
+}
+
+// shape whose pass 3 uses the lane-pair kernel; other passes as launch_reg<Q, Q, CR>
+template <int Q, int CR, int CP>
+static void launch_pair(Context& c, int cj, int compact, const float2* x, float2* y, const float2* adia,
+                        const float2* aoff, hipStream_t st, const MvBatch& bt, int which) {
+  if (which != 3) {
+    launch_reg<Q, Q, CR>(c, cj, compact, x, y, adia, aoff, st, bt, which);
+    return;
+  }
+  const MatvecPlan& m = c.mv;
+  hipLaunchKernelGGL((kr_pass3p<Q, CP>), dim3((m.gx + CP - 1) / CP, m.gy, (unsigned)bt.nrhs), dim3(2 * Q * CP), 0,
+                     st, bt.a2, c.green, m.twz, c.nx, c.ny, c.nz, m.gx, m.gy, m.p2, bt.k);
+}
+
+// shape whose pass 3 pairs mirror rows in one workgroup; other passes as launch_reg<R1, R2, C>
+template <int R1, int R2, int C>
+static void launch_ypair(Context& c, int cj, int compact, const float2* x, float2* y, const float2* adia,
+                         const float2* aoff, hipStream_t st, const MvBatch& bt, int which) {
+  if (which != 3) {
+    launch_reg<R1, R2, C>(c, cj, compact, x, y, adia, aoff, st, bt, which);
+    return;
+  }
+  const MatvecPlan& m = c.mv;
+  hipLaunchKernelGGL((kr_pass3y<R1, R2, C>), dim3((m.gx + C - 1) / C, c.ny, (unsigned)bt.nrhs), dim3(2 * R1 * C), 0,
+                     st, bt.a2, c.green, m.twz, c.nx, c.ny, c.nz, m.gx, m.gy, m.p2, bt.k);
 }
 
 using RegLaunch = void (*)(Context&, int, int, const float2*, float2*, const float2*, const float2*, hipStream_t,
@@ -371,11 +576,22 @@ static const RegShape kShapes[] = {
     DDH_SHAPE(144, 16, 9, 16),  DDH_SHAPE(150, 15, 10, 16), DDH_SHAPE(160, 16, 10, 16), DDH_SHAPE(180, 15, 12, 16),
     DDH_SHAPE(192, 16, 12, 16), DDH_SHAPE(240, 16, 15, 16), DDH_SHAPE(256, 16, 16, 16),
 };
+#define DDH_PAIR(L_, Q_, CR_, CP_) RegShape{L_, Q_, Q_, CR_, &launch_pair<Q_, CR_, CP_>}
+// lane-pair pass-3 candidates (listed after the defaults of their lengths)
+static const RegShape kPairShapes[] = {
+    DDH_PAIR(64, 8, 32, 32), DDH_PAIR(144, 12, 16, 16), DDH_PAIR(256, 16, 16, 16), DDH_PAIR(256, 16, 16, 8),
+    DDH_SHAPE(256, 16, 16, 32),
+    RegShape{256, 16, 16, 16, &launch_ypair<16, 16, 16>}, RegShape{128, 8, 16, 32, &launch_ypair<8, 16, 32>},
+    RegShape{64, 8, 8, 32, &launch_ypair<8, 8, 32>},
+};
+#undef DDH_PAIR
 #undef DDH_SHAPE
 
 // shape `cand` (0 = default) of length L, or nullptr
 static const RegShape* find_shape(int L, int cand) {
   for (const RegShape& r : kShapes)
+    if (r.L == L && cand-- == 0) return &r;
+  for (const RegShape& r : kPairShapes)
     if (r.L == L && cand-- == 0) return &r;
   return nullptr;
 }
@@ -383,6 +599,7 @@ static const RegShape* find_shape(int L, int cand) {
 int reg_candidates(int L) {
   int n = 0;
   for (const RegShape& r : kShapes) n += r.L == L;
+  for (const RegShape& r : kPairShapes) n += r.L == L;
   return n;
 }
 

@@ -31,7 +31,7 @@ EXTENSION_SYMBOLS = [
     "ddscat_hip_time_matvec", "ddscat_hip_set_fft_kernels", "ddscat_hip_solve_device",
     "ddscat_hip_solve_batch", "ddscat_hip_apply_batch", "ddscat_hip_evalq_batch",
     "ddscat_hip_cabs_adjoint", "ddscat_hip_setup_rhs", "ddscat_hip_set_autotune",
-    "ddscat_hip_nearfield",
+    "ddscat_hip_nearfield", "ddscat_hip_set_pass_kernel",
 ]
 # method ids of ddscat_hip_solve_device
 SOLVER_IDS = {"pbcgs2": 0, "qmrccg": 1, "petrkp": 2, "gpbicg": 3, "pbcgst": 4, "bicgstab4": 5,
@@ -129,6 +129,8 @@ class Library:
         L.ddscat_hip_set_autotune.restype = i
         L.ddscat_hip_nearfield.argtypes = [vp, vp, vp]
         L.ddscat_hip_nearfield.restype = i
+        L.ddscat_hip_set_pass_kernel.argtypes = [i, i]
+        L.ddscat_hip_set_pass_kernel.restype = i
         self.nat = self.nat0 = 0
 
     # ---------------------------------------------------------------- errors
@@ -319,6 +321,12 @@ class Library:
         """E_inc + E_sca (or E_sca) on every lattice site, full [3][NAT] device array."""
         return self._check("nearfield", self.lib.ddscat_hip_nearfield(
             C.c_void_p(p_ptr), C.c_void_p(einc_ptr) if einc_ptr else None, C.c_void_p(out_ptr)))
+
+    def set_pass_kernel(self, pass_, kind, check=True):
+        rc = self.lib.ddscat_hip_set_pass_kernel(int(pass_), int(kind))
+        if check:
+            self._check("set_pass_kernel", rc)
+        return rc
 
     def set_autotune(self, on):
         return self._check("set_autotune", self.lib.ddscat_hip_set_autotune(1 if on else 0))

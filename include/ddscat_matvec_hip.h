@@ -123,6 +123,12 @@ DDSCAT_HIP_API int ddscat_hip_set_fft_kernels(int generic);
  * for the process; 0 uses each length's default register shape. */
 DDSCAT_HIP_API int ddscat_hip_set_autotune(int on);
 
+/* Override the FFT kernel of MATVEC pass 1..5 of the prepared grid: kind -1
+ * = LDS-staged, k >= 0 = register candidate k of the pass's axis length
+ * (ddscat_hip_info reports the current choice).  For testing and manual
+ * tuning; the next prepare re-tunes.  Codes 10 not prepared, 11 no such kernel. */
+DDSCAT_HIP_API int ddscat_hip_set_pass_kernel(int pass, int kind);
+
 /* y = A x on DEVICE pointers, stream-ordered on `stream` (NULL = library
  * stream), no synchronisation.  layout 0: full [3][NAT] vectors and
  * operators; layout 1: compact [3][NAT0] vectors and compact operators. */

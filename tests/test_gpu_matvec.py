@@ -263,3 +263,32 @@ def test_nearfield_matches_oracle_on_all_sites(lib):
     assert err < MATVEC_TOL and err_void < MATVEC_TOL
     lib.nearfield(pd.data_ptr(), out.data_ptr())
     assert rel_l2(out.cpu().numpy(), esca) < MATVEC_TOL
+
+
+@pytest.mark.parametrize("dims", [(128, 8, 6), (6, 128, 4), (4, 6, 128), (32, 32, 32), (72, 6, 72), (20, 30, 40)])
+def test_every_kernel_candidate_matches_oracle(lib, dims):
+    """Each FFT kernel the autotuner may pick (every register shape of the
+    pass's axis length, the lane-pair pass 3, and the LDS-staged kernel),
+    forced pass by pass, gives the oracle MATVEC."""
+    import torch
+    nx, ny, nz = dims
+    rng = np.random.default_rng(nx + 3 * ny + 7 * nz)
+    iocc, x, adia, aoff = random_case(rng, nx, ny, nz)
+    akd = 0.29
+    lib.prepare_green(nx, ny, nz, iocc, (akd, 0.0, 0.0))
+    goct = O.green_octant(nx, ny, nz, np.float64(np.float32(akd)))
+    yref = O.matvec_fft("N", x, adia, aoff, goct, nx, ny, nz, iocc)
+    tuned = lib.info()
+    base = [tuned[f"kernel_pass{p}"] for p in range(1, 6)]
+    checked = 0
+    for p in range(1, 6):
+        kind = -1
+        while lib.set_pass_kernel(p, kind, check=False) == 0:
+            y = lib.apply("N", x, adia, aoff)
+            err = rel_l2(y, yref)
+            assert err < MATVEC_TOL, (dims, p, kind, err)
+            checked += 1
+            kind += 1
+        lib.set_pass_kernel(p, base[p - 1])
+    print(f"{dims}: {checked} kernel choices checked, tuned {base}")
+    assert checked >= 6  # at least one register choice besides the LDS kernels
