@@ -26,7 +26,7 @@ namespace ddhip {
 namespace {
 
 constexpr int BT = 256;             // threads per workgroup
-constexpr int BMAX_BLOCKS = 256;    // reduction partial blocks per RHS
+constexpr int BMAX_BLOCKS = 2048;   // reduction partial blocks per RHS
 constexpr int SLOT_BNORM = SL_R + 12;  // |b| of the RHS
 constexpr int SLOT_R2 = SL_D + 20;     // |r|^2 dot slot of the fused update kernels
 
@@ -135,13 +135,13 @@ __device__ __forceinline__ Acc wsum(Acc v) {
 }
 
 // Writes this workgroup's partial sums re[t], im[t] (t < nt) of RHS r.
-template <typename Acc>
+template <typename Acc, int NT>
 __device__ __forceinline__ void put_partials(const Acc* re, const Acc* im, int nt, double* __restrict__ part, int r,
                                              int nbk) {
-  __shared__ Acc sred[2 * DOT_MAX][BT / 64];
+  __shared__ Acc sred[2 * NT][BT / 64];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
-  for (int t = 0; t < DOT_MAX; ++t) {
+  for (int t = 0; t < NT; ++t) {
     if (t < nt) {
       const Acc a = wsum(re[t]);
       const Acc b = wsum(im[t]);
@@ -156,19 +156,19 @@ __device__ __forceinline__ void put_partials(const Acc* re, const Acc* im, int n
   }
 }
 
-template <typename Acc>
+template <typename Acc, int NT>
 __global__ __launch_bounds__(BT) void kb_dots(BDot L, size_t n, const int* __restrict__ active,
                                               double* __restrict__ part, int nbk) {
   const int r = blockIdx.y;
   if (active && !active[r]) return;
   const size_t off = (size_t)r * n;
-  Acc re[DOT_MAX], im[DOT_MAX];
+  Acc re[NT], im[NT];
 #pragma unroll
-  for (int t = 0; t < DOT_MAX; ++t) { re[t] = 0; im[t] = 0; }
+  for (int t = 0; t < NT; ++t) { re[t] = 0; im[t] = 0; }
   const size_t stride = (size_t)nbk * BT;
   for (size_t i = (size_t)blockIdx.x * BT + threadIdx.x; i < n; i += stride) {
 #pragma unroll
-    for (int t = 0; t < DOT_MAX; ++t) {
+    for (int t = 0; t < NT; ++t) {
       if (t < L.nt) {
         const float2 a = L.a[t][off + i];
         const int kd = L.kind[t];
@@ -192,7 +192,7 @@ __global__ __launch_bounds__(BT) void kb_dots(BDot L, size_t n, const int* __res
       }
     }
   }
-  put_partials<Acc>(re, im, L.nt, part, r, nbk);
+  put_partials<Acc, NT>(re, im, L.nt, part, r, nbk);
 }
 
 // what the finishing lane does with the reduced dots
@@ -377,7 +377,7 @@ __global__ __launch_bounds__(BT) void kb_qmr_dsxr(float2* __restrict__ d, float2
     rv[k] = rn;
     re[0] += (R)rn.x * (R)rn.x + (R)rn.y * (R)rn.y;
   }
-  put_partials<R>(re, im, 1, part, r, nbk);
+  put_partials<R, 1>(re, im, 1, part, r, nbk);
 }
 
 // PBCGS2 first half-step update (cu:466-469): x += alpha u0 ; r_j -= alpha u_{j+1}, j < K
@@ -552,7 +552,7 @@ struct Ops {
 
   Ops(Context& cc, BatchState& ss, const float2* ad, const float2* ao, size_t ostride)
       : c(cc), S(ss), st(cc.stream), f64(cc.precision != PREC_FP32), adia(ad), aoff(ao), os(ostride) {
-    nbk_red = (int)std::min<size_t>(BMAX_BLOCKS, std::max<size_t>(1, (S.n + (size_t)BT * 8 - 1) / ((size_t)BT * 8)));
+    nbk_red = (int)std::min<size_t>(BMAX_BLOCKS, std::max<size_t>(1, (S.n + (size_t)BT * 4 - 1) / ((size_t)BT * 4)));
     nbk_vec = (int)std::min<size_t>(2048, std::max<size_t>(1, (S.n + (size_t)BT * 4 - 1) / ((size_t)BT * 4)));
   }
   float2* V(int k) const { return S.W + (size_t)k * S.B * S.n; }
@@ -581,15 +581,24 @@ struct Ops {
     }
     for (int t = d.n; t < DOT_MAX; ++t) { L.a[t] = L.a[0]; L.b[t] = L.b[0]; L.c[t] = L.c[0]; L.kind[t] = DOT_N2; }
     const dim3 g((unsigned)nbk_red, (unsigned)S.B);
+#define DDH_BDOTS(ACC_, NT_) hipLaunchKernelGGL((kb_dots<ACC_, NT_>), g, dim3(BT), 0, st, L, S.n, mask, S.part, nbk_red)
+#define DDH_BDOTS_N(ACC_)                          \
+  if (d.n <= 1) DDH_BDOTS(ACC_, 1);                \
+  else if (d.n <= 2) DDH_BDOTS(ACC_, 2);           \
+  else if (d.n <= 4) DDH_BDOTS(ACC_, 4);           \
+  else if (d.n <= 6) DDH_BDOTS(ACC_, 6);           \
+  else DDH_BDOTS(ACC_, DOT_MAX);
     if (f64) {
-      hipLaunchKernelGGL(kb_dots<double>, g, dim3(BT), 0, st, L, S.n, mask, S.part, nbk_red);
+      DDH_BDOTS_N(double)
       hipLaunchKernelGGL(kb_finish<double>, dim3(S.B), dim3(BT), 0, st, S.part, nbk_red, 2 * d.n, S.scal, dst,
                          mask, op);
     } else {
-      hipLaunchKernelGGL(kb_dots<float>, g, dim3(BT), 0, st, L, S.n, mask, S.part, nbk_red);
+      DDH_BDOTS_N(float)
       hipLaunchKernelGGL(kb_finish<float>, dim3(S.B), dim3(BT), 0, st, S.part, nbk_red, 2 * d.n, S.scal, dst,
                          mask, op);
     }
+#undef DDH_BDOTS_N
+#undef DDH_BDOTS
     DDH_LAUNCH_OK("batched dots", 1);
     return 0;
   }

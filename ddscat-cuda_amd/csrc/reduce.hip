@@ -18,15 +18,16 @@ __device__ __forceinline__ Acc wave_sum(Acc v) {
   return v;
 }
 
-template <typename Acc>
+// NT: compile-time number of term slots (>= L.nt); accumulators for NT terms only
+template <typename Acc, int NT>
 __global__ __launch_bounds__(RED_THREADS) void k_dots(DotLaunch L, size_t n, double* __restrict__ part) {
-  Acc re[DOT_MAX], im[DOT_MAX];
+  Acc re[NT], im[NT];
 #pragma unroll
-  for (int t = 0; t < DOT_MAX; ++t) { re[t] = 0; im[t] = 0; }
+  for (int t = 0; t < NT; ++t) { re[t] = 0; im[t] = 0; }
   const size_t stride = (size_t)gridDim.x * RED_THREADS;
   for (size_t i = (size_t)blockIdx.x * RED_THREADS + threadIdx.x; i < n; i += stride) {
 #pragma unroll
-    for (int t = 0; t < DOT_MAX; ++t) {
+    for (int t = 0; t < NT; ++t) {
       if (t < L.nt) {
         const float2 a = L.a[t][i];
         const int kd = L.kind[t];
@@ -50,10 +51,10 @@ __global__ __launch_bounds__(RED_THREADS) void k_dots(DotLaunch L, size_t n, dou
       }
     }
   }
-  __shared__ Acc sred[2 * DOT_MAX][RED_THREADS / 64];
+  __shared__ Acc sred[2 * NT][RED_THREADS / 64];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
-  for (int t = 0; t < DOT_MAX; ++t) {
+  for (int t = 0; t < NT; ++t) {
     if (t < L.nt) {
       const Acc r = wave_sum(re[t]);
       const Acc m = wave_sum(im[t]);
@@ -96,15 +97,22 @@ int dots(Context& c, const DotSet& s, size_t n, int dst, hipStream_t st) {
     L.kind[t] = s.t[t].kind;
   }
   for (int t = s.n; t < DOT_MAX; ++t) { L.a[t] = L.a[0]; L.b[t] = L.b[0]; L.c[t] = L.c[0]; L.kind[t] = DOT_N2; }
-  size_t want = (n + (size_t)RED_THREADS * 8 - 1) / ((size_t)RED_THREADS * 8);
+  size_t want = (n + (size_t)RED_THREADS * 4 - 1) / ((size_t)RED_THREADS * 4);
   int nb = (int)std::min<size_t>(RED_MAX_BLOCKS, std::max<size_t>(1, want));
-  if (c.precision == PREC_FP32) {
-    hipLaunchKernelGGL(k_dots<float>, dim3(nb), dim3(RED_THREADS), 0, st, L, n, c.red);
+  const bool f32 = c.precision == PREC_FP32;
+#define DDH_DOTS(NT_)                                                                                    \
+  if (f32) hipLaunchKernelGGL((k_dots<float, NT_>), dim3(nb), dim3(RED_THREADS), 0, st, L, n, c.red);  \
+  else hipLaunchKernelGGL((k_dots<double, NT_>), dim3(nb), dim3(RED_THREADS), 0, st, L, n, c.red);
+  if (s.n <= 1) { DDH_DOTS(1) }
+  else if (s.n <= 2) { DDH_DOTS(2) }
+  else if (s.n <= 4) { DDH_DOTS(4) }
+  else if (s.n <= 6) { DDH_DOTS(6) }
+  else { DDH_DOTS(DOT_MAX) }
+#undef DDH_DOTS
+  if (f32)
     hipLaunchKernelGGL(k_dots_finalize<float>, dim3(1), dim3(RED_THREADS), 0, st, c.red, nb, 2 * s.n, c.scal + dst);
-  } else {
-    hipLaunchKernelGGL(k_dots<double>, dim3(nb), dim3(RED_THREADS), 0, st, L, n, c.red);
+  else
     hipLaunchKernelGGL(k_dots_finalize<double>, dim3(1), dim3(RED_THREADS), 0, st, c.red, nb, 2 * s.n, c.scal + dst);
-  }
   DDH_LAUNCH_OK("FP64 dot reduction", 1);
   return 0;
 }

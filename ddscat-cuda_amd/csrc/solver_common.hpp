@@ -12,7 +12,7 @@
 namespace ddhip {
 
 constexpr int RED_THREADS = 256;
-constexpr int RED_MAX_BLOCKS = 512;
+constexpr int RED_MAX_BLOCKS = 2048;
 constexpr int RED_MAX_TERMS = 32;   // real-valued sums per launch
 constexpr int DOT_MAX = 12;         // complex dots per launch
 constexpr int SCAL_COUNT = 512;     // doubles in the device scalar block
