@@ -39,7 +39,6 @@ extern "C" DDSCAT_HIP_API int ddscat_cuda_prepare_green_f32(int nx, int ny, int 
                                                             float aky, float akz, float dx1, float dx2, float dx3,
                                                             const int16_t* iocc) {
   clear_error();
-  (void)pyd; (void)pzd;
   if (!grid_args_ok(nx, ny, nz, nat, nat0, ipbc) || (idipint != 0 && idipint != 1) || !iocc) {
     set_error("prepare_green", "invalid argument");
     return 201;
@@ -48,24 +47,46 @@ extern "C" DDSCAT_HIP_API int ddscat_cuda_prepare_green_f32(int nx, int ny, int 
     set_error("prepare_green", "GAMMA must be >0 for PBC");
     return 202;
   }
-  if (ipbc) {
-    set_error("prepare_green", "IPBC=1 (periodic targets) is not built in this library");
-    return 209;
+  if (ipbc && !(pyd > 0.f) && !(pzd > 0.f)) {
+    // the replica window divides by the period of every periodic axis
+    set_error("prepare_green", "IPBC=1 needs PYD>0 or PZD>0");
+    return 201;
   }
+  GreenBuild gb;
+  gb.ipbc = ipbc; gb.idipint = idipint;
+  gb.gamma = gamma; gb.pyd = pyd; gb.pzd = pzd;
+  gb.ak[0] = akx; gb.ak[1] = aky; gb.ak[2] = akz;
+  gb.dx[0] = dx1; gb.dx[1] = dx2; gb.dx[2] = dx3;
   Context& c = ctx();
   release_context(c);
   if (int rc = ensure_stream(c)) return rc;
   c.nx = nx; c.ny = ny; c.nz = nz; c.ipbc = ipbc; c.nat = nat; c.nat0 = nat0;
-  c.green_count = (size_t)6 * (nx + 1) * (ny + 1) * (nz + 1);
+  if (ipbc) {
+    const double kp2 = pyd <= 0.f ? (double)akx * akx + (double)aky * aky
+                     : pzd <= 0.f ? (double)akx * akx + (double)akz * akz : (double)akx * akx;
+    if (!(kp2 > 0.0)) {
+      set_error("CUDA Green", "Green kernel status=13 (11=CISI,12=IDIPINT,13=KPERP)");
+      release_context(c);
+      return 234;
+    }
+    // bound the replica sums so that one prepare cannot occupy the GPU for long
+    const double work = green_pbc_work(c, gb), limit = 2.0e11;
+    if (work > limit) {
+      char msg[200];
+      std::snprintf(msg, sizeof(msg), "PBC replica sums need ~%.3g interactions (limit %.3g); increase GAMMA", work,
+                    limit);
+      set_error("prepare_green", msg);
+      release_context(c);
+      return 235;
+    }
+  }
+  c.green_count = 6 * green_points(nx, ny, nz, ipbc);
   if (hipMalloc((void**)&c.green, c.green_count * sizeof(float2)) != hipSuccess) {
     set_error("prepare_green", "malloc resident Green failed");
     release_context(c);
     return 204;
   }
-  const double kx = akx, ky = aky, kz = akz;
-  const double akd = std::sqrt(kx * kx + ky * ky + kz * kz);
-  const double dx[3] = {dx1, dx2, dx3};
-  int grc = green_build(c, idipint, akd, dx);
+  int grc = green_build(c, gb);
   if (grc) { release_context(c); return 220 + grc; }
   int rc = finish_prepare(c, iocc, 204, 208);
   if (rc) { release_context(c); return rc; }
@@ -80,15 +101,12 @@ extern "C" DDSCAT_HIP_API int ddscat_cuda_prepare_f32(int nx, int ny, int nz, in
     set_error("prepare", "invalid argument");
     return 1;
   }
-  if (ipbc) {
-    set_error("prepare", "IPBC=1 (periodic targets) is not built in this library");
-    return 2;
-  }
   Context& c = ctx();
   release_context(c);
   if (ensure_stream(c)) return 2;
   c.nx = nx; c.ny = ny; c.nz = nz; c.ipbc = ipbc; c.nat = nat; c.nat0 = nat0;
-  c.green_count = (size_t)6 * (nx + 1) * (ny + 1) * (nz + 1);
+  // CXZC: [6][NZ+1][NY+1][NX+1] for IPBC=0, [6][2NZ][2NY][2NX] for IPBC=1 (cu:1020)
+  c.green_count = 6 * green_points(nx, ny, nz, ipbc);
   if (hipMalloc((void**)&c.green, c.green_count * sizeof(float2)) != hipSuccess) {
     set_error("prepare", "malloc green failed");
     release_context(c);

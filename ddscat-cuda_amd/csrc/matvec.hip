@@ -120,7 +120,7 @@ __global__ __launch_bounds__(MV_THREADS) void k_pass_y(
 template <int PER>
 __global__ __launch_bounds__(MV_THREADS) void k_pass3_z_green(
     float2* __restrict__ a2, const float2* __restrict__ green, const float2* __restrict__ twg,
-    FftPlan p, int nx, int ny, int nz, int gx, int gy, int cz_log2, size_t plane, BatchArgs ba) {
+    FftPlan p, int nx, int ny, int nz, int gx, int gy, int cz_log2, size_t plane, BatchArgs ba, GreenGeo gg) {
   extern __shared__ float2 sm[];
   const int rb = blockIdx.z;
   if (ba.active && !ba.active[rb]) return;
@@ -151,17 +151,18 @@ __global__ __launch_bounds__(MV_THREADS) void k_pass3_z_green(
   fft_run<+1, true, PER>(b0, tw, p, cz, cz, 1);
   fft_run<+1, true, PER>(b1, tw, p, cz, cz, 1);
   fft_run<+1, true, PER>(b2, tw, p, cz, cz, 1);
-  // symmetric 3x3 multiply with the trimmed-octant reconstruction (cu:109-115)
-  const int ox = nx + 1, oy = ny + 1;
-  const int jr = min(ky, gy - ky);
-  const float sy = ky <= ny ? 1.f : -1.f;
+  // symmetric 3x3 multiply; trimmed-octant reconstruction for IPBC=0
+  // (cu:109-115), whole grid for IPBC=1
+  const int ox = gg.ox, oy = gg.oy;
+  const int jr = min(ky, gg.my - ky);
+  const float sy = ky <= gg.ty ? 1.f : -1.f;
   for (int idx = threadIdx.x; idx < L * cz; idx += T) {
     const int kz = idx >> cz_log2, col = idx & (cz - 1);
     const int kx = kx0 + col;
     if (kx >= gx) continue;
-    const int ir = min(kx, gx - kx), kr = min(kz, L - kz);
-    const float sx = kx <= nx ? 1.f : -1.f;
-    const float sz = kz <= nz ? 1.f : -1.f;
+    const int ir = min(kx, gg.mx - kx), kr = min(kz, gg.mz - kz);
+    const float sx = kx <= gg.tx ? 1.f : -1.f;
+    const float sz = kz <= gg.tz ? 1.f : -1.f;
     const Green6 G = green_at(green, (size_t)ir + (size_t)ox * ((size_t)jr + (size_t)oy * kr));
     const float2 g11 = G.xx;
     const float2 g12 = cscale(G.xy, sx * sy);
@@ -387,7 +388,7 @@ static int launch_pass(Context& c, int which, int kind, int cj, int compact, con
       dim3 grid((m.gx + m.cz - 1) / m.cz, m.gy, nb);
       size_t lds = (size_t)(m.gz + 3 * m.cz * m.gz) * sizeof(float2);
       hipLaunchKernelGGL(k_pass3_z_green<PER_NARROW>, grid, dim3(MV_THREADS), lds, st, a2, c.green, m.twz, m.pz,
-                         c.nx, c.ny, c.nz, m.gx, m.gy, ilog2(m.cz), m.p2, bt.k);
+                         c.nx, c.ny, c.nz, m.gx, m.gy, ilog2(m.cz), m.p2, bt.k, green_geo(c));
       DDH_LAUNCH_OK("MATVEC pass 3 (z FFT + Green)", 23);
       break;
     }

@@ -142,7 +142,7 @@ __global__ __launch_bounds__(R1 * C) void kr_passy(const float2* __restrict__ in
 template <int R1, int R2, int C>
 __global__ __launch_bounds__(R1 * C) void kr_pass3(float2* __restrict__ a2, const float2* __restrict__ green,
                                                const float2* __restrict__ twg, int nx, int ny, int nz, int gx,
-                                               int gy, size_t plane, BatchArgs ba) {
+                                               int gy, size_t plane, BatchArgs ba, GreenGeo gg) {
   constexpr int L = R1 * R2, RT = R1 * C, P = C + 1, G = Reg2<R1, R2>::G, VN = Reg2<R1, R2>::VN;
   __shared__ float2 twL[L];
   __shared__ float2 buf[L * P];
@@ -179,15 +179,15 @@ __global__ __launch_bounds__(R1 * C) void kr_pass3(float2* __restrict__ a2, cons
   fft2_fwd<R1, R2, +1>(v1, buf, twL, t, s, P);
   __syncthreads();
   fft2_fwd<R1, R2, +1>(v2, buf, twL, t, s, P);
-  // 3x3 symmetric Green multiply with octant parity reconstruction (cu:109-115).
+  // 3x3 symmetric Green multiply; octant parity reconstruction for IPBC=0 (cu:109-115), whole grid for IPBC=1.
   // Each octant entry serves up to eight (kx, ky, kz) mirrors; kz mirrors
   // are in this block and the kx / ky mirror blocks run on the same XCD at
   // about the same time (grid mapping above), so repeats hit in L2.
-  const int ox = nx + 1, oy = ny + 1;
-  const int ir = live ? min(kx, gx - kx) : 0;
-  const int jr = min(ky, gy - ky);
-  const float sxy = ((kx <= nx) == (ky <= ny)) ? 1.f : -1.f;
-  const float sx = kx <= nx ? 1.f : -1.f, sy = ky <= ny ? 1.f : -1.f;
+  const int ox = gg.ox, oy = gg.oy;
+  const int ir = live ? min(kx, gg.mx - kx) : 0;
+  const int jr = min(ky, gg.my - ky);
+  const float sxy = ((kx <= gg.tx) == (ky <= gg.ty)) ? 1.f : -1.f;
+  const float sx = kx <= gg.tx ? 1.f : -1.f, sy = ky <= gg.ty ? 1.f : -1.f;
   const size_t pb = (size_t)ir + (size_t)ox * jr;
 #pragma unroll
   for (int i = 0; i < G; ++i) {
@@ -195,8 +195,8 @@ __global__ __launch_bounds__(R1 * C) void kr_pass3(float2* __restrict__ a2, cons
 #pragma unroll
     for (int k1 = 0; k1 < R1; ++k1) {
       const int kz = (t + R1 * i) + R2 * k1;
-      const int kr = min(kz, L - kz);
-      const float sz = kz <= nz ? 1.f : -1.f;
+      const int kr = min(kz, gg.mz - kz);
+      const float sz = kz <= gg.tz ? 1.f : -1.f;
       const Green6 G = green_at(green, pb + (size_t)ox * oy * kr);
       const float2 g11 = G.xx;
       const float2 g12 = cscale(G.xy, sxy);
@@ -237,7 +237,7 @@ __global__ __launch_bounds__(R1 * C) void kr_pass3(float2* __restrict__ a2, cons
 template <int R1, int R2, int C>
 __global__ __launch_bounds__(2 * R1 * C) void kr_pass3y(float2* __restrict__ a2, const float2* __restrict__ green,
                                                const float2* __restrict__ twg, int nx, int ny, int nz, int gx,
-                                               int gy, size_t plane, BatchArgs ba) {
+                                               int gy, size_t plane, BatchArgs ba, GreenGeo gg) {
   constexpr int L = R1 * R2, RT = 2 * R1 * C, P = C + 1, G = Reg2<R1, R2>::G, VN = Reg2<R1, R2>::VN;
   __shared__ float2 twL[L];
   __shared__ float2 bufs[2 * L * P];
@@ -277,15 +277,15 @@ __global__ __launch_bounds__(2 * R1 * C) void kr_pass3y(float2* __restrict__ a2,
   fft2_fwd<R1, R2, +1>(v1, buf, twL, t, s, P);
   __syncthreads();
   fft2_fwd<R1, R2, +1>(v2, buf, twL, t, s, P);
-  // 3x3 symmetric Green multiply with octant parity reconstruction (cu:109-115).
+  // 3x3 symmetric Green multiply; octant parity reconstruction for IPBC=0 (cu:109-115), whole grid for IPBC=1.
   // Each octant entry serves up to eight (kx, ky, kz) mirrors; kz mirrors
   // are in this block and the kx / ky mirror blocks run on the same XCD at
   // about the same time (grid mapping above), so repeats hit in L2.
-  const int ox = nx + 1, oy = ny + 1;
-  const int ir = live ? min(kx, gx - kx) : 0;
-  const int jr = min(ky, gy - ky);
-  const float sxy = ((kx <= nx) == (ky <= ny)) ? 1.f : -1.f;
-  const float sx = kx <= nx ? 1.f : -1.f, sy = ky <= ny ? 1.f : -1.f;
+  const int ox = gg.ox, oy = gg.oy;
+  const int ir = live ? min(kx, gg.mx - kx) : 0;
+  const int jr = min(ky, gg.my - ky);
+  const float sxy = ((kx <= gg.tx) == (ky <= gg.ty)) ? 1.f : -1.f;
+  const float sx = kx <= gg.tx ? 1.f : -1.f, sy = ky <= gg.ty ? 1.f : -1.f;
   const size_t pb = (size_t)ir + (size_t)ox * jr;
 #pragma unroll
   for (int i = 0; i < G; ++i) {
@@ -293,8 +293,8 @@ __global__ __launch_bounds__(2 * R1 * C) void kr_pass3y(float2* __restrict__ a2,
 #pragma unroll
     for (int k1 = 0; k1 < R1; ++k1) {
       const int kz = (t + R1 * i) + R2 * k1;
-      const int kr = min(kz, L - kz);
-      const float sz = kz <= nz ? 1.f : -1.f;
+      const int kr = min(kz, gg.mz - kz);
+      const float sz = kz <= gg.tz ? 1.f : -1.f;
       const Green6 G = green_at(green, pb + (size_t)ox * oy * kr);
       const float2 g11 = G.xx;
       const float2 g12 = cscale(G.xy, sxy);
@@ -335,7 +335,7 @@ __global__ __launch_bounds__(2 * R1 * C) void kr_pass3y(float2* __restrict__ a2,
 template <int Q, int C>
 __global__ __launch_bounds__(2 * Q * C) void kr_pass3p(float2* __restrict__ a2, const float2* __restrict__ green,
                                                     const float2* __restrict__ twg, int nx, int ny, int nz, int gx,
-                                                    int gy, size_t plane, BatchArgs ba) {
+                                                    int gy, size_t plane, BatchArgs ba, GreenGeo gg) {
   constexpr int L = Q * Q, H = Q / 2, RT = 2 * Q * C, P2 = pair_stride(C);
   __shared__ float2 twL[L];
   __shared__ float2 buf[H * Q * P2];
@@ -369,18 +369,18 @@ __global__ __launch_bounds__(2 * Q * C) void kr_pass3p(float2* __restrict__ a2, 
   fft2p_fwd<Q, +1>(v1, buf, twL, t, h, s, C, P2);
   __syncthreads();
   fft2p_fwd<Q, +1>(v2, buf, twL, t, h, s, C, P2);
-  // 3x3 symmetric Green multiply with octant parity reconstruction (cu:109-115)
-  const int ox = nx + 1, oy = ny + 1;
-  const int ir = live ? min(kx, gx - kx) : 0;
-  const int jr = min(ky, gy - ky);
-  const float sxy = ((kx <= nx) == (ky <= ny)) ? 1.f : -1.f;
-  const float sx = kx <= nx ? 1.f : -1.f, sy = ky <= ny ? 1.f : -1.f;
+  // 3x3 symmetric Green multiply; octant parity reconstruction for IPBC=0 (cu:109-115), whole grid for IPBC=1
+  const int ox = gg.ox, oy = gg.oy;
+  const int ir = live ? min(kx, gg.mx - kx) : 0;
+  const int jr = min(ky, gg.my - ky);
+  const float sxy = ((kx <= gg.tx) == (ky <= gg.ty)) ? 1.f : -1.f;
+  const float sx = kx <= gg.tx ? 1.f : -1.f, sy = ky <= gg.ty ? 1.f : -1.f;
   const size_t pb = (size_t)ir + (size_t)ox * jr;
 #pragma unroll
   for (int i = 0; i < H; ++i) {
     const int kz = t + Q * (H * h + i);
-    const int kr = min(kz, L - kz);
-    const float sz = kz <= nz ? 1.f : -1.f;
+    const int kr = min(kz, gg.mz - kz);
+    const float sz = kz <= gg.tz ? 1.f : -1.f;
     const Green6 G = green_at(green, pb + (size_t)ox * oy * kr);
     const float2 g12 = cscale(G.xy, sxy);
     const float2 g13 = cscale(G.xz, sx * sz);
@@ -514,7 +514,7 @@ static void launch_reg(Context& c, int cj, int compact, const float2* x, float2*
       break;
     case 3:
       hipLaunchKernelGGL((kr_pass3<R1, R2, C>), dim3((m.gx + C - 1) / C, m.gy, nb), dim3(RT), 0, st, bt.a2,
-                         c.green, m.twz, c.nx, c.ny, c.nz, m.gx, m.gy, m.p2, bt.k);
+                         c.green, m.twz, c.nx, c.ny, c.nz, m.gx, m.gy, m.p2, bt.k, green_geo(c));
       break;
     case 4:
       hipLaunchKernelGGL((kr_passy<R1, R2, C, -1>), dim3((m.gx + C - 1) / C, c.nz, 3 * nb), dim3(RT), 0, st,
@@ -538,7 +538,7 @@ static void launch_pair(Context& c, int cj, int compact, const float2* x, float2
   }
   const MatvecPlan& m = c.mv;
   hipLaunchKernelGGL((kr_pass3p<Q, CP>), dim3((m.gx + CP - 1) / CP, m.gy, (unsigned)bt.nrhs), dim3(2 * Q * CP), 0,
-                     st, bt.a2, c.green, m.twz, c.nx, c.ny, c.nz, m.gx, m.gy, m.p2, bt.k);
+                     st, bt.a2, c.green, m.twz, c.nx, c.ny, c.nz, m.gx, m.gy, m.p2, bt.k, green_geo(c));
 }
 
 // shape whose pass 3 pairs mirror rows in one workgroup; other passes as launch_reg<R1, R2, C>
@@ -551,7 +551,7 @@ static void launch_ypair(Context& c, int cj, int compact, const float2* x, float
   }
   const MatvecPlan& m = c.mv;
   hipLaunchKernelGGL((kr_pass3y<R1, R2, C>), dim3((m.gx + C - 1) / C, c.ny, (unsigned)bt.nrhs), dim3(2 * R1 * C), 0,
-                     st, bt.a2, c.green, m.twz, c.nx, c.ny, c.nz, m.gx, m.gy, m.p2, bt.k);
+                     st, bt.a2, c.green, m.twz, c.nx, c.ny, c.nz, m.gx, m.gy, m.p2, bt.k, green_geo(c));
 }
 
 using RegLaunch = void (*)(Context&, int, int, const float2*, float2*, const float2*, const float2*, hipStream_t,

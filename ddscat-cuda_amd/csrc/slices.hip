@@ -106,10 +106,10 @@ __global__ __launch_bounds__(SL_THREADS) void k_sl_xfwd(const float2* __restrict
 // ------------------------------ y-FFT, Green multiply at kz, inverse y-FFT (in slice)
 __global__ __launch_bounds__(SL_THREADS) void k_sl_y_green(float2* __restrict__ sl, const float2* __restrict__ green,
                                                           const float2* __restrict__ twg, FftPlan p, int nx, int ny,
-                                                          int nz, int k0, int cy_log2) {
+                                                          int nz, int k0, int cy_log2, GreenGeo gg) {
   extern __shared__ float2 sm[];
   const int L = p.L;  // 2 ny
-  const int gx = 2 * nx, gz = 2 * nz;
+  const int gx = 2 * nx;
   const int cy = 1 << cy_log2;
   float2* tw = sm;
   float2* b0 = sm + L;
@@ -135,16 +135,17 @@ __global__ __launch_bounds__(SL_THREADS) void k_sl_y_green(float2* __restrict__ 
   fft_run<+1, true, SL_PER>(b0, tw, p, cy, cy, 1);
   fft_run<+1, true, SL_PER>(b1, tw, p, cy, cy, 1);
   fft_run<+1, true, SL_PER>(b2, tw, p, cy, cy, 1);
-  const int ox = nx + 1, oy = ny + 1;
-  const int kr = min(kz, gz - kz);
-  const float sz = kz <= nz ? 1.f : -1.f;
+  // octant reconstruction for IPBC=0 (cu:166-186), whole grid for IPBC=1
+  const int ox = gg.ox, oy = gg.oy;
+  const int kr = min(kz, gg.mz - kz);
+  const float sz = kz <= gg.tz ? 1.f : -1.f;
   for (int idx = threadIdx.x; idx < L * cy; idx += T) {
     const int ky = idx >> cy_log2, col = idx & (cy - 1);
     const int kx = kx0 + col;
     if (kx >= gx) continue;
-    const int ir = min(kx, gx - kx), jr = min(ky, L - ky);
-    const float sx = kx <= nx ? 1.f : -1.f;
-    const float sy = ky <= ny ? 1.f : -1.f;
+    const int ir = min(kx, gg.mx - kx), jr = min(ky, gg.my - ky);
+    const float sx = kx <= gg.tx ? 1.f : -1.f;
+    const float sy = ky <= gg.ty ? 1.f : -1.f;
     const Green6 G = green_at(green, (size_t)ir + (size_t)ox * ((size_t)jr + (size_t)oy * kr));
     const float2 g11 = G.xx;
     const float2 g12 = cscale(G.xy, sx * sy);
@@ -378,7 +379,7 @@ int slices_matvec(Context& c, int cwhat, const float2* x, float2* y, const float
       dim3 grid((m.gx + s.cy - 1) / s.cy, nb);
       const size_t lds = (size_t)(m.gy + 3 * s.cy * m.gy) * sizeof(float2);
       hipLaunchKernelGGL(k_sl_y_green, grid, dim3(SL_THREADS), lds, ss, sl, c.green, m.twy, m.py, c.nx, c.ny, c.nz,
-                         k0, lg2(s.cy));
+                         k0, lg2(s.cy), green_geo(c));
       DDH_LAUNCH_OK("SLICES y FFT + Green", 24);
     }
     {

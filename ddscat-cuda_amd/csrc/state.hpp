@@ -32,7 +32,8 @@ struct Context {
   int precision = PREC_MIXED;
   MatvecPlan mv;
   hipStream_t stream = nullptr;
-  // Green tensor (trimmed octant for IPBC=0): [6][nz+1][ny+1][nx+1] complex64
+  // Spectral Green tensor, point-interleaved complex64: the trimmed octant
+  // [nz+1][ny+1][nx+1][6] for IPBC=0, the whole grid [2nz][2ny][2nx][6] for IPBC=1
   float2* green = nullptr;
   size_t green_count = 0;
   // FFT3D work: a1 [3][nz][ny][gx], a2 [3][nz][gy][gx]
@@ -100,8 +101,19 @@ int reg_pass(Context& c, int which, int cand, int cj, int compact, const float2*
 // times every FFT kernel candidate of each pass and stores the fastest in c.mv.kind
 int matvec_autotune(Context& c);
 
-// Green tensor build on device (IPBC=0), result in c.green.
-int green_build(Context& c, int idipint, double akd, const double dx[3]);
+// Green tensor build on device, result in c.green (octant for IPBC=0, whole
+// grid for IPBC=1).  gamma, pyd, pzd and the wave-vector components matter
+// only for IPBC=1 (replica sums, cu:634-643).
+struct GreenBuild {
+  int ipbc = 0, idipint = 0;
+  double gamma = 0.0, pyd = 0.0, pzd = 0.0;
+  double ak[3] = {0.0, 0.0, 0.0};
+  double dx[3] = {1.0, 1.0, 1.0};
+};
+int green_build(Context& c, const GreenBuild& b);
+// number of direct interactions the IPBC=1 replica sums would evaluate (upper
+// bound over the lag grid); 0 for IPBC=0
+double green_pbc_work(const Context& c, const GreenBuild& b);
 
 // The resident spectral Green tensor c.green is stored point-interleaved:
 // the six components (xx, xy, xz, yy, yz, zz) of octant point
@@ -117,6 +129,29 @@ __device__ __forceinline__ Green6 green_at(const float2* __restrict__ g, size_t 
   return Green6{make_float2(a.x, a.y), make_float2(a.z, a.w), make_float2(b.x, b.y),
                 make_float2(b.z, b.w), make_float2(c.x, c.y), make_float2(c.z, c.w)};
 }
+// Where the Green entry of spectral point (kx, ky, kz) lives, and its signs.
+// IPBC=0 stores the octant and rebuilds the rest by mirror symmetry
+// (cu:109-115): index min(k, 2N - k) per axis, the off-diagonal components
+// flip sign with the mirrored axes' parities.  IPBC=1 stores the whole grid.
+// Both use the same instructions:
+//   r = min(k, m - k),  sign = (k <= t) ? +1 : -1,  point = rx + ox (ry + oy rz)
+// octant: m = 2N, t = N, ox = NX+1, oy = NY+1; full grid: m = t = 2^30
+// (no mirror, all signs +1), ox = 2NX, oy = 2NY.
+struct GreenGeo {
+  int ox, oy;
+  int mx, my, mz;
+  int tx, ty, tz;
+};
+inline GreenGeo green_geo(const Context& c) {
+  constexpr int NOMIRROR = 1 << 30;
+  if (c.ipbc) return GreenGeo{2 * c.nx, 2 * c.ny, NOMIRROR, NOMIRROR, NOMIRROR, NOMIRROR, NOMIRROR, NOMIRROR};
+  return GreenGeo{c.nx + 1, c.ny + 1, 2 * c.nx, 2 * c.ny, 2 * c.nz, c.nx, c.ny, c.nz};
+}
+// number of spectral Green points stored: octant or whole padded grid
+inline size_t green_points(int nx, int ny, int nz, int ipbc) {
+  return ipbc ? (size_t)(2 * nx) * (2 * ny) * (2 * nz) : (size_t)(nx + 1) * (ny + 1) * (nz + 1);
+}
+
 // component-major [6][npts] <-> point-interleaved [npts][6], device to device
 int green_interleave(Context& c, const float2* comp_major, float2* interleaved, hipStream_t st);
 int green_deinterleave(Context& c, const float2* interleaved, float2* comp_major, hipStream_t st);

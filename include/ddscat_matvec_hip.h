@@ -27,17 +27,21 @@ extern "C" {
 
 /* Build the spectral Green tensor on the device (FP64) and allocate the
  * MATVEC / solver state.  Replaces ddscat_matvec_cuda.h:11-13
- * (implementation ddscat_matvec_cuda.cu:1005-1017).  Codes: 201 invalid
- * argument, 202 GAMMA<=0 with PBC, 203 device setup, 204 allocation,
- * 206 unsupported grid, 208 IOCC count != NAT0, 209 IPBC=1 not built,
- * 220+g Green build failure. */
+ * (implementation ddscat_matvec_cuda.cu:1005-1017).  IPBC=1 (periodic
+ * target, period PYD*DX2 in y and/or PZD*DX3 in z) sums damped replicas per
+ * lag (cu:634-643) and keeps the whole spectral grid.  Codes: 201 invalid
+ * argument (also IPBC=1 with neither PYD>0 nor PZD>0), 202 GAMMA<=0 with PBC,
+ * 203 device setup, 204 allocation, 206 unsupported grid, 208 IOCC count !=
+ * NAT0, 220+g Green build failure (234: k has no component across the
+ * periodic axes, 235: the replica sums would exceed 2e11 interactions). */
 DDSCAT_HIP_API int ddscat_cuda_prepare_green_f32(int nx, int ny, int nz, int ipbc, int nat, int nat0,
                                                  int idipint, float gamma, float pyd, float pzd,
                                                  float akx, float aky, float akz, float dx1,
                                                  float dx2, float dx3, const int16_t *iocc);
 
 /* Same lifetime with a host-supplied transformed Green tensor CXZC
- * [6][NZ+1][NY+1][NX+1] complex64 (IPBC=0).  Replaces ddscat_matvec_cuda.h:14-15
+ * [6][NZ+1][NY+1][NX+1] complex64 (IPBC=0) or [6][2NZ][2NY][2NX] (IPBC=1).
+ * Replaces ddscat_matvec_cuda.h:14-15
  * (ddscat_matvec_cuda.cu:1018-1026).  Codes 1-6 as the reference. */
 DDSCAT_HIP_API int ddscat_cuda_prepare_f32(int nx, int ny, int nz, int ipbc, int nat, int nat0,
                                            const void *green, const int16_t *iocc);
@@ -214,7 +218,7 @@ DDSCAT_HIP_API int ddscat_hip_setup_rhs(int nrhs, const float *ak, const float *
 DDSCAT_HIP_API int ddscat_hip_nearfield(const void *p_compact, const void *einc_full, void *out_full);
 
 /* Copy the resident spectral Green tensor (ddscat_hip_info field 5 complex64
- * values) to host memory. */
+ * values) to host memory in the CXZC layout ddscat_cuda_prepare_f32 takes. */
 DDSCAT_HIP_API int ddscat_hip_get_green(void *out);
 
 /* Fill up to n int64 fields: nx ny nz nat nat0 green_count matvec_count

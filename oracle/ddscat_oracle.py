@@ -14,6 +14,11 @@ Pinning (what fixes each function independently of its own formula):
   * cisi: scipy.special.sici.
   * green_octant: brute-force spectral sums on tiny grids and the dense-matrix
     MATVEC below (test_oracle_matvec.py).
+  * green_sum_lag / green_full_pbc (IPBC=1): Bloch periodicity
+    G(l + P) = e^{i k.P} G(l), the replica window against a much wider
+    brute-force replica sum, the undamped isolated term on the stationary-phase
+    line, and the dense operator of replica sums against the full-grid FFT
+    MATVEC (test_oracle_pbc.py).
   * matvec_fft: dense 3NAT x 3NAT matrix built directly from green_elements on
     2^3..6^3 grids with holes; complex symmetry A^T = A; 'C' identity.
   * evalq / scat: the reference's own numpy identities
@@ -232,6 +237,122 @@ def green_full_from_octant(goct: np.ndarray, nx, ny, nz) -> np.ndarray:
 
 
 # ----------------------------------------------------------------------------
+# Periodic targets, IPBC=1  (cu:634-645, cu:650)
+# ----------------------------------------------------------------------------
+def pbc_window(x0, y0, z0, akvec, gamma, pyddx, pzddx):
+    """Replica window of one lag (cu:636-640): (jpbc, jpy1, jpy2, jpz1, jpz2,
+    krf, phasf).  jpbc 1: periodic in y only (pzddx <= 0), 2: in z only
+    (pyddx <= 0), 3: both.  Bounds truncate toward zero like C's (int)."""
+    akx, aky, akz = (float(v) for v in akvec)
+    akd = math.sqrt(akx * akx + aky * aky + akz * akz)
+    if pyddx <= 0.0:
+        jpbc, kp2 = 2, akx * akx + aky * aky
+    elif pzddx <= 0.0:
+        jpbc, kp2 = 1, akx * akx + akz * akz
+    else:
+        jpbc, kp2 = 3, akx * akx
+    if kp2 <= 0.0:
+        raise ValueError("k has no component across the periodic axes (status 13)")
+    kperp = math.sqrt(kp2)
+    phimax = 1.0 / gamma
+    term0 = akd * akd * phimax * phimax / kp2
+    term1 = 2.0 * akd * phimax
+    term2y = aky * phimax / kp2
+    term2z = akz * phimax / kp2
+    rperp = (math.sqrt(x0 * x0 + z0 * z0) if jpbc == 1 else
+             math.sqrt(x0 * x0 + y0 * y0) if jpbc == 2 else abs(x0))
+    term3 = math.sqrt(term0 + term1 * rperp) / kperp
+    krf = akd * akd * rperp / kperp
+    jpy1 = jpy2 = jpz1 = jpz2 = 0
+    yf = zf = 0.0
+    if jpbc != 2:
+        yf = y0 - aky * rperp / kperp
+        jpy1 = int((yf - term2y - term3) / pyddx + 0.999999)
+        jpy2 = int((yf - term2y + term3) / pyddx)
+    if jpbc != 1:
+        zf = z0 - akz * rperp / kperp
+        jpz1 = int((zf - term2z - term3) / pzddx + 0.999999)
+        jpz2 = int((zf - term2z + term3) / pzddx)
+    phasf = aky * yf if jpbc == 1 else akz * zf if jpbc == 2 else aky * yf + akz * zf
+    return jpbc, jpy1, jpy2, jpz1, jpz2, krf, phasf
+
+
+def green_sum_lag(lx, ly, lz, akvec, gamma, pyd, pzd, dx=(1.0, 1.0, 1.0), idipint=0, window=None):
+    """Six components at lag (lx, ly, lz) of a periodic target (cu:634-643):
+    sum over source replicas J = (jpy, jpz), displaced by J*(PYD*DX2*DX2,
+    PZD*DX3*DX3) as the reference displaces them, of
+        G(r_lag - J P) e^{i k.J P} e^{-beta phi^4},  beta = 16 gamma^4,
+        phi = k|r| - krf + k.J P - phasf.
+    ``window`` = (jpy1, jpy2, jpz1, jpz2) replaces the computed window (tests)."""
+    akvec = tuple(float(v) for v in akvec)
+    akd = math.sqrt(sum(v * v for v in akvec))
+    x0, y0, z0 = lx * dx[0], ly * dx[1], lz * dx[2]
+    pyddx, pzddx = pyd * dx[1], pzd * dx[2]
+    jpbc, jpy1, jpy2, jpz1, jpz2, krf, phasf = pbc_window(x0, y0, z0, akvec, gamma, pyddx, pzddx)
+    if window is not None:
+        jpy1, jpy2, jpz1, jpz2 = window
+    if jpy2 < jpy1 or jpz2 < jpz1:
+        return np.zeros(6, np.complex128)
+    JY, JZ = np.meshgrid(np.arange(jpy1, jpy2 + 1, dtype=np.float64),
+                         np.arange(jpz1, jpz2 + 1, dtype=np.float64), indexing="ij")
+    rjpy = JY * pyddx * dx[1]
+    rjpz = JZ * pzddx * dx[2]
+    y, z = y0 - rjpy, z0 - rjpz
+    kr = akd * np.sqrt(x0 * x0 + y * y + z * z)
+    phasyz = akvec[1] * rjpy + akvec[2] * rjpz
+    beta = 16.0 * gamma ** 4
+    phi = kr - krf + phasyz - phasf
+    f = np.exp(-beta * phi ** 4) * np.exp(1j * phasyz)
+    g = green_elements(np.full_like(y, x0), y, z, akd, idipint)
+    return (g * f[None]).reshape(6, -1).sum(axis=1)
+
+
+def green_lag_grid_pbc(nx, ny, nz, akvec, gamma, pyd, pzd, dx=(1.0, 1.0, 1.0), idipint=0) -> np.ndarray:
+    """Whole lag grid [6][2nz][2ny][2nx] of replica sums, Nyquist planes zero
+    (cu:645)."""
+    out = np.zeros((6, 2 * nz, 2 * ny, 2 * nx), np.complex128)
+    lag = lambda i, n: i if i < n else i - 2 * n
+    for iz in range(2 * nz):
+        for iy in range(2 * ny):
+            for ix in range(2 * nx):
+                if ix == nx or iy == ny or iz == nz:
+                    continue
+                out[:, iz, iy, ix] = green_sum_lag(lag(ix, nx), lag(iy, ny), lag(iz, nz), akvec, gamma, pyd,
+                                                   pzd, dx, idipint)
+    return out
+
+
+def green_full_pbc(nx, ny, nz, akvec, gamma, pyd, pzd, dx=(1.0, 1.0, 1.0), idipint=0) -> np.ndarray:
+    """Whole spectral Green [6][2nz][2ny][2nx] for IPBC=1 (cu:650): the lag
+    grid transformed with the +i exponent, unnormalised, nothing trimmed.
+    This is the CXZC array ddscat_cuda_prepare_f32 takes for IPBC=1 (cu:1020)."""
+    g = green_lag_grid_pbc(nx, ny, nz, akvec, gamma, pyd, pzd, dx, idipint)
+    G = (2 * nx) * (2 * ny) * (2 * nz)
+    return np.fft.ifftn(g, axes=(1, 2, 3)) * G
+
+
+def dense_green_matrix_pbc(nx, ny, nz, akvec, gamma, pyd, pzd, dx=(1.0, 1.0, 1.0), idipint=0) -> np.ndarray:
+    """Dense interaction matrix of a periodic target, Gm[c*NAT+j, d*NAT+l] =
+    G_pbc(r_j - r_l)_cd from green_sum_lag directly (no FFT).  G_pbc is not
+    even in the lag, so the orientation r_j - r_l matters."""
+    nat = nx * ny * nz
+    cache = {}
+    M = np.zeros((3 * nat, 3 * nat), np.complex128)
+    j = np.arange(nat)
+    ix, iy, iz = j % nx, (j // nx) % ny, j // (nx * ny)
+    for a in range(nat):
+        for b in range(nat):
+            key = (ix[a] - ix[b], iy[a] - iy[b], iz[a] - iz[b])
+            if key not in cache:
+                cache[key] = green_sum_lag(*key, akvec, gamma, pyd, pzd, dx, idipint)
+            g = cache[key]
+            for c, (p, q) in enumerate(COMP_IJ):
+                M[p * nat + a, q * nat + b] = g[c]
+                M[q * nat + a, p * nat + b] = g[c]
+    return M
+
+
+# ----------------------------------------------------------------------------
 # MATVEC  (cu:104-118 FFT3D; CPU semantics cp:446-607)
 # ----------------------------------------------------------------------------
 def _finish(cwhat, x3, adia3, aoff3, w3, iocc):
@@ -250,8 +371,10 @@ def _finish(cwhat, x3, adia3, aoff3, w3, iocc):
 
 
 def matvec_fft(cwhat, x, adia, aoff, goct, nx, ny, nz, iocc=None, dtype=np.complex128,
-               workers=None):
+               workers=None, ipbc=0):
     """y = A x with A = adia + offdiag - G (cu:406-415 + kernels cu:104-118).
+    goct: the trimmed octant (IPBC=0, parity reconstruction) or, with ipbc=1,
+    the whole spectral grid [6][2nz][2ny][2nx] used as it is (cu:113).
     x, adia, aoff: full component-major [3*NAT] arrays, site j = ix + nx(iy + ny iz).
     'C' applies conj(A conj(x)) (cp:519-593).  dtype complex64 is the FP32
     CPU baseline (scipy.fft with ``workers`` threads); complex128 the oracle."""
@@ -268,7 +391,7 @@ def matvec_fft(cwhat, x, adia, aoff, goct, nx, ny, nz, iocc=None, dtype=np.compl
     else:
         import scipy.fft as sfft
         wh = sfft.ifftn(w, axes=(1, 2, 3), workers=workers, overwrite_x=True) * dtype(G).real
-    gf = green_full_from_octant(goct, nx, ny, nz).astype(dtype)
+    gf = (np.asarray(goct) if ipbc else green_full_from_octant(goct, nx, ny, nz)).astype(dtype)
     p0, p1, p2 = wh[0], wh[1], wh[2]
     q = np.empty_like(wh)
     q[0] = gf[0] * p0 + gf[1] * p1 + gf[2] * p2
@@ -343,12 +466,13 @@ def dense_green_matrix(nx, ny, nz, akd, dx=(1.0, 1.0, 1.0), idipint=0) -> np.nda
     return M
 
 
-def dense_operator(nx, ny, nz, akd, adia, aoff, iocc=None, dx=(1.0, 1.0, 1.0), idipint=0):
+def dense_operator(nx, ny, nz, akd, adia, aoff, iocc=None, dx=(1.0, 1.0, 1.0), idipint=0, gm=None):
     """Full dense A = diag(adia) + offdiag(aoff) - Gm (cp:503-515), rows and
     columns of unoccupied sites zeroed (the MATVEC masks y there and x is
-    zero there in every solver vector)."""
+    zero there in every solver vector).  ``gm``: a precomputed interaction
+    matrix (e.g. dense_green_matrix_pbc) instead of the isolated one."""
     nat = nx * ny * nz
-    A = -dense_green_matrix(nx, ny, nz, akd, dx, idipint)
+    A = -(dense_green_matrix(nx, ny, nz, akd, dx, idipint) if gm is None else gm)
     adia = np.asarray(adia).reshape(3, nat)
     aoff = np.asarray(aoff).reshape(3, nat)
     idx = np.arange(nat)

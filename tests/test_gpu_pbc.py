@@ -1,0 +1,167 @@
+"""IPBC=1 (periodic targets) on the device against the oracle.
+
+The device builds the whole spectral Green grid from damped replica sums
+(cu:634-645) with a two-factor FP64 DFT per axis, and every pass-3 / SLICES
+kernel reads it without the octant mirror rules (cu:113, cu:179-181).
+
+Bars: spectral Green within 1e-6 of the oracle's (max-norm relative; the
+device rounds the FP64 result to complex64 once); MATVEC within 1e-5 relative
+L2 of the FP64 oracle (the same bar as IPBC=0); solutions within 30 tol of
+the dense direct solve of the replica-sum operator, true residual <= 2 tol.
+Wave-vector components and GAMMA are rounded to float32 for the oracle as
+the ABI rounds them."""
+import numpy as np
+import pytest
+
+from oracle import ddscat_oracle as O
+from ddscat_hip import targets as T
+
+pytestmark = pytest.mark.gpu
+
+f32 = lambda v: float(np.float32(v))
+AK = tuple(f32(v) for v in (0.3, 0.12, -0.07))
+GAMMA = f32(0.05)
+CASES = [(5.0, 0.0), (0.0, 4.0), (5.0, 4.0)]  # y-periodic, z-periodic, both
+MATVEC_TOL = 1e-5
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(np.asarray(a, complex) - b) / np.linalg.norm(b))
+
+
+def random_case(rng, nx, ny, nz, fill=0.8):
+    nat = nx * ny * nz
+    iocc = (rng.random(nat) < fill).astype(np.int16)
+    iocc[0] = 1
+    occ3 = np.tile(iocc != 0, 3)
+    cv = lambda s=1.0: s * (rng.normal(size=3 * nat) + 1j * rng.normal(size=3 * nat))
+    return iocc, cv() * occ3, cv() + 3.0, cv(0.1)
+
+
+def prepare_pbc(lib, nx, ny, nz, iocc, pyd, pzd, idipint=0):
+    lib.prepare_green(nx, ny, nz, iocc, AK, ipbc=1, idipint=idipint, gamma=GAMMA, pyd=pyd, pzd=pzd)
+
+
+@pytest.mark.parametrize("pyd,pzd", CASES)
+def test_pbc_green_matches_oracle(lib, pyd, pzd):
+    nx, ny, nz = 6, 5, 4
+    iocc = np.ones(nx * ny * nz, np.int16)
+    prepare_pbc(lib, nx, ny, nz, iocc, pyd, pzd)
+    g = lib.get_green().reshape(6, 2 * nz, 2 * ny, 2 * nx)
+    gref = O.green_full_pbc(nx, ny, nz, AK, GAMMA, pyd, pzd)
+    err = np.max(np.abs(g - gref)) / np.max(np.abs(gref))
+    print(f"PBC Green (pyd={pyd}, pzd={pzd}) {nx}x{ny}x{nz}: max rel {err:.2e}")
+    assert err < 1e-6
+    lib.release()
+
+
+def test_pbc_green_fcd_matches_oracle(lib):
+    nx, ny, nz = 3, 2, 2
+    iocc = np.ones(nx * ny * nz, np.int16)
+    prepare_pbc(lib, nx, ny, nz, iocc, 4.0, 0.0, idipint=1)
+    g = lib.get_green().reshape(6, 2 * nz, 2 * ny, 2 * nx)
+    gref = O.green_full_pbc(nx, ny, nz, AK, GAMMA, 4.0, 0.0, idipint=1)
+    err = np.max(np.abs(g - gref)) / np.max(np.abs(gref))
+    print(f"PBC FCD Green: max rel {err:.2e}")
+    assert err < 1e-6
+    lib.release()
+
+
+@pytest.mark.parametrize("backend", ["fft3d", "slices"])
+@pytest.mark.parametrize("pyd,pzd", CASES)
+def test_pbc_matvec_matches_oracle(lib, backend, pyd, pzd):
+    from ddscat_hip import BACKEND_FFT3D, BACKEND_SLICES
+    nx, ny, nz = 6, 5, 4
+    rng = np.random.default_rng(int(pyd * 10 + pzd))
+    iocc, x, adia, aoff = random_case(rng, nx, ny, nz)
+    gref = O.green_full_pbc(nx, ny, nz, AK, GAMMA, pyd, pzd)
+    lib.set_backend(BACKEND_SLICES if backend == "slices" else BACKEND_FFT3D)
+    try:
+        prepare_pbc(lib, nx, ny, nz, iocc, pyd, pzd)
+        for cw in ("N", "C"):
+            y = lib.apply(cw, x, adia, aoff)
+            yref = O.matvec_fft(cw, x, adia, aoff, gref, nx, ny, nz, iocc, ipbc=1)
+            err = rel_l2(y, yref)
+            print(f"{backend} PBC MATVEC {cw} (pyd={pyd}, pzd={pzd}): {err:.2e}")
+            assert err < MATVEC_TOL
+        # the CPU-supplied whole-grid CXZC through ddscat_cuda_prepare_f32 (cu:1018-1026)
+        lib.prepare(nx, ny, nz, gref.astype(np.complex64).reshape(-1), iocc, ipbc=1)
+        y = lib.apply("N", x, adia, aoff)
+        err = rel_l2(y, O.matvec_fft("N", x, adia, aoff, gref, nx, ny, nz, iocc, ipbc=1))
+        print(f"{backend} PBC MATVEC via prepare_f32: {err:.2e}")
+        assert err < MATVEC_TOL
+    finally:
+        lib.set_backend(BACKEND_FFT3D)
+        lib.release()
+
+
+def test_pbc_every_kernel_candidate(lib):
+    """At 16^3 (padded axes 32) the register kernels run; every pass-3
+    candidate and the LDS-staged kernel give the oracle MATVEC on the
+    device-built whole-grid Green (pinned against the oracle above)."""
+    nx = ny = nz = 16
+    rng = np.random.default_rng(16)
+    iocc, x, adia, aoff = random_case(rng, nx, ny, nz)
+    prepare_pbc(lib, nx, ny, nz, iocc, 16.0, 16.0)
+    g = lib.get_green().astype(np.complex128).reshape(6, 2 * nz, 2 * ny, 2 * nx)
+    yref = O.matvec_fft("N", x, adia, aoff, g, nx, ny, nz, iocc, ipbc=1)
+    base = lib.info()["kernel_pass3"]
+    checked = 0
+    kind = -1
+    while lib.set_pass_kernel(3, kind, check=False) == 0:
+        err = rel_l2(lib.apply("N", x, adia, aoff), yref)
+        print(f"pass-3 kernel {kind}: {err:.2e}")
+        assert err < MATVEC_TOL, (kind, err)
+        checked += 1
+        kind += 1
+    lib.set_pass_kernel(3, base)
+    assert checked >= 2
+    lib.release()
+
+
+def test_pbc_solve_matches_dense(lib):
+    nx, ny, nz = 6, 5, 4
+    rng = np.random.default_rng(5)
+    iocc = (rng.random(nx * ny * nz) < 0.85).astype(np.int16)
+    iocc[0] = 1
+    nat = nx * ny * nz
+    m = 1.33 + 0.01j
+    kd = float(np.linalg.norm(AK))
+    alpha_inv = (1.0 / T.ldr_alpha(np.array([m * m]), kd, np.asarray(AK) / kd, (0, 1, 0)))[0]
+    occ3 = np.tile(iocc != 0, 3)
+    adia = np.where(occ3, alpha_inv, 0).astype(np.complex64)
+    aoff = np.zeros(3 * nat, np.complex64)
+    b = ((rng.normal(size=3 * nat) + 1j * rng.normal(size=3 * nat)) * occ3).astype(np.complex64)
+    gm = O.dense_green_matrix_pbc(nx, ny, nz, AK, GAMMA, 5.0, 4.0)
+    A = O.dense_operator(nx, ny, nz, kd, adia.astype(complex), aoff.astype(complex), iocc, gm=gm)
+    idx = np.flatnonzero(occ3)
+    Ao = A[np.ix_(idx, idx)]
+    xref = np.linalg.solve(Ao, b.astype(complex)[idx])
+    prepare_pbc(lib, nx, ny, nz, iocc, 5.0, 4.0)
+    tol = 1e-6
+    for name in ("qmrccg", "pbcgs2"):
+        x, it, tole, nc, rc = lib.solve(name, b, adia, aoff, maxit=500, tol=tol)
+        xo = x.astype(complex)[idx]
+        true_res = np.linalg.norm(Ao @ xo - b.astype(complex)[idx]) / np.linalg.norm(b)
+        err = np.linalg.norm(xo - xref) / np.linalg.norm(xref)
+        print(f"PBC {name}: it={it} true={true_res:.2e} err={err:.2e}")
+        assert rc == 0
+        assert true_res <= 2 * tol
+        assert err <= 30 * tol
+    lib.release()
+
+
+def test_pbc_argument_errors(lib):
+    nx, ny, nz = 4, 4, 4
+    iocc = np.ones(nx * ny * nz, np.int16)
+    # GAMMA must be > 0 with PBC (cu:1007)
+    assert lib.prepare_green(nx, ny, nz, iocc, AK, ipbc=1, gamma=0.0, pyd=4.0, check=False) == 202
+    # no periodic axis
+    assert lib.prepare_green(nx, ny, nz, iocc, AK, ipbc=1, gamma=GAMMA, check=False) == 201
+    # z-periodic with k along z: no k component across the periodic axis (status 13 -> 220 + 14)
+    assert lib.prepare_green(nx, ny, nz, iocc, (0.0, 0.0, 0.3), ipbc=1, gamma=GAMMA, pzd=4.0, check=False) == 234
+    # a replica window far beyond any practical size is refused before any kernel runs
+    assert lib.prepare_green(nx, ny, nz, iocc, AK, ipbc=1, gamma=1e-7, pyd=1.0, pzd=1.0, check=False) == 235
+    # the library is usable again afterwards
+    prepare_pbc(lib, nx, ny, nz, iocc, 4.0, 0.0)
+    lib.release()
