@@ -295,6 +295,61 @@ def batched_targets(lib, n_targets=48, n_orient=8, kd=0.1, method="pbcgs2", tol=
                                       if t_batch_sample > 0 else None))
 
 
+def large_target(lib, edge=384, kd=0.25, m=1.7 + 0.05j, steps=10, warmup=2, seed=20261202):
+    """The large single target: sphere of diameter ``edge`` dipoles (384:
+    29.6 M dipoles, padded grid 768^3 x 3 complex64), m = 1.7+0.05i, kd = 0.25,
+    isotropic LDR operators.  MATVEC/s on compact device vectors for the
+    FFT3D and SLICES backends (the padded axis 768 runs the LDS-staged FFT
+    kernels), with the algorithmic bytes 696 NAT + 48 (N+1)^3 per MATVEC."""
+    import numpy as np
+    import torch
+    from ddscat_hip import targets as T, BACKEND_SLICES, BACKEND_FFT3D
+    tg = T.sphere(edge)
+    n0 = 3 * tg.nat0
+    alpha_inv = (1.0 / T.ldr_alpha(np.array([m * m]), kd, (1, 0, 0), (0, 1, 0)))[0]
+    x = torch.complex(torch.randn(n0, generator=torch.Generator().manual_seed(seed)),
+                      torch.randn(n0, generator=torch.Generator().manual_seed(seed + 1))).to(torch.complex64).cuda()
+    y = torch.empty_like(x)
+    adia = torch.full((n0,), complex(alpha_inv), dtype=torch.complex64, device="cuda")
+    aoff = torch.zeros(n0, dtype=torch.complex64, device="cuda")
+    torch.cuda.synchronize()
+    pb = sum(pass_bytes(tg.nat, tg.nat0, (tg.nx, tg.ny, tg.nz)))
+    out = dict(workload=f"sphere D={edge} ({tg.nat0} dipoles, grid {tg.nx}^3, padded {2 * tg.nx}^3 x 3), "
+                        f"m=1.7+0.05i, kd={kd}, compact vectors", matvec_bytes=pb)
+    for name, backend in (("fft3d", BACKEND_FFT3D), ("slices", BACKEND_SLICES)):
+        lib.set_backend(backend)
+        t0 = time.perf_counter()
+        lib.prepare_green(tg.nx, tg.ny, tg.nz, tg.iocc, (kd, 0.0, 0.0))
+        prep_s = time.perf_counter() - t0
+        st = torch.cuda.ExternalStream(lib.stream())
+        args = (x.data_ptr(), y.data_ptr(), adia.data_ptr(), aoff.data_ptr())
+        for _ in range(warmup):
+            lib.apply_device("N", *args, layout=1)
+        lib.sync()
+        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        with torch.cuda.stream(st):
+            ev0.record(st)
+            for _ in range(steps):
+                lib.apply_device("N", *args, layout=1)
+            ev1.record(st)
+        ev1.synchronize()
+        ms = ev0.elapsed_time(ev1) / steps
+        info = lib.info()
+        leg = dict(ms_per_matvec=round(ms, 3), matvecs_per_s=round(1000.0 / ms, 2),
+                   algorithmic_gbs=round(pb / (ms * 1e-3) / 1e9, 1),
+                   frac_of_hbm_peak=round(pb / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                   prepare_seconds=round(prep_s, 3), green_build_ms=round(info["green_build_us"] / 1000.0, 1))
+        if name == "fft3d":
+            passes = lib.time_matvec("N", *args, layout=1, reps=5)
+            leg["pass_ms"] = [round(v, 3) for v in passes[:5]]
+        out[name] = leg
+        lib.release()
+    lib.set_backend(BACKEND_FFT3D)
+    del x, y, adia, aoff
+    torch.cuda.empty_cache()
+    return out
+
+
 def T_incident_compact(tg, kvec, e0):
     import numpy as np
     from ddscat_hip import targets as T
@@ -378,6 +433,7 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--backend", choices=["fft3d", "slices"], default="fft3d")
     ap.add_argument("--pmc-calib", action="store_true", help="run one torch copy of known size (rocprof scale)")
+    ap.add_argument("--no-large", action="store_true", help="skip the 384^3 sphere MATVEC leg")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build measures one GPU only (single-GPU box); --gpus must be 1", file=sys.stderr)
@@ -474,6 +530,8 @@ def main():
         # small-target batched leg last: it re-prepares the library for its own target
         out["batched_sphere32"] = batched_small_target(lib)
         out["batched_targets"] = batched_targets(lib, n_targets=args.targets)
+    if not args.no_large:
+        out["large_target"] = large_target(lib)
     print(json.dumps(out))
     lib.release()
 

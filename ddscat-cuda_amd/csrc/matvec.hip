@@ -29,6 +29,11 @@ namespace ddhip {
 static constexpr int MV_THREADS = 256;
 static constexpr int PER_WIDE = 16;   // passes 1, 2, 4
 static constexpr int PER_NARROW = 8;  // passes 3, 5 (three components in LDS)
+// Wide tiles: long axes (a 256-thread tile would hold fewer than 8 columns,
+// i.e. rows of < 64 contiguous bytes) run passes 2-4 with 1024-thread
+// workgroups and up to the device's per-workgroup LDS, so each tile reads
+// rows of 8-16 columns.
+static constexpr int WIDE_THREADS = 1024;
 
 // ---------------------------------------------------------------- pass 1 --
 template <int PER>
@@ -82,8 +87,8 @@ __global__ __launch_bounds__(MV_THREADS) void k_pass1_xfwd(
 
 // ------------------------------------------------------------- pass 2, 4 --
 // in [3][nz][nin][gx] -> out [3][nz][nout][gx], FFT along the middle axis.
-template <int DIR, int PER>
-__global__ __launch_bounds__(MV_THREADS) void k_pass_y(
+template <int DIR, int PER, int NT = MV_THREADS>
+__global__ __launch_bounds__(NT) void k_pass_y(
     const float2* __restrict__ in, float2* __restrict__ out, const float2* __restrict__ twg,
     FftPlan p, int gx, int nin, int nout, int nz, int cy_log2, size_t ins, size_t outs, BatchArgs ba) {
   extern __shared__ float2 sm[];
@@ -117,8 +122,8 @@ __global__ __launch_bounds__(MV_THREADS) void k_pass_y(
 }
 
 // ---------------------------------------------------------------- pass 3 --
-template <int PER>
-__global__ __launch_bounds__(MV_THREADS) void k_pass3_z_green(
+template <int PER, int NT = MV_THREADS>
+__global__ __launch_bounds__(NT) void k_pass3_z_green(
     float2* __restrict__ a2, const float2* __restrict__ green, const float2* __restrict__ twg,
     FftPlan p, int nx, int ny, int nz, int gx, int gy, int cz_log2, size_t plane, BatchArgs ba, GreenGeo gg) {
   extern __shared__ float2 sm[];
@@ -317,6 +322,33 @@ static int upload_twiddles(int L, float2** dst) {
   return 0;
 }
 
+// per-workgroup LDS bytes of the current device (160 KB on gfx950)
+static size_t lds_limit() {
+  int dev = 0, v = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || v <= 0)
+    return 64 * 1024;
+  return (size_t)v;
+}
+
+// lets the wide-tile kernels take more than 64 KB of dynamic LDS
+static int allow_big_lds(size_t bytes) {
+  const int b = (int)bytes;
+  const hipError_t e[] = {
+      hipFuncSetAttribute((const void*)k_pass_y<+1, PER_WIDE, WIDE_THREADS>,
+                          hipFuncAttributeMaxDynamicSharedMemorySize, b),
+      hipFuncSetAttribute((const void*)k_pass_y<-1, PER_WIDE, WIDE_THREADS>,
+                          hipFuncAttributeMaxDynamicSharedMemorySize, b),
+      hipFuncSetAttribute((const void*)k_pass3_z_green<PER_NARROW, WIDE_THREADS>,
+                          hipFuncAttributeMaxDynamicSharedMemorySize, b)};
+  for (hipError_t x : e)
+    if (x != hipSuccess) {
+      set_hip_error("MATVEC plan (wide LDS tiles)", x);
+      return 1;
+    }
+  return 0;
+}
+
 int matvec_setup(Context& c) {
   MatvecPlan& m = c.mv;
   m = MatvecPlan{};
@@ -340,6 +372,19 @@ int matvec_setup(Context& c) {
   while (m.rx_inv > 1 && (size_t)((nrows + m.rx_inv - 1) / m.rx_inv) < 1024) m.rx_inv /= 2;
   m.cy = std::min(16, pow2floor(std::max(1, wide / m.gy)));
   m.cz = std::min(16, pow2floor(std::max(1, narrow / m.gz)));
+  m.ty = m.tz = MV_THREADS;
+  const size_t lds_max = lds_limit();
+  if (m.cy < 8) {
+    int cw = std::min(16, pow2floor(std::max(1, PER_WIDE * WIDE_THREADS / m.gy)));
+    while (cw > 1 && (size_t)(m.gy + cw * m.gy) * sizeof(float2) > lds_max) cw /= 2;
+    if (cw > m.cy) { m.cy = cw; m.ty = WIDE_THREADS; }
+  }
+  if (m.cz < 8) {
+    int cw = std::min(16, pow2floor(std::max(1, PER_NARROW * WIDE_THREADS / m.gz)));
+    while (cw > 1 && (size_t)(m.gz + 3 * cw * m.gz) * sizeof(float2) > lds_max) cw /= 2;
+    if (cw > m.cz) { m.cz = cw; m.tz = WIDE_THREADS; }
+  }
+  if ((m.ty == WIDE_THREADS || m.tz == WIDE_THREADS) && allow_big_lds(lds_max)) return 206;
   if (upload_twiddles(m.gx, &m.twx) || upload_twiddles(m.gy, &m.twy) || upload_twiddles(m.gz, &m.twz)) return 206;
   m.p1 = (size_t)c.ny * m.gx;
   m.p2 = (size_t)m.gy * m.gx;
@@ -379,24 +424,36 @@ static int launch_pass(Context& c, int which, int kind, int cj, int compact, con
     case 2: {
       dim3 grid((m.gx + m.cy - 1) / m.cy, c.nz, 3 * nb);
       size_t lds = (size_t)(m.gy + m.cy * m.gy) * sizeof(float2);
-      hipLaunchKernelGGL((k_pass_y<+1, PER_WIDE>), grid, dim3(MV_THREADS), lds, st, a1, a2, m.twy, m.py, m.gx,
-                         c.ny, m.gy, c.nz, ilog2(m.cy), m.p1, m.p2, bt.k);
+      if (m.ty == WIDE_THREADS)
+        hipLaunchKernelGGL((k_pass_y<+1, PER_WIDE, WIDE_THREADS>), grid, dim3(WIDE_THREADS), lds, st, a1, a2, m.twy,
+                           m.py, m.gx, c.ny, m.gy, c.nz, ilog2(m.cy), m.p1, m.p2, bt.k);
+      else
+        hipLaunchKernelGGL((k_pass_y<+1, PER_WIDE>), grid, dim3(MV_THREADS), lds, st, a1, a2, m.twy, m.py, m.gx,
+                           c.ny, m.gy, c.nz, ilog2(m.cy), m.p1, m.p2, bt.k);
       DDH_LAUNCH_OK("MATVEC pass 2 (y FFT)", 22);
       break;
     }
     case 3: {
       dim3 grid((m.gx + m.cz - 1) / m.cz, m.gy, nb);
       size_t lds = (size_t)(m.gz + 3 * m.cz * m.gz) * sizeof(float2);
-      hipLaunchKernelGGL(k_pass3_z_green<PER_NARROW>, grid, dim3(MV_THREADS), lds, st, a2, c.green, m.twz, m.pz,
-                         c.nx, c.ny, c.nz, m.gx, m.gy, ilog2(m.cz), m.p2, bt.k, green_geo(c));
+      if (m.tz == WIDE_THREADS)
+        hipLaunchKernelGGL((k_pass3_z_green<PER_NARROW, WIDE_THREADS>), grid, dim3(WIDE_THREADS), lds, st, a2,
+                           c.green, m.twz, m.pz, c.nx, c.ny, c.nz, m.gx, m.gy, ilog2(m.cz), m.p2, bt.k, green_geo(c));
+      else
+        hipLaunchKernelGGL(k_pass3_z_green<PER_NARROW>, grid, dim3(MV_THREADS), lds, st, a2, c.green, m.twz, m.pz,
+                           c.nx, c.ny, c.nz, m.gx, m.gy, ilog2(m.cz), m.p2, bt.k, green_geo(c));
       DDH_LAUNCH_OK("MATVEC pass 3 (z FFT + Green)", 23);
       break;
     }
     case 4: {
       dim3 grid((m.gx + m.cy - 1) / m.cy, c.nz, 3 * nb);
       size_t lds = (size_t)(m.gy + m.cy * m.gy) * sizeof(float2);
-      hipLaunchKernelGGL((k_pass_y<-1, PER_WIDE>), grid, dim3(MV_THREADS), lds, st, a2, a1, m.twy, m.py, m.gx,
-                         m.gy, c.ny, c.nz, ilog2(m.cy), m.p2, m.p1, bt.k);
+      if (m.ty == WIDE_THREADS)
+        hipLaunchKernelGGL((k_pass_y<-1, PER_WIDE, WIDE_THREADS>), grid, dim3(WIDE_THREADS), lds, st, a2, a1, m.twy,
+                           m.py, m.gx, m.gy, c.ny, c.nz, ilog2(m.cy), m.p2, m.p1, bt.k);
+      else
+        hipLaunchKernelGGL((k_pass_y<-1, PER_WIDE>), grid, dim3(MV_THREADS), lds, st, a2, a1, m.twy, m.py, m.gx,
+                           m.gy, c.ny, c.nz, ilog2(m.cy), m.p2, m.p1, bt.k);
       DDH_LAUNCH_OK("MATVEC pass 4 (y inverse FFT)", 24);
       break;
     }

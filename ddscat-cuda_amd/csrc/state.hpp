@@ -18,6 +18,7 @@ struct MatvecPlan {
   float2 *twx = nullptr, *twy = nullptr, *twz = nullptr;  // device twiddle tables
   // tile parameters
   int rx_fwd = 1, rx_inv = 1, cy = 1, cz = 1;
+  int ty = 256, tz = 256;  // workgroup size of the LDS-staged y passes and z pass
   size_t p1 = 0, p2 = 0;  // plane strides (elements) of a1 [3][nz][ny][gx] and a2 [3][nz][plane p2]
   // FFT kernel per pass 1..5: index into the register shapes of that pass's
   // axis length, or -1 for the LDS-staged kernel (chosen by the autotuner)

@@ -74,6 +74,13 @@ class Library:
             raise FileNotFoundError(
                 f"HIP library {path} is not built; run __graft_entry__.build() (no CPU fallback exists)")
         self.path = path
+        # torch ships its own HIP runtime.  Loading it first lets this library
+        # bind to that runtime; loaded the other way round, the process ends up
+        # with a runtime that finds no device after torch initialises the GPU.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         self.lib = C.CDLL(str(path), mode=C.RTLD_LOCAL)
         L = self.lib
         i, f, vp, cp = C.c_int, C.c_float, C.c_void_p, C.c_char_p

@@ -68,6 +68,30 @@ def test_matvec_matches_oracle(lib, dims, cwhat):
     assert np.all(y.reshape(3, -1)[:, iocc == 0] == 0)
 
 
+@pytest.mark.parametrize("dims", [(4, 6, 384), (6, 384, 4), (384, 4, 6), (5, 4, 320), (4, 3, 512)])
+def test_wide_lds_tiles_match_oracle(lib, dims):
+    """Padded axes of 640-1024 points (no register shape): the LDS-staged y
+    and z passes switch to 1024-thread wide tiles (8-16 columns per tile)."""
+    nx, ny, nz = dims
+    rng = np.random.default_rng(nx * ny * nz)
+    iocc, x, adia, aoff = random_case(rng, nx, ny, nz)
+    akd = 0.23
+    lib.prepare_green(nx, ny, nz, iocc, (akd, 0.0, 0.0))
+    info = lib.info()
+    goct = O.green_octant(nx, ny, nz, np.float64(np.float32(akd)))
+    for cwhat in ("N", "C"):
+        y = lib.apply(cwhat, x, adia, aoff)
+        yref = O.matvec_fft(cwhat, x.astype(np.complex128), adia.astype(np.complex128),
+                            aoff.astype(np.complex128), goct, nx, ny, nz, iocc)
+        err = rel_l2(y, yref)
+        print(f"wide tiles {dims} {cwhat} (cy={info['cy']}, cz={info['cz']}): rel L2 = {err:.3e}")
+        assert err < MATVEC_TOL
+    if 2 * ny >= 640:
+        assert info["cy"] >= 8
+    if 2 * nz >= 640:
+        assert info["cz"] >= 4
+
+
 def test_matvec_dense_bruteforce(lib):
     """GPU MATVEC equals the dense 3NAT x 3NAT operator built from the closed form."""
     nx, ny, nz = 4, 3, 5
