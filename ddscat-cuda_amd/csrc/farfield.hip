@@ -182,9 +182,9 @@ __global__ void k_farfield_reduce(const double2* __restrict__ part, int nchunks,
   S[id] = acc;
 }
 
-// S[d][3] for all directions (host result)
-static int farfield_sums(Context& c, const std::vector<float>& kdirs, const float dx[3], const float x0[3],
-                         std::vector<std::complex<double>>& S) {
+// S[d][3] for all directions (host result) of the compact polarization p
+static int farfield_sums(Context& c, const float2* p, const std::vector<float>& kdirs, const float dx[3],
+                         const float x0[3], std::vector<std::complex<double>>& S) {
   const int ndir = (int)(kdirs.size() / 3);
   S.assign((size_t)ndir * 3, 0.0);
   if (ndir == 0) return 0;
@@ -217,7 +217,7 @@ static int farfield_sums(Context& c, const std::vector<float>& kdirs, const floa
     hipLaunchKernelGGL(k_phase_tables, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, st, dk + 3 * d0, nd, c.nx,
                        c.ny, c.nz, (double)dx[0], (double)dx[1], (double)dx[2], (double)x0[0], (double)x0[1],
                        (double)x0[2], tab);
-    hipLaunchKernelGGL(k_farfield_rows, dim3(nchunks, (nd + FF_DIRS - 1) / FF_DIRS), dim3(FF_DIRS), 0, st, c.sx,
+    hipLaunchKernelGGL(k_farfield_rows, dim3(nchunks, (nd + FF_DIRS - 1) / FF_DIRS), dim3(FF_DIRS), 0, st, p,
                        c.occ, c.rowptr, tab, nd, c.nx, c.ny, c.nz, c.nat0, part);
     hipLaunchKernelGGL(k_farfield_reduce, dim3((unsigned)((nd * 3 + 255) / 256)), dim3(256), 0, st, part, nchunks,
                        nd, dS + (size_t)3 * d0);
@@ -254,7 +254,7 @@ int scat(Context& c, int jpbc, int ndir, const float* ak, const float* aks, cons
     for (size_t i = 0; i < dirs.size(); ++i)
       for (int k = 0; k < 3; ++k) kd[3 * i + k] = dirs[i].k[k];
     std::vector<std::complex<double>> S;
-    if (farfield_sums(c, kd, dx, x0, S)) return 325;
+    if (farfield_sums(c, c.sx, kd, dx, x0, S)) return 325;
     double acc[5] = {0, 0, 0, 0, 0}, back = 0.0;
     for (size_t i = 0; i < dirs.size(); ++i) {
       const double nxv = (double)kd[3 * i] / akk, nyv = (double)kd[3 * i + 1] / akk, nzv = (double)kd[3 * i + 2] / akk;
@@ -273,7 +273,7 @@ int scat(Context& c, int jpbc, int ndir, const float* ak, const float* aks, cons
   if (ndir > 0) {
     std::vector<float> kd(aks, aks + 3 * (size_t)ndir);
     std::vector<std::complex<double>> S;
-    if (farfield_sums(c, kd, dx, x0, S)) return 330;
+    if (farfield_sums(c, c.sx, kd, dx, x0, S)) return 330;
     const double term = ak3 / std::sqrt((double)e02);
     float2* f1 = (float2*)f1v;
     float2* f2 = (float2*)f2v;
@@ -290,9 +290,113 @@ int scat(Context& c, int jpbc, int ndir, const float* ak, const float* aks, cons
   return 0;
 }
 
+// Amplitude and Mueller matrices of incident-polarization pairs.  The
+// amplitudes are SCAT's (cu:841-845), f_ij = k^3/|E0| e_i . S_j with
+// e_1 = e_par_s, e_2 = e_perp the scattered basis and j the incident
+// polarization; the conversion that DDSCAT's GETMUELLER makes on the host
+// (DDSCAT_cuda.f90:3294-3300, routine not in the reference repository) is
+// restated from Bohren & Huffman ch. 3:
+//   [[S2, S3], [S4, S1]] = -i F M,  M_jk = conj(e0j) . (e_par_i, e_perp)_k,
+//   e_par_i = khat x e_perp,
+// and the Mueller matrix is their eq. 3.16.
+static void amplitude_to_mueller(const std::complex<double> f[4], const double khat[3], const double ep[3],
+                                 const std::complex<double> e01[3], const std::complex<double> e02[3], double* M) {
+  using cd = std::complex<double>;
+  const double pi_[3] = {khat[1] * ep[2] - khat[2] * ep[1], khat[2] * ep[0] - khat[0] * ep[2],
+                         khat[0] * ep[1] - khat[1] * ep[0]};
+  cd m11 = 0.0, m12 = 0.0, m21 = 0.0, m22 = 0.0;
+  for (int k = 0; k < 3; ++k) {
+    m11 += std::conj(e01[k]) * pi_[k];
+    m12 += std::conj(e01[k]) * ep[k];
+    m21 += std::conj(e02[k]) * pi_[k];
+    m22 += std::conj(e02[k]) * ep[k];
+  }
+  const cd mi(0.0, -1.0);
+  const cd f11 = f[0], f21 = f[1], f12 = f[2], f22 = f[3];
+  const cd S2 = mi * (f11 * m11 + f12 * m21), S3 = mi * (f11 * m12 + f12 * m22);
+  const cd S4 = mi * (f21 * m11 + f22 * m21), S1 = mi * (f21 * m12 + f22 * m22);
+  const double a1 = std::norm(S1), a2 = std::norm(S2), a3 = std::norm(S3), a4 = std::norm(S4);
+  const cd s23 = S2 * std::conj(S3), s14 = S1 * std::conj(S4), s24 = S2 * std::conj(S4), s13 = S1 * std::conj(S3);
+  const cd s12 = S1 * std::conj(S2), s34 = S3 * std::conj(S4);
+  M[0] = 0.5 * (a2 + a3 + a4 + a1);
+  M[1] = 0.5 * (a2 - a3 + a4 - a1);
+  M[2] = (s23 + s14).real();
+  M[3] = (s23 - s14).imag();
+  M[4] = 0.5 * (a2 - a4 + a3 - a1);
+  M[5] = 0.5 * (a2 - a3 - a4 + a1);
+  M[6] = (s23 - s14).real();
+  M[7] = (s23 + s14).imag();
+  M[8] = (s24 + s13).real();
+  M[9] = (s24 - s13).real();
+  M[10] = (s12 + s34).real();
+  M[11] = (std::conj(s12) + std::conj(s34)).imag();
+  M[12] = (std::conj(s24) + s13).imag();  // Im(S2* S4 + S3* S1)
+  M[13] = (std::conj(s24) - s13).imag();  // Im(S2* S4 - S3* S1)
+  M[14] = (s12 - s34).imag();
+  M[15] = (s12 - s34).real();
+}
+
+int scat_matrix(Context& c, int npair, const float2* p, int ndir, const float* ak, const float2* e0,
+                const float* aks, const float* em1, const float* em2, const float* dx, const float* x0, float e02,
+                float2* famp, double* mueller) {
+  clear_error();
+  if (!c.ready || npair <= 0 || ndir <= 0 || !p || !ak || !e0 || !aks || !em1 || !em2 || !dx || !x0 || !famp ||
+      !(e02 > 0.f)) {
+    set_error("scat_matrix", "invalid state/argument");
+    return 340;
+  }
+  const size_t n = (size_t)3 * c.nat0;
+  for (int q = 0; q < npair; ++q) {
+    const float* akq = ak + 3 * q;
+    const double k2 = (double)akq[0] * akq[0] + (double)akq[1] * akq[1] + (double)akq[2] * akq[2], k = std::sqrt(k2);
+    if (!(k > 0.0)) { set_error("scat_matrix", "zero wave vector"); return 340; }
+    const double term = k * k2 / std::sqrt((double)e02);
+    const double khat[3] = {akq[0] / k, akq[1] / k, akq[2] / k};
+    std::complex<double> e01v[3], e02v[3];
+    for (int i = 0; i < 3; ++i) {
+      e01v[i] = {e0[6 * q + i].x, e0[6 * q + i].y};
+      e02v[i] = {e0[6 * q + 3 + i].x, e0[6 * q + 3 + i].y};
+    }
+    const size_t o = (size_t)q * ndir * 3;
+    std::vector<float> kd(aks + o, aks + o + 3 * (size_t)ndir);
+    std::vector<std::complex<double>> S[2];
+    for (int j = 0; j < 2; ++j)
+      if (farfield_sums(c, p + (2 * (size_t)q + j) * n, kd, dx, x0, S[j])) return 341;
+    for (int d = 0; d < ndir; ++d) {
+      const float* a = em1 + o + 3 * d;
+      const float* b = em2 + o + 3 * d;
+      std::complex<double> f[4];  // f11, f21, f12, f22
+      for (int j = 0; j < 2; ++j) {
+        std::complex<double> s1 = 0.0, s2 = 0.0;
+        for (int kk = 0; kk < 3; ++kk) {
+          s1 += (double)a[kk] * S[j][3 * d + kk];
+          s2 += (double)b[kk] * S[j][3 * d + kk];
+        }
+        f[2 * j] = term * s1;
+        f[2 * j + 1] = term * s2;
+      }
+      float2* fo = famp + ((size_t)q * ndir + d) * 4;
+      for (int i = 0; i < 4; ++i) fo[i] = make_float2((float)f[i].real(), (float)f[i].imag());
+      if (mueller) {
+        const double ep[3] = {b[0], b[1], b[2]};
+        amplitude_to_mueller(f, khat, ep, e01v, e02v, mueller + ((size_t)q * ndir + d) * 16);
+      }
+    }
+  }
+  return 0;
+}
+
 }  // namespace ddhip
 
 using namespace ddhip;
+
+extern "C" DDSCAT_HIP_API int ddscat_hip_scat_matrix(int npair, const void* p, int ndir, const float* ak,
+                                                     const void* e0, const float* aks, const float* em1,
+                                                     const float* em2, const float* dx, const float* x0, float e02,
+                                                     void* famp, double* mueller) {
+  return scat_matrix(ctx(), npair, (const float2*)p, ndir, ak, (const float2*)e0, aks, em1, em2, dx, x0, e02,
+                     (float2*)famp, mueller);
+}
 
 extern "C" DDSCAT_HIP_API int ddscat_cuda_evalq_f32(int use_resident, const void* e_compact, const void* p_compact,
                                                     const void* adia_compact, const void* aoff_compact, float akx,

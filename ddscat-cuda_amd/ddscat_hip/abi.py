@@ -31,7 +31,7 @@ EXTENSION_SYMBOLS = [
     "ddscat_hip_time_matvec", "ddscat_hip_set_fft_kernels", "ddscat_hip_solve_device",
     "ddscat_hip_solve_batch", "ddscat_hip_apply_batch", "ddscat_hip_evalq_batch",
     "ddscat_hip_cabs_adjoint", "ddscat_hip_setup_rhs", "ddscat_hip_set_autotune",
-    "ddscat_hip_nearfield", "ddscat_hip_set_pass_kernel",
+    "ddscat_hip_nearfield", "ddscat_hip_set_pass_kernel", "ddscat_hip_scat_matrix",
 ]
 # method ids of ddscat_hip_solve_device
 SOLVER_IDS = {"pbcgs2": 0, "qmrccg": 1, "petrkp": 2, "gpbicg": 3, "pbcgst": 4, "bicgstab4": 5,
@@ -136,6 +136,8 @@ class Library:
         L.ddscat_hip_set_autotune.restype = i
         L.ddscat_hip_nearfield.argtypes = [vp, vp, vp]
         L.ddscat_hip_nearfield.restype = i
+        L.ddscat_hip_scat_matrix.argtypes = [i, vp, i, vp, vp, vp, vp, vp, vp, vp, f, vp, vp]
+        L.ddscat_hip_scat_matrix.restype = i
         L.ddscat_hip_set_pass_kernel.argtypes = [i, i]
         L.ddscat_hip_set_pass_kernel.restype = i
         self.nat = self.nat0 = 0
@@ -328,6 +330,26 @@ class Library:
         """E_inc + E_sca (or E_sca) on every lattice site, full [3][NAT] device array."""
         return self._check("nearfield", self.lib.ddscat_hip_nearfield(
             C.c_void_p(p_ptr), C.c_void_p(einc_ptr) if einc_ptr else None, C.c_void_p(out_ptr)))
+
+    def scat_matrix(self, p_ptr, npair, ak, e0, aks, em1, em2, x0, dx=(1.0, 1.0, 1.0), e02=1.0,
+                    mueller=True):
+        """Amplitudes f11 f21 f12 f22 [npair, ndir, 4] and Mueller matrices
+        [npair, ndir, 4, 4] of incident-polarization pairs whose compact
+        polarizations sit at the device pointer p_ptr ([2*npair][3][NAT0])."""
+        ak = np.ascontiguousarray(np.asarray(ak, np.float32).reshape(npair, 3))
+        e0 = np.ascontiguousarray(np.asarray(e0, np.complex64).reshape(npair, 2, 3))
+        aks = np.ascontiguousarray(np.asarray(aks, np.float32).reshape(npair, -1, 3))
+        ndir = aks.shape[1]
+        em1 = np.ascontiguousarray(np.asarray(em1, np.float32).reshape(npair, ndir, 3))
+        em2 = np.ascontiguousarray(np.asarray(em2, np.float32).reshape(npair, ndir, 3))
+        dxa = np.asarray(dx, np.float32)
+        x0a = np.asarray(x0, np.float32)
+        f = np.empty((npair, ndir, 4), np.complex64)
+        M = np.empty((npair, ndir, 4, 4), np.float64) if mueller else None
+        self._check("scat_matrix", self.lib.ddscat_hip_scat_matrix(
+            int(npair), C.c_void_p(p_ptr), int(ndir), _p(ak), _p(e0), _p(aks), _p(em1), _p(em2), _p(dxa), _p(x0a),
+            float(e02), _p(f), _p(M) if mueller else None))
+        return f, M
 
     def set_pass_kernel(self, pass_, kind, check=True):
         rc = self.lib.ddscat_hip_set_pass_kernel(int(pass_), int(kind))

@@ -217,6 +217,28 @@ DDSCAT_HIP_API int ddscat_hip_setup_rhs(int nrhs, const float *ak, const float *
  * Codes 10 not prepared, 11 argument / backend, 12-13 launch or copy. */
 DDSCAT_HIP_API int ddscat_hip_nearfield(const void *p_compact, const void *einc_full, void *out_full);
 
+/* Amplitude and Mueller matrices for npair incident-polarization pairs (the
+ * S-matrix output path: the amplitudes of ddscat_cuda_scat_f32,
+ * ddscat_matvec_cuda.cu:841-845, for both polarizations at once, converted
+ * as DDSCAT's GETMUELLER does on the host, DDSCAT_cuda.f90:3294-3300;
+ * formulas of Bohren & Huffman ch. 3, eq. 3.16).
+ *   p      device, compact [2*npair][3][NAT0] complex64: pair q solved for
+ *          incident polarizations e01 (2q) and e02 (2q+1)
+ *   ak     [npair][3] k d of the incident wave of each pair (target frame)
+ *   e0     [npair][2][3] complex64: e01, e02 of each pair (orthonormal)
+ *   aks    [npair][ndir][3] scattering wave vectors k d khat_s
+ *   em1    [npair][ndir][3] e_par_s = khat_s x e_perp (in the scattering plane)
+ *   em2    [npair][ndir][3] e_perp, the scattering-plane normal shared by the
+ *          incident and scattered bases (Bohren & Huffman: sin(phi) x - cos(phi) y
+ *          for incidence along z)
+ *   dx, x0 lattice ratios and origin offset as for ddscat_cuda_scat_f32
+ *   famp   host [npair][ndir][4] complex64: f11 f21 f12 f22
+ *   mueller host [npair][ndir][4][4] doubles (may be NULL)
+ * Codes: 340 invalid state or argument, 341 far-field failure. */
+DDSCAT_HIP_API int ddscat_hip_scat_matrix(int npair, const void *p, int ndir, const float *ak, const void *e0,
+                                          const float *aks, const float *em1, const float *em2, const float *dx,
+                                          const float *x0, float e02, void *famp, double *mueller);
+
 /* Copy the resident spectral Green tensor (ddscat_hip_info field 5 complex64
  * values) to host memory in the CXZC layout ddscat_cuda_prepare_f32 takes. */
 DDSCAT_HIP_API int ddscat_hip_get_green(void *out);
