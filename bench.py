@@ -439,6 +439,7 @@ def main():
         print("bench.py: this build measures one GPU only (single-GPU box); --gpus must be 1", file=sys.stderr)
         sys.exit(2)
 
+    import numpy as np
     import torch
     from ddscat_hip import Library, BACKEND_SLICES, BACKEND_FFT3D
     assert torch.cuda.is_available(), "bench.py needs an MI355X"
@@ -493,6 +494,22 @@ def main():
     roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=round(achieved / HBM_PEAK_GBS, 4), traffic=measured_traffic(args, dom), kernel=dom_name)
 
+    # the reference ABI's own MATVEC entry takes host buffers: x, adia, aoff
+    # in and y out over PCIe at every call (cu:1027-1029); reported beside
+    # the HBM-resident value, never as it
+    x_h, adia_h, aoff_h = host
+    xf = np.zeros(3 * tg.nat, np.complex64)
+    xf.reshape(3, -1)[:, tg.occ] = x_h.reshape(3, -1)
+    lib.apply("N", xf, adia_h, aoff_h)
+    t0 = time.perf_counter()
+    reps = 5
+    for _ in range(reps):
+        lib.apply("N", xf, adia_h, aoff_h)
+    abi_ms = (time.perf_counter() - t0) / reps * 1e3
+    pcie = dict(matvecs_per_s=round(1000.0 / abi_ms, 2), ms_per_matvec=round(abi_ms, 3),
+                host_bytes_per_matvec=4 * 3 * tg.nat * 8,
+                note="ddscat_cuda_apply_f32 on host full-layout vectors (3 H2D + 1 D2H per call)")
+
     solve = small = None
     if not args.no_solve:
         solve = time_to_qext(lib, tg, kd, method=args.solver)
@@ -523,6 +540,7 @@ def main():
         "matvec_gbs": round(total_bytes / (ms_step * 1e-3) / 1e9, 1),
         "matvec_bytes": total_bytes,
         "passes": per_pass,
+        "abi_apply_pcie_inclusive": pcie,
         "time_to_qext": solve,
         "cpu_baseline": cpu,
     }
