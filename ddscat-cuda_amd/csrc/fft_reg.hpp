@@ -177,6 +177,23 @@ struct Reg2 {
   static constexpr int G = (R2 + R1 - 1) / R1;
   static constexpr int VN = R2 > G * R1 ? R2 : G * R1;
 };
+// The padded axes have length L = R1 R2 = 2N with data (forward input) and
+// kept results (inverse output) in n < N.  Element n = t + R1 n2 of thread t
+// (t < R1) lies below N for every t when 2 n2 + 1 < R2 and above it when
+// 2 n2 + 1 > R2; only the middle element of an odd R2 depends on t.  With
+// the test in this form the zero half of a forward input is a compile-time
+// constant (its butterflies fold away) and the discarded half of an inverse
+// output is never computed.
+template <int R2>
+__device__ __forceinline__ bool lower_half(int n2, int n, int N) {
+  return (2 * n2 + 1 < R2) || (2 * n2 + 1 == R2 && n < N);
+}
+// lane-pair layout: element i of lane-parity h holds n = t + Q (h + 2 i), t < Q
+template <int Q>
+__device__ __forceinline__ bool lower_half_pair(int i, int n, int N) {
+  return (2 * i + 1 < Q / 2) || (2 * i < Q / 2 && n < N);
+}
+
 template <int R1, int R2>
 __device__ __forceinline__ bool reg2_owns(int t, int i) {
   return Reg2<R1, R2>::G * R1 == R2 || t + R1 * i < R2;

@@ -58,7 +58,7 @@ __global__ __launch_bounds__(R1 * C) void kr_pass1(const float2* __restrict__ x,
   for (int n2 = 0; n2 < R2; ++n2) {
     const int n = t + R1 * n2;
     float2 val = make_float2(0.f, 0.f);
-    if (live && n < nx) {
+    if (live && lower_half<R2>(n2, n, nx)) {
       if (!compact) {
         val = x[(size_t)c * nat + (size_t)g * nx + n];
       } else if (sm.contiguous) {
@@ -108,7 +108,7 @@ __global__ __launch_bounds__(R1 * C) void kr_passy(const float2* __restrict__ in
 #pragma unroll
     for (int n2 = 0; n2 < R2; ++n2) {
       const int n = t + R1 * n2;
-      v[n2] = (live && n < nin) ? src[(size_t)n * gx] : make_float2(0.f, 0.f);
+      v[n2] = (live && lower_half<R2>(n2, n, nin)) ? src[(size_t)n * gx] : make_float2(0.f, 0.f);
     }
     __syncthreads();
     fft2_fwd<R1, R2, +1>(v, buf, twL, t, s, P);
@@ -132,7 +132,7 @@ __global__ __launch_bounds__(R1 * C) void kr_passy(const float2* __restrict__ in
 #pragma unroll
       for (int n2 = 0; n2 < R2; ++n2) {
         const int n = t + R1 * n2;
-        if (n < nout) dst[(size_t)n * gx] = v[n2];
+        if (lower_half<R2>(n2, n, nout)) dst[(size_t)n * gx] = v[n2];
       }
     }
   }
@@ -167,7 +167,7 @@ __global__ __launch_bounds__(R1 * C) void kr_pass3(float2* __restrict__ a2, cons
 #pragma unroll
   for (int n2 = 0; n2 < R2; ++n2) {
     const int n = t + R1 * n2;
-    const bool ok = live && n < nz;
+    const bool ok = live && lower_half<R2>(n2, n, nz);
     const size_t o = (size_t)n * plane;
     v0[n2] = ok ? col[o] : make_float2(0.f, 0.f);
     v1[n2] = ok ? col[o + cs] : make_float2(0.f, 0.f);
@@ -221,7 +221,7 @@ __global__ __launch_bounds__(R1 * C) void kr_pass3(float2* __restrict__ a2, cons
 #pragma unroll
     for (int n2 = 0; n2 < R2; ++n2) {
       const int n = t + R1 * n2;
-      if (n < nz) {
+      if (lower_half<R2>(n2, n, nz)) {
         const size_t o = (size_t)n * plane;
         col[o] = v0[n2];
         col[o + cs] = v1[n2];
@@ -265,7 +265,7 @@ __global__ __launch_bounds__(2 * R1 * C) void kr_pass3y(float2* __restrict__ a2,
 #pragma unroll
   for (int n2 = 0; n2 < R2; ++n2) {
     const int n = t + R1 * n2;
-    const bool ok = live && n < nz;
+    const bool ok = live && lower_half<R2>(n2, n, nz);
     const size_t o = (size_t)n * plane;
     v0[n2] = ok ? col[o] : make_float2(0.f, 0.f);
     v1[n2] = ok ? col[o + cs] : make_float2(0.f, 0.f);
@@ -319,7 +319,7 @@ __global__ __launch_bounds__(2 * R1 * C) void kr_pass3y(float2* __restrict__ a2,
 #pragma unroll
     for (int n2 = 0; n2 < R2; ++n2) {
       const int n = t + R1 * n2;
-      if (n < nz) {
+      if (lower_half<R2>(n2, n, nz)) {
         const size_t o = (size_t)n * plane;
         col[o] = v0[n2];
         col[o + cs] = v1[n2];
@@ -357,7 +357,7 @@ __global__ __launch_bounds__(2 * Q * C) void kr_pass3p(float2* __restrict__ a2, 
 #pragma unroll
   for (int i = 0; i < H; ++i) {
     const int n = t + Q * (h + 2 * i);
-    const bool ok = live && n < nz;
+    const bool ok = live && lower_half_pair<Q>(i, n, nz);
     const size_t o = (size_t)n * plane;
     v0[i] = ok ? col[o] : make_float2(0.f, 0.f);
     v1[i] = ok ? col[o + cs] : make_float2(0.f, 0.f);
@@ -400,7 +400,7 @@ __global__ __launch_bounds__(2 * Q * C) void kr_pass3p(float2* __restrict__ a2, 
 #pragma unroll
     for (int i = 0; i < H; ++i) {
       const int n = t + Q * (h + 2 * i);
-      if (n < nz) {
+      if (lower_half_pair<Q>(i, n, nz)) {
         const size_t o = (size_t)n * plane;
         col[o] = v0[i];
         col[o + cs] = v1[i];
@@ -459,7 +459,7 @@ __global__ __launch_bounds__(R1 * C) void kr_pass5(const float2* __restrict__ a1
 #pragma unroll
   for (int n2 = 0; n2 < R2; ++n2) {
     const int ix = t + R1 * n2;
-    if (ix >= nx) continue;
+    if (!lower_half<R2>(n2, ix, nx)) continue;
     size_t o, stride;
     if (!compact) {
       o = (size_t)g * nx + ix;
