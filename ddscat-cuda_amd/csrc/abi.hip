@@ -181,7 +181,7 @@ extern "C" DDSCAT_HIP_API int ddscat_hip_set_pass_kernel(int pass, int kind) {
   if (!c.ready || c.backend != BACKEND_FFT3D) { set_error("set_pass_kernel", "not prepared (FFT3D)"); return 10; }
   if (pass < 1 || pass > 5) { set_error("set_pass_kernel", "pass must be 1..5"); return 11; }
   const int lens[6] = {0, c.mv.gx, c.mv.gy, c.mv.gz, c.mv.gy, c.mv.gx};
-  if (kind < -1 || kind >= reg_candidates(lens[pass])) { set_error("set_pass_kernel", "no such kernel"); return 11; }
+  if (kind < -1 || kind >= reg_candidates(lens[pass], pass)) { set_error("set_pass_kernel", "no such kernel"); return 11; }
   c.mv.kind[pass] = kind;
   c.mv.tuned = 0;
   return 0;

@@ -509,7 +509,7 @@ std::map<TuneKey, std::array<int, 6>>& tune_cache() {
 int matvec_autotune(Context& c) {
   MatvecPlan& m = c.mv;
   const int lens[6] = {0, m.gx, m.gy, m.gz, m.gy, m.gx};
-  for (int p = 1; p <= 5; ++p) m.kind[p] = reg_candidates(lens[p]) > 0 ? 0 : -1;
+  for (int p = 1; p <= 5; ++p) m.kind[p] = reg_candidates(lens[p], p) > 0 ? 0 : -1;
   m.tuned = 0;
   if (!c.autotune || c.fft_generic || !c.sx || !c.sy || !c.adia_c || !c.aoff_c) return 0;
   const TuneKey key{c.nx, c.ny, c.nz};
@@ -532,7 +532,7 @@ int matvec_autotune(Context& c) {
   bt.a2 = c.a2;
   int rc = 0;
   for (int p = 1; p <= 5 && !rc; ++p) {
-    const int ncand = reg_candidates(lens[p]);
+    const int ncand = reg_candidates(lens[p], p);
     if (ncand == 0) continue;
     float best = 1e30f;
     int best_kind = m.kind[p];

@@ -554,13 +554,38 @@ static void launch_ypair(Context& c, int cj, int compact, const float2* x, float
                      st, bt.a2, c.green, m.twz, c.nx, c.ny, c.nz, m.gx, m.gy, m.p2, bt.k, green_geo(c));
 }
 
+// single-component passes only (1, 2, 4): shapes whose per-thread arrays are
+// too large to hold the three components of passes 3 and 5 in registers
+template <int R1, int R2, int C>
+static void launch_reg1(Context& c, int cj, int compact, const float2* x, float2* y, const float2* adia,
+                        const float2* aoff, hipStream_t st, const MvBatch& bt, int which) {
+  const MatvecPlan& m = c.mv;
+  constexpr int RT = R1 * C;
+  const int nrows = c.ny * c.nz;
+  const unsigned nb = (unsigned)bt.nrhs;
+  SiteMap sm{c.rowptr, c.rowlo, c.qmap, c.rows_contiguous};
+  if (which == 1)
+    hipLaunchKernelGGL((kr_pass1<R1, R2, C>), dim3((nrows + C - 1) / C, 3, nb), dim3(RT), 0, st, x, bt.a1, sm,
+                       m.twx, c.nx, nrows, c.nat, c.nat0, compact, cj, bt.k);
+  else if (which == 2)
+    hipLaunchKernelGGL((kr_passy<R1, R2, C, +1>), dim3((m.gx + C - 1) / C, c.nz, 3 * nb), dim3(RT), 0, st, bt.a1,
+                       bt.a2, m.twy, m.gx, c.ny, m.gy, c.nz, m.p1, m.p2, bt.k);
+  else if (which == 4)
+    hipLaunchKernelGGL((kr_passy<R1, R2, C, -1>), dim3((m.gx + C - 1) / C, c.nz, 3 * nb), dim3(RT), 0, st, bt.a2,
+                       bt.a1, m.twy, m.gx, m.gy, c.ny, c.nz, m.p2, m.p1, bt.k);
+}
+
 using RegLaunch = void (*)(Context&, int, int, const float2*, float2*, const float2*, const float2*, hipStream_t,
                            const MvBatch&, int);
+constexpr int ALL_PASSES = 0x3e, ONE_COMPONENT_PASSES = (1 << 1) | (1 << 2) | (1 << 4);
 struct RegShape {
   int L, r1, r2, c;
   RegLaunch fn;
+  int passes = ALL_PASSES;  // bit p set: serves pass p
 };
 #define DDH_SHAPE(L_, R1_, R2_, C_) RegShape{L_, R1_, R2_, C_, &launch_reg<R1_, R2_, C_>}
+#define DDH_SHAPE1(L_, R1_, R2_, C_) \
+  RegShape{L_, R1_, R2_, C_, &launch_reg1<R1_, R2_, C_>, ONE_COMPONENT_PASSES}
 // One register shape per padded axis length 2N (L = R1 R2, R1 threads per
 // sequence, C sequences per workgroup of R1 C threads).  Shapes keep a
 // thread's register array at <= 16 complex values so the three-component
@@ -575,6 +600,8 @@ static const RegShape kShapes[] = {
     DDH_SHAPE(120, 8, 15, 32),  DDH_SHAPE(128, 8, 16, 32),  DDH_SHAPE(128, 16, 8, 16), DDH_SHAPE(144, 12, 12, 16),
     DDH_SHAPE(144, 16, 9, 16),  DDH_SHAPE(150, 15, 10, 16), DDH_SHAPE(160, 16, 10, 16), DDH_SHAPE(180, 15, 12, 16),
     DDH_SHAPE(192, 16, 12, 16), DDH_SHAPE(240, 16, 15, 16), DDH_SHAPE(256, 16, 16, 16),
+    // 768 (the 384^3 sphere): 32 values per thread and component
+    DDH_SHAPE1(768, 32, 24, 8), DDH_SHAPE1(768, 32, 24, 4),
 };
 #define DDH_PAIR(L_, Q_, CR_, CP_) RegShape{L_, Q_, Q_, CR_, &launch_pair<Q_, CR_, CP_>}
 // lane-pair pass-3 candidates (listed after the defaults of their lengths)
@@ -587,19 +614,19 @@ static const RegShape kPairShapes[] = {
 #undef DDH_PAIR
 #undef DDH_SHAPE
 
-// shape `cand` (0 = default) of length L, or nullptr
-static const RegShape* find_shape(int L, int cand) {
+// shape `cand` (0 = default) of length L serving pass `which`, or nullptr
+static const RegShape* find_shape(int L, int which, int cand) {
   for (const RegShape& r : kShapes)
-    if (r.L == L && cand-- == 0) return &r;
+    if (r.L == L && (r.passes >> which & 1) && cand-- == 0) return &r;
   for (const RegShape& r : kPairShapes)
-    if (r.L == L && cand-- == 0) return &r;
+    if (r.L == L && (r.passes >> which & 1) && cand-- == 0) return &r;
   return nullptr;
 }
 
-int reg_candidates(int L) {
+int reg_candidates(int L, int which) {
   int n = 0;
-  for (const RegShape& r : kShapes) n += r.L == L;
-  for (const RegShape& r : kPairShapes) n += r.L == L;
+  for (const RegShape& r : kShapes) n += r.L == L && (r.passes >> which & 1);
+  for (const RegShape& r : kPairShapes) n += r.L == L && (r.passes >> which & 1);
   return n;
 }
 
@@ -609,7 +636,7 @@ int reg_pass(Context& c, int which, int cand, int cj, int compact, const float2*
              const float2* aoff, hipStream_t st, const MvBatch& bt) {
   const MatvecPlan& m = c.mv;
   const int L = (which == 1 || which == 5) ? m.gx : (which == 3 ? m.gz : m.gy);
-  const RegShape* r = cand >= 0 ? find_shape(L, cand) : nullptr;
+  const RegShape* r = cand >= 0 ? find_shape(L, which, cand) : nullptr;
   if (!r) return 0;
   r->fn(c, cj, compact, x, y, adia, aoff, st, bt, which);
   return 1;

@@ -94,8 +94,8 @@ void matvec_release(Context& c);
 // FFT work elements per RHS: a1 [3][nz][ny][gx], a2 [3][nz][gy][gx]
 size_t matvec_a1_count(const Context& c);
 size_t matvec_a2_count(const Context& c);
-// register-FFT shapes available for a padded axis length (0: none)
-int reg_candidates(int L);
+// register-FFT shapes available for pass `which` (1..5) on a padded axis length (0: none)
+int reg_candidates(int L, int which);
 // register-FFT pass `which` (1..5) with shape `cand` of its axis length; 1 = launched
 int reg_pass(Context& c, int which, int cand, int cj, int compact, const float2* x, float2* y, const float2* adia,
              const float2* aoff, hipStream_t st, const MvBatch& bt);
