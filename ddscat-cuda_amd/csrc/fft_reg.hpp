@@ -311,6 +311,49 @@ __device__ __forceinline__ void fft2_fwd_s(float2* v, float* __restrict__ lds, c
   }
 }
 
+// Split-exchange form of fft2_inv (real and imaginary parts cross LDS in turn).
+template <int R1, int R2, int DIR, int LAY = 0>
+__device__ __forceinline__ void fft2_inv_s(float2* v, float* __restrict__ lds, const float2* __restrict__ twL, int t,
+                                           int s, int P) {
+  constexpr int G = Reg2<R1, R2>::G;
+#pragma unroll
+  for (int i = 0; i < G; ++i) {
+    if (!reg2_owns<R1, R2>(t, i)) continue;
+    const int k2 = t + R1 * i;
+    float2 u[R1];
+#pragma unroll
+    for (int k1 = 0; k1 < R1; ++k1) u[k1] = v[i * R1 + k1];
+    dft_reg<R1, DIR>(u);
+#pragma unroll
+    for (int n1 = 0; n1 < R1; ++n1) {
+      float2 w = twL[n1 * k2];
+      if (DIR < 0) w.y = -w.y;
+      v[i * R1 + n1] = cmul(u[n1], w);
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < G; ++i) {
+    if (!reg2_owns<R1, R2>(t, i)) continue;
+#pragma unroll
+    for (int n1 = 0; n1 < R1; ++n1) lds[lidx<LAY, R1>(t + R1 * i, n1, s, P)] = v[i * R1 + n1].x;
+  }
+  __syncthreads();
+  float re[R2];
+#pragma unroll
+  for (int k2 = 0; k2 < R2; ++k2) re[k2] = lds[lidx<LAY, R1>(k2, t, s, P)];
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < G; ++i) {
+    if (!reg2_owns<R1, R2>(t, i)) continue;
+#pragma unroll
+    for (int n1 = 0; n1 < R1; ++n1) lds[lidx<LAY, R1>(t + R1 * i, n1, s, P)] = v[i * R1 + n1].y;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k2 = 0; k2 < R2; ++k2) v[k2] = make_float2(re[k2], lds[lidx<LAY, R1>(k2, t, s, P)]);
+  dft_reg<R2, DIR>(v);
+}
+
 // ---- lane-pair split DFTs: a Q-point DFT (Q even) held by two adjacent
 // lanes (h = lane & 1), Q/2 values each; one radix-2 step crosses the pair
 // through a lane shuffle.

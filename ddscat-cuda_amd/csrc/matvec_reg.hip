@@ -231,6 +231,92 @@ __global__ __launch_bounds__(R1 * C) void kr_pass3(float2* __restrict__ a2, cons
   }
 }
 
+// Pass 3 for long axes (32 values per thread and component): the three
+// components go to three thread groups of one workgroup (R1 C threads each),
+// so no thread holds more than one component.  Each group runs its own
+// register FFT (split real / imaginary exchange); the spectra then meet in
+// LDS, each group forms its own row of the symmetric 3x3 Green product, and
+// runs its own inverse transform.  LDS: the three spectra, 3 L C complex,
+// which the exchange buffers alias.
+template <int R1, int R2, int C>
+__global__ __launch_bounds__(3 * R1 * C) void kr_pass3c(float2* __restrict__ a2, const float2* __restrict__ green,
+                                                     const float2* __restrict__ twg, int nx, int ny, int nz, int gx,
+                                                     int gy, size_t plane, BatchArgs ba, GreenGeo gg) {
+  constexpr int L = R1 * R2, GT = R1 * C, RT = 3 * GT, P = C + 1, G = Reg2<R1, R2>::G, VN = Reg2<R1, R2>::VN;
+  constexpr int XB = L * P;  // floats of one group's exchange buffer
+  static_assert(3 * XB <= 2 * 3 * L * C, "exchange buffers must fit inside the spectrum area");
+  __shared__ float2 twL[L];
+  __shared__ float2 spec[3 * L * C];  // [component][kz][column]; aliased by the exchange buffers
+  const int rb = blockIdx.z;
+  if (ba.active && !ba.active[rb]) return;
+  a2 += rb * ba.a2s;
+  const int grp = threadIdx.x / GT, lt = threadIdx.x % GT;
+  const int s = lt % C, t = lt / C;
+  float* xb = reinterpret_cast<float*>(spec) + grp * XB;
+  // XCD-aware grid order as in kr_pass3
+  const int j = blockIdx.y;
+  const int ky = j == 0 ? 0 : j == 1 ? ny : ((j & 1) ? gy - ((j - 2) / 2 + 1) : (j - 2) / 2 + 1);
+  const int NT = gridDim.x, h = (NT + 1) / 2;
+  const int tile = (int)blockIdx.x < h ? (int)blockIdx.x : NT - 1 - ((int)blockIdx.x - h);
+  const int kx = tile * C + s;
+  for (int i = threadIdx.x; i < L; i += RT) twL[i] = twg[i];
+  const bool live = kx < gx;
+  float2* col = a2 + (size_t)grp * nz * plane + (size_t)ky * gx + kx;
+  float2 v[VN];
+#pragma unroll
+  for (int n2 = 0; n2 < R2; ++n2) {
+    const int n = t + R1 * n2;
+    v[n2] = (live && lower_half<R2>(n2, n, nz)) ? col[(size_t)n * plane] : make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  fft2_fwd_s<R1, R2, +1>(v, xb, twL, t, s, P);
+  __syncthreads();
+  // own spectrum out: v[i R1 + k1] = X[(t + R1 i) + R2 k1]
+#pragma unroll
+  for (int i = 0; i < G; ++i) {
+    if (!reg2_owns<R1, R2>(t, i)) continue;
+#pragma unroll
+    for (int k1 = 0; k1 < R1; ++k1) spec[(grp * L + (t + R1 * i) + R2 * k1) * C + s] = v[i * R1 + k1];
+  }
+  __syncthreads();
+  // row grp of the Green product; octant mirror rules for IPBC=0 (cu:109-115), whole grid for IPBC=1
+  const int ox = gg.ox, oy = gg.oy;
+  const int ir = live ? min(kx, gg.mx - kx) : 0;
+  const int jr = min(ky, gg.my - ky);
+  const float sxy = ((kx <= gg.tx) == (ky <= gg.ty)) ? 1.f : -1.f;
+  const float sx = kx <= gg.tx ? 1.f : -1.f, sy = ky <= gg.ty ? 1.f : -1.f;
+  const size_t pb = (size_t)ir + (size_t)ox * jr;
+#pragma unroll
+  for (int i = 0; i < G; ++i) {
+    if (!reg2_owns<R1, R2>(t, i)) continue;
+#pragma unroll
+    for (int k1 = 0; k1 < R1; ++k1) {
+      const int kz = (t + R1 * i) + R2 * k1;
+      const int kr = min(kz, gg.mz - kz);
+      const float sz = kz <= gg.tz ? 1.f : -1.f;
+      const Green6 Gm = green_at(green, pb + (size_t)ox * oy * kr);
+      const float2 p0 = spec[(0 * L + kz) * C + s], p1 = spec[(1 * L + kz) * C + s], p2 = spec[(2 * L + kz) * C + s];
+      float2 r;
+      if (grp == 0)
+        r = cadd(cadd(cmul(Gm.xx, p0), cmul(cscale(Gm.xy, sxy), p1)), cmul(cscale(Gm.xz, sx * sz), p2));
+      else if (grp == 1)
+        r = cadd(cadd(cmul(cscale(Gm.xy, sxy), p0), cmul(Gm.yy, p1)), cmul(cscale(Gm.yz, sy * sz), p2));
+      else
+        r = cadd(cadd(cmul(cscale(Gm.xz, sx * sz), p0), cmul(cscale(Gm.yz, sy * sz), p1)), cmul(Gm.zz, p2));
+      v[i * R1 + k1] = r;
+    }
+  }
+  __syncthreads();
+  fft2_inv_s<R1, R2, -1>(v, xb, twL, t, s, P);
+  if (live) {
+#pragma unroll
+    for (int n2 = 0; n2 < R2; ++n2) {
+      const int n = t + R1 * n2;
+      if (lower_half<R2>(n2, n, nz)) col[(size_t)n * plane] = v[n2];
+    }
+  }
+}
+
 // Pass 3 with the mirror rows ky and gy - ky in the two halves of one
 // workgroup: both halves read the same Green octant entries at the same
 // time, so one of each pair of loads is served by the CU's L1.
@@ -575,6 +661,17 @@ static void launch_reg1(Context& c, int cj, int compact, const float2* x, float2
                        bt.a1, m.twy, m.gx, m.gy, c.ny, c.nz, m.p2, m.p1, bt.k);
 }
 
+// pass 3 by component-split thread groups (kr_pass3c); serves pass 3 only
+template <int R1, int R2, int C>
+static void launch_split3(Context& c, int cj, int compact, const float2* x, float2* y, const float2* adia,
+                          const float2* aoff, hipStream_t st, const MvBatch& bt, int which) {
+  (void)cj; (void)compact; (void)x; (void)y; (void)adia; (void)aoff;
+  if (which != 3) return;
+  const MatvecPlan& m = c.mv;
+  hipLaunchKernelGGL((kr_pass3c<R1, R2, C>), dim3((m.gx + C - 1) / C, m.gy, (unsigned)bt.nrhs), dim3(3 * R1 * C), 0,
+                     st, bt.a2, c.green, m.twz, c.nx, c.ny, c.nz, m.gx, m.gy, m.p2, bt.k, green_geo(c));
+}
+
 using RegLaunch = void (*)(Context&, int, int, const float2*, float2*, const float2*, const float2*, hipStream_t,
                            const MvBatch&, int);
 constexpr int ALL_PASSES = 0x3e, ONE_COMPONENT_PASSES = (1 << 1) | (1 << 2) | (1 << 4);
@@ -602,6 +699,8 @@ static const RegShape kShapes[] = {
     DDH_SHAPE(192, 16, 12, 16), DDH_SHAPE(240, 16, 15, 16), DDH_SHAPE(256, 16, 16, 16),
     // 768 (the 384^3 sphere): 32 values per thread and component
     DDH_SHAPE1(768, 32, 24, 8), DDH_SHAPE1(768, 32, 24, 4),
+    RegShape{768, 32, 24, 8, &launch_split3<32, 24, 8>, 1 << 3},
+    RegShape{768, 32, 24, 4, &launch_split3<32, 24, 4>, 1 << 3},
 };
 #define DDH_PAIR(L_, Q_, CR_, CP_) RegShape{L_, Q_, Q_, CR_, &launch_pair<Q_, CR_, CP_>}
 // lane-pair pass-3 candidates (listed after the defaults of their lengths)
