@@ -350,6 +350,52 @@ def large_target(lib, edge=384, kd=0.25, m=1.7 + 0.05j, steps=10, warmup=2, seed
     return out
 
 
+def adjoint_design(lib, edge=64, xsize=6.0, tol=1e-5, h=1e-3, seed=20261202):
+    """The inverse-design workload: a 64^3 block (262,144 dipoles) with a
+    smooth random permittivity (eps_j = U[1,4] smoothed by a Gaussian of
+    sigma = 3 dipoles, + 0.05i), J = mean Q_abs of both polarizations at +z
+    incidence, x = 6.  Times the forward batched solve and the autograd
+    backward (adjoint solve) and checks dJ/d eps along a smooth random
+    direction against a central difference with step h."""
+    import numpy as np
+    import torch
+    import scipy.ndimage as ndi
+    from ddscat_hip import targets as T
+    from ddscat_hip.adjoint import design_q_abs
+    tg = T.block(edge)
+    kd = T.kd_for_size_parameter(tg, xsize)
+    khat = np.array([0.0, 0.0, 1.0])
+    pols = list(T.polarization_pair(khat))
+    rng = np.random.default_rng(seed)
+    field = ndi.gaussian_filter(rng.uniform(1.0, 4.0, size=(edge, edge, edge)), sigma=3.0)
+    eps0 = field.reshape(-1)[tg.occ] + 0.05j
+    v = ndi.gaussian_filter(rng.normal(size=(edge, edge, edge)), sigma=3.0).reshape(-1)[tg.occ]
+    v = (v + 1j * np.roll(v, 7)) / np.abs(v).max()
+    lib.prepare_green(tg.nx, tg.ny, tg.nz, tg.iocc, khat * kd)
+    J = design_q_abs(lib, tg, torch.tensor(eps0), kd, khat, pols, tol=tol)   # warm-up (graph capture)
+    eps = torch.tensor(eps0, dtype=torch.complex128, requires_grad=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    J = design_q_abs(lib, tg, eps, kd, khat, pols, tol=tol)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    J.backward()
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    ana = float(np.sum(np.real(np.conj(eps.grad.numpy()) * v)))
+    with torch.no_grad():
+        jp = design_q_abs(lib, tg, torch.tensor(eps0 + h * v), kd, khat, pols, tol=tol).item()
+        jm = design_q_abs(lib, tg, torch.tensor(eps0 - h * v), kd, khat, pols, tol=tol).item()
+    num = (jp - jm) / (2 * h)
+    lib.release()
+    return dict(workload=f"block {edge}^3 ({tg.nat0} dipoles), smooth random eps in [1,4] + 0.05i, "
+                         f"Q_abs at +z, both polarizations, x={xsize} (kd={kd:.4f}), QMRCCG tol {tol}",
+                q_abs=round(J.item(), 6), forward_seconds=round(t1 - t0, 4), backward_seconds=round(t2 - t1, 4),
+                gradient_seconds=round(t2 - t0, 4),
+                directional_derivative=dict(adjoint=ana, central_difference=num, step=h,
+                                            rel_diff=abs(num - ana) / abs(ana)))
+
+
 def T_incident_compact(tg, kvec, e0):
     import numpy as np
     from ddscat_hip import targets as T
@@ -548,6 +594,8 @@ def main():
         # small-target batched leg last: it re-prepares the library for its own target
         out["batched_sphere32"] = batched_small_target(lib)
         out["batched_targets"] = batched_targets(lib, n_targets=args.targets)
+    if not args.no_solve:
+        out["adjoint_design"] = adjoint_design(lib)
     if not args.no_large:
         out["large_target"] = large_target(lib)
     print(json.dumps(out))
