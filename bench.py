@@ -295,6 +295,22 @@ def batched_targets(lib, n_targets=48, n_orient=8, kd=0.1, method="pbcgs2", tol=
                                       if t_batch_sample > 0 else None))
 
 
+def large_solve(lib, edge=256, xsize=20.0):
+    """Time to Q_ext on a 256^3 block (16.8 M dipoles, padded grid 512^3 x 3),
+    m = 1.5+0.1i, x = 20 (kd = 0.126, the 128^3 workload's lattice
+    resolution), QMRCCG 1e-5: plain FP32, mixed, batched mixed."""
+    from ddscat_hip import targets as T
+    tg = T.block(edge)
+    kd = T.kd_for_size_parameter(tg, xsize)
+    lib.prepare_green(tg.nx, tg.ny, tg.nz, tg.iocc, (kd, 0.0, 0.0))
+    r = time_to_qext(lib, tg, kd)
+    r["workload"] = (f"block {edge}^3 ({tg.nat0} dipoles), padded {2 * edge}^3 x 3, m=1.5+0.1i, x={xsize} "
+                     f"(kd={kd:.4f}), both polarizations")
+    r["fft_kernels"] = {f"pass{p}": lib.info().get(f"kernel_pass{p}") for p in range(1, 6)}
+    lib.release()
+    return r
+
+
 def large_target(lib, edge=384, kd=0.25, m=1.7 + 0.05j, steps=10, warmup=2, seed=20261202):
     """The large single target: sphere of diameter ``edge`` dipoles (384:
     29.6 M dipoles, padded grid 768^3 x 3 complex64), m = 1.7+0.05i, kd = 0.25,
@@ -403,15 +419,65 @@ def T_incident_compact(tg, kvec, e0):
     return np.ascontiguousarray(e.reshape(3, -1)[:, tg.occ].reshape(-1))
 
 
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_small_target(cores, budget_s=5.0):
+    """The small workload on the host: sphere D=32 (17,256 dipoles), m =
+    1.33+0.01i, x = 2, incidence +x, E || y: the complex64 CPU MATVEC rate and
+    the CPU PBCGS2 time to tol 1e-5 with that MATVEC (oracle restatements)."""
+    import numpy as np
+    from oracle import ddscat_oracle as O
+    from oracle import solvers as S
+    from ddscat_hip import targets as T
+    tg = T.sphere(32)
+    kd = T.kd_for_size_parameter(tg, 2.0)
+    khat = np.array([1.0, 0.0, 0.0])
+    e0 = np.array([0, 1, 0], complex)
+    adia, aoff = T.operators(tg, 1.33 + 0.01j, kd, khat, e0)
+    goct = _green_octant_fast(tg.nx, tg.ny, tg.nz, float(np.float32(kd)))
+    mv = O.CpuMatvecFP32(goct, tg.nx, tg.ny, tg.nz, workers=cores)
+    occ3 = np.flatnonzero(np.tile(tg.iocc != 0, 3))
+    full = np.zeros(3 * tg.nat, np.complex64)
+
+    def mvc(cw, v):
+        full[occ3] = v
+        return mv.apply(cw, full, adia, aoff, tg.iocc)[occ3].astype(np.complex128)
+
+    b = T.incident_field(tg, khat * kd, e0, tg.centroid_x0()).astype(np.complex128)[occ3]
+    mvc("N", b)
+    t0 = time.perf_counter()
+    n = 0
+    while time.perf_counter() - t0 < budget_s and n < 200:
+        mvc("N", b)
+        n += 1
+    rate = n / (time.perf_counter() - t0)
+    t0 = time.perf_counter()
+    res = S.pbcgs2(mvc, b, maxit=300, tol=1e-5)
+    solve_s = time.perf_counter() - t0
+    return dict(matvecs_per_s=round(rate, 2), pbcgs2_seconds=round(solve_s, 3),
+                pbcgs2_iterations=int(res[1]), sample=f"{n} MATVECs, one PBCGS2 solve (both on {cores} workers)")
+
+
 def cpu_baseline(tg, kd, host, budget_s=20.0):
-    """Oracle FP32 (complex64) FFT MATVEC on all host cores (scipy.fft workers)."""
+    """Oracle FP32 (complex64) FFT MATVEC on the box's host cores (scipy.fft
+    workers), 128^3 block; plus the small sphere's MATVEC rate and PBCGS2
+    time to solution."""
     import numpy as np
     from oracle import ddscat_oracle as O
     x, adia, aoff = host
     xf = np.zeros(3 * tg.nat, np.complex64)
     xf.reshape(3, -1)[:, tg.occ] = x.reshape(3, -1)
     goct = _green_octant_fast(tg.nx, tg.ny, tg.nz, float(np.float32(kd)))
-    cores = os.cpu_count() or 1
+    # the GPU box's host share is 16 cores whatever os.cpu_count() reports
+    cores = max(1, min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "16") or 16), 16))
     mv = O.CpuMatvecFP32(goct, tg.nx, tg.ny, tg.nz, workers=cores)
     mv.apply("N", xf, adia, aoff)
     t0 = time.perf_counter()
@@ -424,7 +490,9 @@ def cpu_baseline(tg, kd, host, budget_s=20.0):
             break
     return dict(value=round(n / dt, 4), unit="matvecs/s", cores=cores, kind="port",
                 sample=f"{n} complex64 FFT MATVECs of the {tg.nx}^3 block (oracle CpuMatvecFP32: scipy.fft "
-                       f"workers={cores} on the full padded grid, Green expanded once) in {dt:.1f} s")
+                       f"workers={cores} on the full padded grid, Green expanded once) in {dt:.1f} s",
+                cpu_model=_cpu_model(), nproc=os.cpu_count(), fft_backend="scipy.fft (pocketfft), complex64",
+                sphere32=cpu_small_target(cores))
 
 
 def _green_octant_fast(nx, ny, nz, akd):
@@ -596,6 +664,8 @@ def main():
         out["batched_targets"] = batched_targets(lib, n_targets=args.targets)
     if not args.no_solve:
         out["adjoint_design"] = adjoint_design(lib)
+    if not args.no_large and not args.no_solve:
+        out["time_to_qext_256"] = large_solve(lib)
     if not args.no_large:
         out["large_target"] = large_target(lib)
     print(json.dumps(out))

@@ -177,6 +177,29 @@ struct Reg2 {
   static constexpr int G = (R2 + R1 - 1) / R1;
   static constexpr int VN = R2 > G * R1 ? R2 : G * R1;
 };
+// Column tile of block bx (tiles of C complex64 columns, NT tiles) for
+// kernels whose tiles are narrower than a 128-byte line (C < 16).  Blocks go
+// round-robin over the 8 XCDs by linear id, and each XCD has its own L2, so
+// the W = 16/C tiles sharing one line are given to blocks 8 ids apart (same
+// XCD, back to back); with `mirror` the group also holds the kx mirror tiles
+// NT-1-b, which read the same Green octant entries.  Falls back to the plain
+// order (mirror: b and NT-1-b h = ceil(NT/2) apart) when NT is not a multiple
+// of 8 times the group size or C >= 16.
+template <int C>
+__device__ __forceinline__ int xcd_tile(int bx, int NT, bool mirror) {
+  constexpr int W = C >= 16 ? 1 : 16 / C;
+  const int g = mirror ? 2 * W : W;
+  if (W > 1 && NT % (8 * g) == 0) {
+    const int m = bx >> 3, r = bx & 7;
+    const int q = r + 8 * (m / g), slot = m % g;
+    if (!mirror) return W * q + slot;
+    return slot < W ? W * q + slot : NT - 1 - (W * q + (slot - W));
+  }
+  if (!mirror) return bx;
+  const int h = (NT + 1) / 2;
+  return bx < h ? bx : NT - 1 - (bx - h);
+}
+
 // The padded axes have length L = R1 R2 = 2N with data (forward input) and
 // kept results (inverse output) in n < N.  Element n = t + R1 n2 of thread t
 // (t < R1) lies below N for every t when 2 n2 + 1 < R2 and above it when

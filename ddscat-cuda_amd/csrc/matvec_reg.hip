@@ -98,7 +98,7 @@ __global__ __launch_bounds__(R1 * C) void kr_passy(const float2* __restrict__ in
   in += rb * (DIR > 0 ? ba.a1s : ba.a2s);
   out += rb * (DIR > 0 ? ba.a2s : ba.a1s);
   const int s = threadIdx.x % C, t = threadIdx.x / C;
-  const int kx = blockIdx.x * C + s, iz = blockIdx.y, c = blockIdx.z % 3;
+  const int kx = xcd_tile<C>((int)blockIdx.x, (int)gridDim.x, false) * C + s, iz = blockIdx.y, c = blockIdx.z % 3;
   for (int i = threadIdx.x; i < L; i += RT) twL[i] = twg[i];
   const bool live = kx < gx;
   const float2* src = in + ((size_t)c * nz + iz) * ins + kx;
@@ -256,8 +256,7 @@ __global__ __launch_bounds__(3 * R1 * C) void kr_pass3c(float2* __restrict__ a2,
   // XCD-aware grid order as in kr_pass3
   const int j = blockIdx.y;
   const int ky = j == 0 ? 0 : j == 1 ? ny : ((j & 1) ? gy - ((j - 2) / 2 + 1) : (j - 2) / 2 + 1);
-  const int NT = gridDim.x, h = (NT + 1) / 2;
-  const int tile = (int)blockIdx.x < h ? (int)blockIdx.x : NT - 1 - ((int)blockIdx.x - h);
+  const int tile = xcd_tile<C>((int)blockIdx.x, (int)gridDim.x, true);
   const int kx = tile * C + s;
   for (int i = threadIdx.x; i < L; i += RT) twL[i] = twg[i];
   const bool live = kx < gx;
@@ -697,7 +696,11 @@ static const RegShape kShapes[] = {
     DDH_SHAPE(120, 8, 15, 32),  DDH_SHAPE(128, 8, 16, 32),  DDH_SHAPE(128, 16, 8, 16), DDH_SHAPE(144, 12, 12, 16),
     DDH_SHAPE(144, 16, 9, 16),  DDH_SHAPE(150, 15, 10, 16), DDH_SHAPE(160, 16, 10, 16), DDH_SHAPE(180, 15, 12, 16),
     DDH_SHAPE(192, 16, 12, 16), DDH_SHAPE(240, 16, 15, 16), DDH_SHAPE(256, 16, 16, 16),
-    // 768 (the 384^3 sphere): 32 values per thread and component
+    // 512 and 768 (256^3 and 384^3 targets): 32 values per thread and component;
+    // single-component passes in registers, pass 3 split over component groups
+    DDH_SHAPE1(512, 32, 16, 8), DDH_SHAPE1(512, 32, 16, 4),
+    RegShape{512, 32, 16, 8, &launch_split3<32, 16, 8>, 1 << 3},
+    RegShape{512, 32, 16, 4, &launch_split3<32, 16, 4>, 1 << 3},
     DDH_SHAPE1(768, 32, 24, 8), DDH_SHAPE1(768, 32, 24, 4),
     RegShape{768, 32, 24, 8, &launch_split3<32, 24, 8>, 1 << 3},
     RegShape{768, 32, 24, 4, &launch_split3<32, 24, 4>, 1 << 3},
