@@ -219,9 +219,14 @@ extern "C" DDSCAT_HIP_API int ddscat_hip_info(long long* out, int n) {
   const long long v[] = {c.nx, c.ny, c.nz, c.nat, c.nat0, (long long)c.green_count, (long long)c.matvec_count,
                          c.backend, c.precision, c.mv.gx, c.mv.gy, c.mv.gz, c.mv.rx_fwd, c.mv.cy, c.mv.cz,
                          c.mv.rx_inv, (long long)(c.green_build_ms * 1000.0), c.mv.kind[1], c.mv.kind[2],
-                         c.mv.kind[3], c.mv.kind[4], c.mv.kind[5], c.mv.tuned};
+                         c.mv.kind[3], c.mv.kind[4], c.mv.kind[5], c.mv.tuned, c.restarts};
   const int m = (int)(sizeof(v) / sizeof(v[0]));
   for (int i = 0; i < n && i < m; ++i) out[i] = v[i];
+  return 0;
+}
+
+extern "C" DDSCAT_HIP_API int ddscat_hip_force_restart(void) {
+  ctx().forced_gap = 1.0;
   return 0;
 }
 

@@ -32,6 +32,7 @@ int solver_begin(Context& c, const SolverArgs& a, int code_arg, int code_io, con
   }
   if (int rc = ensure_solver_workspace(c, 13)) return rc;
   c.solution_valid = false;
+  c.restarts = 0;
   const size_t fb = (size_t)3 * c.nat * sizeof(float2);
   hipStream_t st = c.stream;
   if (a.device) {
@@ -88,8 +89,13 @@ int reliable_check(Context& c, const float2* rhs, const float2* xcur, const floa
   const double gapn = std::sqrt(std::max(0.0, c.hscal[SL_D + 2]));
   rr.true_res = tr / rhsnorm;
   rr.gap = tr > 0.0 ? gapn / tr : (gapn == 0.0 ? 0.0 : INFINITY);
+  if (c.forced_gap > 0.0) {  // ddscat_hip_force_restart: one forced large gap
+    rr.gap += c.forced_gap;
+    c.forced_gap = 0.0;
+  }
   rr.converged = rr.true_res <= tol;
   rr.restart = rr.gap >= RELIABLE_GAP || (rec_res <= tol && !rr.converged);
+  if (rr.restart) ++c.restarts;
   return 0;
 }
 

@@ -60,6 +60,8 @@ struct Context {
   double* hscal = nullptr;      // pinned host mirror
   bool solution_valid = false;
   unsigned long long matvec_count = 0;
+  int restarts = 0;               // reliable-residual restarts of the last single-RHS solve
+  double forced_gap = 0.0;        // added once to the next reliable check's gap (test hook)
   double green_build_ms = 0.0;
 };
 

@@ -31,14 +31,15 @@ EXTENSION_SYMBOLS = [
     "ddscat_hip_time_matvec", "ddscat_hip_set_fft_kernels", "ddscat_hip_solve_device",
     "ddscat_hip_solve_batch", "ddscat_hip_apply_batch", "ddscat_hip_evalq_batch",
     "ddscat_hip_cabs_adjoint", "ddscat_hip_setup_rhs", "ddscat_hip_set_autotune",
-    "ddscat_hip_nearfield", "ddscat_hip_set_pass_kernel", "ddscat_hip_scat_matrix",
+    "ddscat_hip_nearfield", "ddscat_hip_set_pass_kernel", "ddscat_hip_scat_matrix", "ddscat_hip_force_restart",
 ]
 # method ids of ddscat_hip_solve_device
 SOLVER_IDS = {"pbcgs2": 0, "qmrccg": 1, "petrkp": 2, "gpbicg": 3, "pbcgst": 4, "bicgstab4": 5,
               "gpbicgstab2": 6, "gpbicgstab4": 7}
 INFO_FIELDS = ("nx", "ny", "nz", "nat", "nat0", "green_count", "matvec_count", "backend", "precision",
                "gx", "gy", "gz", "rx_fwd", "cy", "cz", "rx_inv", "green_build_us",
-               "kernel_pass1", "kernel_pass2", "kernel_pass3", "kernel_pass4", "kernel_pass5", "autotuned")
+               "kernel_pass1", "kernel_pass2", "kernel_pass3", "kernel_pass4", "kernel_pass5", "autotuned",
+               "restarts")
 
 BACKEND_FFT3D, BACKEND_SLICES = 0, 1
 PREC_MIXED, PREC_FP32 = 0, 1
@@ -138,6 +139,8 @@ class Library:
         L.ddscat_hip_nearfield.restype = i
         L.ddscat_hip_scat_matrix.argtypes = [i, vp, i, vp, vp, vp, vp, vp, vp, vp, f, vp, vp]
         L.ddscat_hip_scat_matrix.restype = i
+        L.ddscat_hip_force_restart.argtypes = []
+        L.ddscat_hip_force_restart.restype = i
         L.ddscat_hip_set_pass_kernel.argtypes = [i, i]
         L.ddscat_hip_set_pass_kernel.restype = i
         self.nat = self.nat0 = 0
@@ -356,6 +359,10 @@ class Library:
         if check:
             self._check("set_pass_kernel", rc)
         return rc
+
+    def force_restart(self):
+        """The next reliable-residual check of a single-RHS solve restarts (test hook)."""
+        return self._check("force_restart", self.lib.ddscat_hip_force_restart())
 
     def set_autotune(self, on):
         return self._check("set_autotune", self.lib.ddscat_hip_set_autotune(1 if on else 0))

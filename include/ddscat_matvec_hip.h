@@ -239,6 +239,12 @@ DDSCAT_HIP_API int ddscat_hip_scat_matrix(int npair, const void *p, int ndir, co
                                           const float *aks, const float *em1, const float *em2, const float *dx,
                                           const float *x0, float e02, void *famp, double *mueller);
 
+/* Test hook for the reliable-residual policy (cu:543-556): the next
+ * reliable check of a single-RHS solve sees its gap raised by 1 (above the
+ * 1e-3 restart threshold) and restarts from the true residual.  The restart
+ * count of the last single-RHS solve is ddscat_hip_info field 23. */
+DDSCAT_HIP_API int ddscat_hip_force_restart(void);
+
 /* Copy the resident spectral Green tensor (ddscat_hip_info field 5 complex64
  * values) to host memory in the CXZC layout ddscat_cuda_prepare_f32 takes. */
 DDSCAT_HIP_API int ddscat_hip_get_green(void *out);
@@ -246,7 +252,8 @@ DDSCAT_HIP_API int ddscat_hip_get_green(void *out);
 /* Fill up to n int64 fields: nx ny nz nat nat0 green_count matvec_count
  * backend precision gx gy gz rx_fwd cy cz rx_inv green_build_us, then the FFT
  * kernel of passes 1..5 (-1 LDS-staged, k >= 0 register shape k of the axis
- * length) and whether the autotuner chose them. */
+ * length), whether the autotuner chose them, and the reliable-residual
+ * restarts of the last single-RHS solve. */
 DDSCAT_HIP_API int ddscat_hip_info(long long *out, int n);
 
 /* Run `reps` MATVECs on device pointers with HIP events between the five

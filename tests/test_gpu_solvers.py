@@ -195,3 +195,36 @@ def test_solve_device_matches_host_entry(lib, small, name):
     assert abs(tole_d - tole_h) <= 1e-12 + 1e-6 * tole_h
     cabs, cext, _, _ = lib.evalq(P["khat"] * P["kd"])
     assert cext > 0 and cabs > 0
+
+
+@pytest.mark.parametrize("name", ["pbcgs2", "qmrccg", "gpbicg", "bicgstab4"])
+def test_forced_reliable_restart_still_converges(lib, small, name):
+    """The reliable-residual policy (cu:543-556): a gap >= 1e-3 at a check
+    restarts the recursion from the true residual.  The hook forces one such
+    gap; the solve must restart and still reach the dense solution."""
+    P = small
+    tg = P["tg"]
+    lib.prepare_green(tg.nx, tg.ny, tg.nz, tg.iocc, P["khat"] * P["kd"])
+    tol = 1e-6
+    x0, it0, *_ = lib.solve(name, P["b"], P["adia"], P["aoff"], maxit=300, tol=tol)
+    lib.force_restart()
+    x, it, tole, nc, rc = lib.solve(name, P["b"], P["adia"], P["aoff"], maxit=300, tol=tol)
+    restarts = lib.info()["restarts"]
+    xo = x.astype(complex)[P["occ3"]]
+    err = np.linalg.norm(xo - P["xref"]) / np.linalg.norm(P["xref"])
+    print(f"{name}: restarts={restarts} it={it} (unforced {it0}) err={err:.2e}")
+    assert rc == 0
+    assert restarts >= 1
+    assert err <= 30 * tol
+
+
+def test_iocc_count_mismatch_codes(lib):
+    """IOCC with a different number of occupied sites than NAT0: prepare_green
+    returns 208 and prepare returns 6 (cu:688, cu:1016, cu:1025)."""
+    nx, ny, nz = 4, 4, 4
+    iocc = np.ones(nx * ny * nz, np.int16)
+    iocc[:5] = 0
+    assert lib.prepare_green(nx, ny, nz, iocc, (0.2, 0, 0), nat0=int(iocc.sum()) + 1, check=False) == 208
+    g = np.zeros(6 * (nx + 1) * (ny + 1) * (nz + 1), np.complex64)
+    assert lib.prepare(nx, ny, nz, g, iocc, nat0=int(iocc.sum()) - 1, check=False) == 6
+    lib.release()
