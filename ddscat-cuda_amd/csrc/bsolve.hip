@@ -14,6 +14,7 @@
 // kernel; the per-step dots and the scalar recurrence share the reduction's
 // finishing kernel.
 #include "solver_common.hpp"
+#include "dots.hpp"
 #include "solvers.hpp"
 #include "krylov_scalars.hpp"
 #include "../../include/ddscat_matvec_hip.h"
@@ -119,14 +120,6 @@ int bs_ensure(Context& c, BatchState& S, int B, int nvec) {
 }
 
 // ------------------------------------------------------------ reductions --
-struct BDot {
-  const float2* a[DOT_MAX];
-  const float2* b[DOT_MAX];
-  const float2* c[DOT_MAX];
-  int kind[DOT_MAX];
-  int nt;
-};
-
 template <typename Acc>
 __device__ __forceinline__ Acc wsum(Acc v) {
 #pragma unroll
@@ -156,42 +149,15 @@ __device__ __forceinline__ void put_partials(const Acc* re, const Acc* im, int n
   }
 }
 
-template <typename Acc, int NT>
-__global__ __launch_bounds__(BT) void kb_dots(BDot L, size_t n, const int* __restrict__ active,
+template <typename Acc, int NT, int NV>
+__global__ __launch_bounds__(BT) void kb_dots(DotOperands L, size_t n, const int* __restrict__ active,
                                               double* __restrict__ part, int nbk) {
   const int r = blockIdx.y;
   if (active && !active[r]) return;
-  const size_t off = (size_t)r * n;
   Acc re[NT], im[NT];
 #pragma unroll
   for (int t = 0; t < NT; ++t) { re[t] = 0; im[t] = 0; }
-  const size_t stride = (size_t)nbk * BT;
-  for (size_t i = (size_t)blockIdx.x * BT + threadIdx.x; i < n; i += stride) {
-#pragma unroll
-    for (int t = 0; t < NT; ++t) {
-      if (t < L.nt) {
-        const float2 a = L.a[t][off + i];
-        const int kd = L.kind[t];
-        if (kd == DOT_N2) {
-          re[t] += (Acc)a.x * (Acc)a.x + (Acc)a.y * (Acc)a.y;
-        } else if (kd >= DOT_D2) {
-          const float2 b = L.b[t][off + i];
-          Acc dx = (Acc)a.x - (Acc)b.x, dy = (Acc)a.y - (Acc)b.y;
-          if (kd == DOT_G2) {
-            const float2 cc = L.c[t][off + i];
-            dx -= (Acc)cc.x;
-            dy -= (Acc)cc.y;
-          }
-          re[t] += dx * dx + dy * dy;
-        } else {
-          const float2 b = L.b[t][off + i];
-          const Acc ay = kd == DOT_C ? -(Acc)a.y : (Acc)a.y;
-          re[t] += (Acc)a.x * (Acc)b.x - ay * (Acc)b.y;
-          im[t] += (Acc)a.x * (Acc)b.y + ay * (Acc)b.x;
-        }
-      }
-    }
-  }
+  dot_accumulate<Acc, NT, NV>(L, (size_t)r * n, (size_t)blockIdx.x * BT + threadIdx.x, (size_t)nbk * BT, n, re, im);
   put_partials<Acc, NT>(re, im, L.nt, part, r, nbk);
 }
 
@@ -209,6 +175,9 @@ enum FinOp {
   FIN_P2_CONVEX,    // p2::convex
 };
 
+// One workgroup per RHS, one wave per sum: lane l loads partials l, l + 64,
+// ... < nbk (all in flight), adds them in order, then a fixed butterfly;
+// the first lane then stores the sums and applies `op`.
 template <typename Acc>
 __global__ __launch_bounds__(BT) void kb_finish(const double* __restrict__ part, int nbk, int nsums,
                                                 double* __restrict__ scal, int dst, const int* __restrict__ active,
@@ -216,21 +185,23 @@ __global__ __launch_bounds__(BT) void kb_finish(const double* __restrict__ part,
   const int r = blockIdx.x;
   if (active && !active[r]) return;
   double* s = scal + (size_t)r * SCAL_COUNT;
-  __shared__ Acc sh[BT];
-  for (int k = 0; k < nsums; ++k) {
-    Acc v = 0;
+  __shared__ double sums[2 * DOT_MAX];
+  constexpr int PER_LANE = BMAX_BLOCKS / 64;
+  const int lane = threadIdx.x & 63;
+  for (int k = threadIdx.x >> 6; k < nsums; k += BT / 64) {
     const double* pp = part + ((size_t)r * 2 * DOT_MAX + k) * nbk;
-    for (int b = threadIdx.x; b < nbk; b += BT) v += (Acc)pp[b];
-    sh[threadIdx.x] = v;
-    __syncthreads();
-    for (int o = BT / 2; o > 0; o >>= 1) {
-      if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) s[dst + k] = (double)sh[0];
-    __syncthreads();
+    double p[PER_LANE];
+#pragma unroll
+    for (int j = 0; j < PER_LANE; ++j) p[j] = pp[min(lane + 64 * j, nbk - 1)];
+    Acc v = 0;
+#pragma unroll
+    for (int j = 0; j < PER_LANE; ++j) v += lane + 64 * j < nbk ? (Acc)p[j] : (Acc)0;
+    v = wsum(v);
+    if (lane == 0) sums[k] = (double)v;
   }
+  __syncthreads();
   if (threadIdx.x != 0) return;
+  for (int k = 0; k < nsums; ++k) s[dst + k] = sums[k];
   switch (op) {
     case FIN_BNORM: {
       const double b = sqrt(fmax(s[SL_D], 0.0));
@@ -539,6 +510,49 @@ __global__ void kb_pack(const double* __restrict__ scal, double* __restrict__ ou
   out[3 * r + 2] = scal[(size_t)r * SCAL_COUNT + SL_FLAGS];
 }
 
+}  // namespace
+
+// ------------------------------------------------ single-RHS QMRCCG steps --
+// One RHS: grid.y = 1, no activity mask, scalars at c.scal.  The |r|^2
+// partials of dsxr go to c.red with the stride finalize_partials reads.
+static_assert(BMAX_BLOCKS == RED_MAX_BLOCKS, "single-RHS partials share the reduction buffer layout");
+static dim3 single_grid(size_t n) {
+  return dim3((unsigned)std::min<size_t>(RED_MAX_BLOCKS, std::max<size_t>(1, (n + (size_t)BT * 4 - 1) / ((size_t)BT * 4))));
+}
+
+int qmr1_pq(Context& c, float2* p, float2* q, const float2* vt, const float2* atw, size_t n) {
+  if (c.precision != PREC_FP32)
+    hipLaunchKernelGGL(kb_qmr_pq<double>, single_grid(n), dim3(BT), 0, c.stream, p, q, vt, atw, c.scal, n, nullptr);
+  else
+    hipLaunchKernelGGL(kb_qmr_pq<float>, single_grid(n), dim3(BT), 0, c.stream, p, q, vt, atw, c.scal, n, nullptr);
+  DDH_LAUNCH_OK("QMRCCG p, q update", 1);
+  return 0;
+}
+
+int qmr1_vw(Context& c, float2* vt, float2* wt, const float2* ap, const float2* q, size_t n) {
+  if (c.precision != PREC_FP32)
+    hipLaunchKernelGGL(kb_qmr_vw<double>, single_grid(n), dim3(BT), 0, c.stream, vt, wt, ap, q, c.scal, n, nullptr);
+  else
+    hipLaunchKernelGGL(kb_qmr_vw<float>, single_grid(n), dim3(BT), 0, c.stream, vt, wt, ap, q, c.scal, n, nullptr);
+  DDH_LAUNCH_OK("QMRCCG v~, w~ update", 1);
+  return 0;
+}
+
+int qmr1_dsxr(Context& c, float2* d, float2* sv, float2* x, float2* r, const float2* p, const float2* ap, size_t n,
+              int dst) {
+  const dim3 g = single_grid(n);
+  if (c.precision != PREC_FP32)
+    hipLaunchKernelGGL(kb_qmr_dsxr<double>, g, dim3(BT), 0, c.stream, d, sv, x, r, p, ap, c.scal, n, nullptr, c.red,
+                       RED_MAX_BLOCKS);
+  else
+    hipLaunchKernelGGL(kb_qmr_dsxr<float>, g, dim3(BT), 0, c.stream, d, sv, x, r, p, ap, c.scal, n, nullptr, c.red,
+                       RED_MAX_BLOCKS);
+  DDH_LAUNCH_OK("QMRCCG d, s, x, r update", 1);
+  return finalize_partials(c, (int)g.x, 2, dst, c.stream);
+}
+
+namespace {
+
 // ------------------------------------------------------------- host ops --
 struct Ops {
   Context& c;
@@ -570,35 +584,33 @@ struct Ops {
     bt.a2 = S.a2;
     return matvec_launch(c, 'N', x, y, adia, aoff, 1, st, nullptr, &bt);
   }
-  int dots(const DotSet& d, int dst, const int* mask, int op) {
-    BDot L{};
-    L.nt = d.n;
-    for (int t = 0; t < d.n; ++t) {
-      L.a[t] = d.t[t].a;
-      L.b[t] = d.t[t].b ? d.t[t].b : d.t[t].a;
-      L.c[t] = d.t[t].c ? d.t[t].c : L.b[t];
-      L.kind[t] = d.t[t].kind;
+  template <typename Acc>
+  struct DotsRun {
+    const DotOperands& L;
+    const Ops& o;
+    const int* mask;
+    template <int NT, int NV>
+    void run() const {
+      hipLaunchKernelGGL((kb_dots<Acc, NT, NV>), dim3((unsigned)o.nbk_red, (unsigned)o.S.B), dim3(BT), 0, o.st, L,
+                         o.S.n, mask, o.S.part, o.nbk_red);
     }
-    for (int t = d.n; t < DOT_MAX; ++t) { L.a[t] = L.a[0]; L.b[t] = L.b[0]; L.c[t] = L.c[0]; L.kind[t] = DOT_N2; }
-    const dim3 g((unsigned)nbk_red, (unsigned)S.B);
-#define DDH_BDOTS(ACC_, NT_) hipLaunchKernelGGL((kb_dots<ACC_, NT_>), g, dim3(BT), 0, st, L, S.n, mask, S.part, nbk_red)
-#define DDH_BDOTS_N(ACC_)                          \
-  if (d.n <= 1) DDH_BDOTS(ACC_, 1);                \
-  else if (d.n <= 2) DDH_BDOTS(ACC_, 2);           \
-  else if (d.n <= 4) DDH_BDOTS(ACC_, 4);           \
-  else if (d.n <= 6) DDH_BDOTS(ACC_, 6);           \
-  else DDH_BDOTS(ACC_, DOT_MAX);
+  };
+  int dots(const DotSet& d, int dst, const int* mask, int op) {
+    DotOperands L;
+    const int nv = dot_operands(d, 0, d.n, L);
+    if (nv < 0) {
+      set_error("batched dots", "too many distinct operand vectors");
+      return 1;
+    }
     if (f64) {
-      DDH_BDOTS_N(double)
+      dot_dispatch(d.n, nv, DotsRun<double>{L, *this, mask});
       hipLaunchKernelGGL(kb_finish<double>, dim3(S.B), dim3(BT), 0, st, S.part, nbk_red, 2 * d.n, S.scal, dst,
                          mask, op);
     } else {
-      DDH_BDOTS_N(float)
+      dot_dispatch(d.n, nv, DotsRun<float>{L, *this, mask});
       hipLaunchKernelGGL(kb_finish<float>, dim3(S.B), dim3(BT), 0, st, S.part, nbk_red, 2 * d.n, S.scal, dst,
                          mask, op);
     }
-#undef DDH_BDOTS_N
-#undef DDH_BDOTS
     DDH_LAUNCH_OK("batched dots", 1);
     return 0;
   }

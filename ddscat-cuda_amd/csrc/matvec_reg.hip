@@ -496,13 +496,103 @@ __global__ __launch_bounds__(2 * Q * C) void kr_pass3p(float2* __restrict__ a2, 
 }
 
 // ------------------------------------------------------------------ pass 5
+// finish of row g (cu:116-118): y = adia x - w/G + offdiag(aoff) x on the
+// unpadded outputs ix = t + R1 n2 < nx; w0..w2 hold the inverse-transformed
+// components at index n2 (compact or full layout, conj for 'C').  Site
+// indices are resolved first with the layout test outside the element loop;
+// the loads then run unconditionally in groups of FG outputs (an unoccupied
+// site reads entry 0 and stores nothing), so each group's 9 FG loads are in
+// flight together instead of waiting on per-output branches.
+template <int R1, int R2, int FG>
+__device__ __forceinline__ void p5_finish(const float2* w0, const float2* w1, const float2* w2, int t, int g,
+                                          const float2* __restrict__ x, float2* __restrict__ y,
+                                          const float2* __restrict__ adia, const float2* __restrict__ aoff,
+                                          const int16_t* __restrict__ iocc, const SiteMap& sm, int nx, int nat,
+                                          int nat0, int compact, int cj, float inv) {
+  constexpr int H = (R2 + 1) / 2;  // outputs n2 >= H lie in the padding
+  size_t o[H];
+  bool st[H], zr[H];
+  if (!compact) {
+#pragma unroll
+    for (int n2 = 0; n2 < H; ++n2) {
+      const int ix = t + R1 * n2;
+      st[n2] = lower_half<R2>(n2, ix, nx);
+      o[n2] = st[n2] ? (size_t)g * nx + ix : 0;
+      zr[n2] = iocc && iocc[o[n2]] == 0;
+    }
+  } else if (sm.contiguous) {
+    const int lo = sm.rowlo[g], q0 = sm.rowptr[g], cnt = sm.rowptr[g + 1] - q0;
+#pragma unroll
+    for (int n2 = 0; n2 < H; ++n2) {
+      const int ix = t + R1 * n2, d = ix - lo;
+      st[n2] = lower_half<R2>(n2, ix, nx) && d >= 0 && d < cnt;
+      o[n2] = st[n2] ? (size_t)(q0 + d) : 0;
+      zr[n2] = false;
+    }
+  } else {
+#pragma unroll
+    for (int n2 = 0; n2 < H; ++n2) {
+      const int ix = t + R1 * n2;
+      const int q = lower_half<R2>(n2, ix, nx) ? sm.qmap[(size_t)g * nx + ix] : -1;
+      st[n2] = q >= 0;
+      o[n2] = st[n2] ? (size_t)q : 0;
+      zr[n2] = false;
+    }
+  }
+  const size_t sd = compact ? (size_t)nat0 : (size_t)nat;
+#pragma unroll
+  for (int g0 = 0; g0 < H; g0 += FG) {
+    float2 xv[FG][3], dv[FG][3], av[FG][3];
+#pragma unroll
+    for (int j = 0; j < FG && g0 + j < H; ++j)
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const size_t e = o[g0 + j] + k * sd;
+        xv[j][k] = x[e];
+        dv[j][k] = adia[e];
+        av[j][k] = aoff[e];
+      }
+#pragma unroll
+    for (int j = 0; j < FG && g0 + j < H; ++j) {
+      const int n2 = g0 + j;
+      float2 x1 = xv[j][0], x2 = xv[j][1], x3 = xv[j][2];
+      const float2 a23 = av[j][0], a31 = av[j][1], a12 = av[j][2];
+      if (cj) { x1.y = -x1.y; x2.y = -x2.y; x3.y = -x3.y; }
+      float2 r1 = csub(cmul(dv[j][0], x1), cscale(w0[n2], inv));
+      float2 r2 = csub(cmul(dv[j][1], x2), cscale(w1[n2], inv));
+      float2 r3 = csub(cmul(dv[j][2], x3), cscale(w2[n2], inv));
+      r1 = cadd(r1, cadd(cmul(a31, x3), cmul(a12, x2)));
+      r2 = cadd(r2, cadd(cmul(a12, x1), cmul(a23, x3)));
+      r3 = cadd(r3, cadd(cmul(a23, x2), cmul(a31, x1)));
+      if (cj) { r1.y = -r1.y; r2.y = -r2.y; r3.y = -r3.y; }
+      if (zr[n2]) r1 = r2 = r3 = make_float2(0.f, 0.f);
+      if (st[n2]) {
+        const size_t e = o[n2];
+        y[e] = r1; y[e + sd] = r2; y[e + 2 * sd] = r3;
+      }
+    }
+  }
+}
+
+// pass-4 output of one component of row g in the register order of fft2_inv;
+// src of a row past the end is clamped to the last row by the caller (its
+// results are never stored), so only the register-ownership test remains
+template <int R1, int R2>
+__device__ __forceinline__ void p5_load(float2* v, const float2* __restrict__ src, int t) {
+#pragma unroll
+  for (int i = 0; i < Reg2<R1, R2>::G; ++i)
+#pragma unroll
+    for (int k1 = 0; k1 < R1; ++k1)
+      v[i * R1 + k1] = reg2_owns<R1, R2>(t, i) ? src[(t + R1 * i) + R2 * k1] : make_float2(0.f, 0.f);
+}
+
 template <int R1, int R2, int C>
 __global__ __launch_bounds__(R1 * C) void kr_pass5(const float2* __restrict__ a1, const float2* __restrict__ twg,
                                                const float2* __restrict__ x, float2* __restrict__ y,
                                                const float2* __restrict__ adia, const float2* __restrict__ aoff,
                                                const int16_t* __restrict__ iocc, SiteMap sm, int nx, int nrows,
                                                int nat, int nat0, int compact, int cj, float inv, BatchArgs ba) {
-  constexpr int L = R1 * R2, RT = R1 * C, P = R2 * (R1 + 1), G = Reg2<R1, R2>::G, VN = Reg2<R1, R2>::VN;
+  constexpr int L = R1 * R2, RT = R1 * C, P = R2 * (R1 + 1), VN = Reg2<R1, R2>::VN;
   __shared__ float2 twL[L];
   __shared__ float2 buf[C * P];  // row layout
   const int rb = blockIdx.y;
@@ -517,19 +607,11 @@ __global__ __launch_bounds__(R1 * C) void kr_pass5(const float2* __restrict__ a1
   for (int i = threadIdx.x; i < L; i += RT) twL[i] = twg[i];
   const bool live = g < nrows;
   const size_t cs = (size_t)nrows * L;
-  const float2* src = a1 + (size_t)g * L;
+  const float2* src = a1 + (size_t)(live ? g : nrows - 1) * L;
   float2 v0[VN], v1[VN], v2[VN];
-#pragma unroll
-  for (int i = 0; i < G; ++i)
-#pragma unroll
-    for (int k1 = 0; k1 < R1; ++k1) {
-      const int k = (t + R1 * i) + R2 * k1;
-      const int e = i * R1 + k1;
-      const bool ok = live && reg2_owns<R1, R2>(t, i);
-      v0[e] = ok ? src[k] : make_float2(0.f, 0.f);
-      v1[e] = ok ? src[k + cs] : make_float2(0.f, 0.f);
-      v2[e] = ok ? src[k + 2 * cs] : make_float2(0.f, 0.f);
-    }
+  p5_load<R1, R2>(v0, src, t);
+  p5_load<R1, R2>(v1, src + cs, t);
+  p5_load<R1, R2>(v2, src + 2 * cs, t);
   __syncthreads();
   fft2_inv<R1, R2, -1, 1>(v0, buf, twL, t, s, P);
   __syncthreads();
@@ -537,46 +619,7 @@ __global__ __launch_bounds__(R1 * C) void kr_pass5(const float2* __restrict__ a1
   __syncthreads();
   fft2_inv<R1, R2, -1, 1>(v2, buf, twL, t, s, P);
   if (!live) return;
-  int lo = 0, q0 = 0, cnt = 0;
-  if (compact && sm.contiguous) {
-    lo = sm.rowlo[g]; q0 = sm.rowptr[g]; cnt = sm.rowptr[g + 1] - q0;
-  }
-#pragma unroll
-  for (int n2 = 0; n2 < R2; ++n2) {
-    const int ix = t + R1 * n2;
-    if (!lower_half<R2>(n2, ix, nx)) continue;
-    size_t o, stride;
-    if (!compact) {
-      o = (size_t)g * nx + ix;
-      stride = (size_t)nat;
-      if (iocc && iocc[o] == 0) {
-        y[o] = y[o + stride] = y[o + 2 * stride] = make_float2(0.f, 0.f);
-        continue;
-      }
-    } else {
-      int q;
-      if (sm.contiguous) {
-        const int d = ix - lo;
-        q = (d >= 0 && d < cnt) ? q0 + d : -1;
-      } else {
-        q = sm.qmap[(size_t)g * nx + ix];
-      }
-      if (q < 0) continue;
-      o = (size_t)q;
-      stride = (size_t)nat0;
-    }
-    float2 x1 = x[o], x2 = x[o + stride], x3 = x[o + 2 * stride];
-    if (cj) { x1.y = -x1.y; x2.y = -x2.y; x3.y = -x3.y; }
-    const float2 a23 = aoff[o], a31 = aoff[o + stride], a12 = aoff[o + 2 * stride];
-    float2 r1 = csub(cmul(adia[o], x1), cscale(v0[n2], inv));
-    float2 r2 = csub(cmul(adia[o + stride], x2), cscale(v1[n2], inv));
-    float2 r3 = csub(cmul(adia[o + 2 * stride], x3), cscale(v2[n2], inv));
-    r1 = cadd(r1, cadd(cmul(a31, x3), cmul(a12, x2)));
-    r2 = cadd(r2, cadd(cmul(a12, x1), cmul(a23, x3)));
-    r3 = cadd(r3, cadd(cmul(a23, x2), cmul(a31, x1)));
-    if (cj) { r1.y = -r1.y; r2.y = -r2.y; r3.y = -r3.y; }
-    y[o] = r1; y[o + stride] = r2; y[o + 2 * stride] = r3;
-  }
+  p5_finish<R1, R2, 4>(v0, v1, v2, t, g, x, y, adia, aoff, iocc, sm, nx, nat, nat0, compact, cj, inv);
 }
 
 // ------------------------------------------------------------------ host --
@@ -705,13 +748,15 @@ static const RegShape kShapes[] = {
     RegShape{768, 32, 24, 8, &launch_split3<32, 24, 8>, 1 << 3},
     RegShape{768, 32, 24, 4, &launch_split3<32, 24, 4>, 1 << 3},
 };
-#define DDH_PAIR(L_, Q_, CR_, CP_) RegShape{L_, Q_, Q_, CR_, &launch_pair<Q_, CR_, CP_>}
-// lane-pair pass-3 candidates (listed after the defaults of their lengths)
+#define DDH_PAIR(L_, Q_, CR_, CP_) RegShape{L_, Q_, Q_, CR_, &launch_pair<Q_, CR_, CP_>, 1 << 3}
+// Autotuner candidates listed after the defaults of their lengths: lane-pair
+// and mirror-row pass-3 kernels, and a narrower pass-1 tile.
 static const RegShape kPairShapes[] = {
     DDH_PAIR(64, 8, 32, 32), DDH_PAIR(144, 12, 16, 16), DDH_PAIR(256, 16, 16, 16), DDH_PAIR(256, 16, 16, 8),
     DDH_SHAPE(256, 16, 16, 32),
-    RegShape{256, 16, 16, 16, &launch_ypair<16, 16, 16>}, RegShape{128, 8, 16, 32, &launch_ypair<8, 16, 32>},
-    RegShape{64, 8, 8, 32, &launch_ypair<8, 8, 32>},
+    RegShape{256, 16, 16, 16, &launch_ypair<16, 16, 16>, 1 << 3}, RegShape{128, 8, 16, 32, &launch_ypair<8, 16, 32>, 1 << 3},
+    RegShape{64, 8, 8, 32, &launch_ypair<8, 8, 32>, 1 << 3},
+    RegShape{256, 16, 16, 8, &launch_reg1<16, 16, 8>, 1 << 1},
 };
 #undef DDH_PAIR
 #undef DDH_SHAPE

@@ -1,56 +1,18 @@
 // FP64 (or FP32) multi-dot reductions, linear combinations and layout maps.
 #include "solver_common.hpp"
+#include "dots.hpp"
 
 namespace ddhip {
 
-struct DotLaunch {
-  const float2* a[DOT_MAX];
-  const float2* b[DOT_MAX];
-  const float2* c[DOT_MAX];
-  int kind[DOT_MAX];
-  int nt;
-};
-
-template <typename Acc>
-__device__ __forceinline__ Acc wave_sum(Acc v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
-}
-
-// NT: compile-time number of term slots (>= L.nt); accumulators for NT terms only
-template <typename Acc, int NT>
-__global__ __launch_bounds__(RED_THREADS) void k_dots(DotLaunch L, size_t n, double* __restrict__ part) {
+// NT: compile-time term slots (>= L.nt); NV: compile-time operand slots
+// (>= distinct vectors)
+template <typename Acc, int NT, int NV>
+__global__ __launch_bounds__(RED_THREADS) void k_dots(DotOperands L, size_t n, double* __restrict__ part) {
   Acc re[NT], im[NT];
 #pragma unroll
   for (int t = 0; t < NT; ++t) { re[t] = 0; im[t] = 0; }
-  const size_t stride = (size_t)gridDim.x * RED_THREADS;
-  for (size_t i = (size_t)blockIdx.x * RED_THREADS + threadIdx.x; i < n; i += stride) {
-#pragma unroll
-    for (int t = 0; t < NT; ++t) {
-      if (t < L.nt) {
-        const float2 a = L.a[t][i];
-        const int kd = L.kind[t];
-        if (kd == DOT_N2) {
-          re[t] += (Acc)a.x * (Acc)a.x + (Acc)a.y * (Acc)a.y;
-        } else if (kd >= DOT_D2) {
-          const float2 b = L.b[t][i];
-          Acc dx = (Acc)a.x - (Acc)b.x, dy = (Acc)a.y - (Acc)b.y;
-          if (kd == DOT_G2) {
-            const float2 cc = L.c[t][i];
-            dx -= (Acc)cc.x;
-            dy -= (Acc)cc.y;
-          }
-          re[t] += dx * dx + dy * dy;
-        } else {
-          const float2 b = L.b[t][i];
-          const Acc ay = L.kind[t] == 0 ? -(Acc)a.y : (Acc)a.y;
-          re[t] += (Acc)a.x * (Acc)b.x - ay * (Acc)b.y;
-          im[t] += (Acc)a.x * (Acc)b.y + ay * (Acc)b.x;
-        }
-      }
-    }
-  }
+  dot_accumulate<Acc, NT, NV>(L, 0, (size_t)blockIdx.x * RED_THREADS + threadIdx.x, (size_t)gridDim.x * RED_THREADS,
+                              n, re, im);
   __shared__ Acc sred[2 * NT][RED_THREADS / 64];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
@@ -69,51 +31,91 @@ __global__ __launch_bounds__(RED_THREADS) void k_dots(DotLaunch L, size_t n, dou
   }
 }
 
+// one wave per sum: lane l loads partials l, l + 64, ... (all in flight at
+// once; they were written on other XCDs and come from beyond this XCD's L2),
+// adds them in order, then a fixed butterfly; deterministic for a given
+// block count
+constexpr int FIN_THREADS = 1024;
 template <typename Acc>
-__global__ __launch_bounds__(RED_THREADS) void k_dots_finalize(const double* __restrict__ part, int nblocks,
+__global__ __launch_bounds__(FIN_THREADS) void k_dots_finalize(const double* __restrict__ part, int nblocks,
                                                               int nsums, double* __restrict__ out) {
-  __shared__ Acc sh[RED_THREADS];
-  for (int s = 0; s < nsums; ++s) {
+  constexpr int PER_LANE = RED_MAX_BLOCKS / 64;
+  const int lane = threadIdx.x & 63;
+  for (int s = threadIdx.x >> 6; s < nsums; s += FIN_THREADS / 64) {
+    const double* ps = part + (size_t)s * RED_MAX_BLOCKS + lane;
+    double p[PER_LANE];  // the buffer holds RED_MAX_BLOCKS partials per sum
+#pragma unroll
+    for (int j = 0; j < PER_LANE; ++j) p[j] = ps[64 * j];
     Acc v = 0;
-    for (int b = threadIdx.x; b < nblocks; b += RED_THREADS) v += (Acc)part[(size_t)s * RED_MAX_BLOCKS + b];
-    sh[threadIdx.x] = v;
-    __syncthreads();
-    for (int o = RED_THREADS / 2; o > 0; o >>= 1) {
-      if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) out[s] = (double)sh[0];
-    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < PER_LANE; ++j) v += lane + 64 * j < nblocks ? (Acc)p[j] : (Acc)0;
+    v = wave_sum(v);
+    if (lane == 0) out[s] = (double)v;
   }
 }
 
-int dots(Context& c, const DotSet& s, size_t n, int dst, hipStream_t st) {
-  DotLaunch L{};
-  L.nt = s.n;
-  for (int t = 0; t < s.n; ++t) {
-    L.a[t] = s.t[t].a;
-    L.b[t] = s.t[t].b ? s.t[t].b : s.t[t].a;
-    L.c[t] = s.t[t].c ? s.t[t].c : L.b[t];
-    L.kind[t] = s.t[t].kind;
+int finalize_partials(Context& c, int nblocks, int nsums, int dst, hipStream_t st) {
+  if (c.precision == PREC_FP32)
+    hipLaunchKernelGGL(k_dots_finalize<float>, dim3(1), dim3(FIN_THREADS), 0, st, c.red, nblocks, nsums, c.scal + dst);
+  else
+    hipLaunchKernelGGL(k_dots_finalize<double>, dim3(1), dim3(FIN_THREADS), 0, st, c.red, nblocks, nsums,
+                       c.scal + dst);
+  DDH_LAUNCH_OK("reduction finalize", 1);
+  return 0;
+}
+
+template <typename Acc>
+struct DotsRun {
+  const DotOperands& L;
+  size_t n;
+  int nb;
+  double* part;
+  hipStream_t st;
+  template <int NT, int NV>
+  void run() const {
+    hipLaunchKernelGGL((k_dots<Acc, NT, NV>), dim3(nb), dim3(RED_THREADS), 0, st, L, n, part);
   }
-  for (int t = s.n; t < DOT_MAX; ++t) { L.a[t] = L.a[0]; L.b[t] = L.b[0]; L.c[t] = L.c[0]; L.kind[t] = DOT_N2; }
+};
+
+// launches the terms [t0, t1) of s as one reduction (operand list fits DOT_VMAX)
+static int dots_chunk(Context& c, const DotSet& s, int t0, int t1, size_t n, int dst, hipStream_t st) {
+  DotOperands L;
+  const int nv = dot_operands(s, t0, t1, L);
+  if (nv < 0) {
+    set_error("dot reduction", "too many distinct operand vectors");
+    return 1;
+  }
   size_t want = (n + (size_t)RED_THREADS * 4 - 1) / ((size_t)RED_THREADS * 4);
   int nb = (int)std::min<size_t>(RED_MAX_BLOCKS, std::max<size_t>(1, want));
-  const bool f32 = c.precision == PREC_FP32;
-#define DDH_DOTS(NT_)                                                                                    \
-  if (f32) hipLaunchKernelGGL((k_dots<float, NT_>), dim3(nb), dim3(RED_THREADS), 0, st, L, n, c.red);  \
-  else hipLaunchKernelGGL((k_dots<double, NT_>), dim3(nb), dim3(RED_THREADS), 0, st, L, n, c.red);
-  if (s.n <= 1) { DDH_DOTS(1) }
-  else if (s.n <= 2) { DDH_DOTS(2) }
-  else if (s.n <= 4) { DDH_DOTS(4) }
-  else if (s.n <= 6) { DDH_DOTS(6) }
-  else { DDH_DOTS(DOT_MAX) }
-#undef DDH_DOTS
-  if (f32)
-    hipLaunchKernelGGL(k_dots_finalize<float>, dim3(1), dim3(RED_THREADS), 0, st, c.red, nb, 2 * s.n, c.scal + dst);
-  else
-    hipLaunchKernelGGL(k_dots_finalize<double>, dim3(1), dim3(RED_THREADS), 0, st, c.red, nb, 2 * s.n, c.scal + dst);
+  if (c.precision == PREC_FP32) dot_dispatch(L.nt, nv, DotsRun<float>{L, n, nb, c.red, st});
+  else dot_dispatch(L.nt, nv, DotsRun<double>{L, n, nb, c.red, st});
   DDH_LAUNCH_OK("FP64 dot reduction", 1);
+  return finalize_partials(c, nb, 2 * L.nt, dst, st);
+}
+
+int dots(Context& c, const DotSet& s, size_t n, int dst, hipStream_t st) {
+  // split where a term would bring the distinct operands past DOT_VMAX
+  int t0 = 0;
+  while (t0 < s.n) {
+    const float2* seen[3 * DOT_MAX];
+    int nv = 0, t = t0;
+    for (; t < s.n; ++t) {
+      const float2* ops[3] = {s.t[t].a, s.t[t].b ? s.t[t].b : s.t[t].a,
+                              s.t[t].c ? s.t[t].c : (s.t[t].b ? s.t[t].b : s.t[t].a)};
+      int add = 0;
+      const float2* fresh[3];
+      for (const float2* p : ops) {
+        bool have = false;
+        for (int v = 0; v < nv && !have; ++v) have = seen[v] == p;
+        for (int v = 0; v < add && !have; ++v) have = fresh[v] == p;
+        if (!have) fresh[add++] = p;
+      }
+      if (nv + add > DOT_VMAX) break;
+      for (int v = 0; v < add; ++v) seen[nv++] = fresh[v];
+    }
+    if (int rc = dots_chunk(c, s, t0, t, n, dst + 2 * t0, st)) return rc;
+    t0 = t;
+  }
   return 0;
 }
 

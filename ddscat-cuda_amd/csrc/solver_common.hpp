@@ -38,6 +38,9 @@ struct DotSet {
 // Computes the dots of `s` over n complex elements; result i is written as
 // (re, im) to c.scal[dst + 2 i], c.scal[dst + 2 i + 1].  Stream-ordered, no sync.
 int dots(Context& c, const DotSet& s, size_t n, int dst, hipStream_t st);
+// Adds the per-workgroup partials c.red[k][0..nblocks) (stride RED_MAX_BLOCKS)
+// of nsums real sums into c.scal[dst + k], in the precision of c.precision.
+int finalize_partials(Context& c, int nblocks, int nsums, int dst, hipStream_t st);
 
 // Generic linear combination d = sum_i coef_i v_i over n elements.  A
 // coefficient is either a device scalar (complex double at c.scal[idx]) or a

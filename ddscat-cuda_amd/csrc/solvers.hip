@@ -304,35 +304,19 @@ int solve_qmrccg(Context& c, const SolverArgs& a) {
   int it = 0;
   while (it < a.maxit && resid > a.tol) {
     ++it;
+    // fused vector steps; the |r|^2 of the update goes to SL_D + 8
     // p = v~/gamma - mu p ; q = (A w~ - gamma mu q)/ksi   (cu:437)
-    { LinComb l; l.dev(vt, qmr::GINV).dev(p, qmr::MU, 1); if (lincomb(c, p, l, n, st)) return 56; }
-    {
-      LinComb l1; l1.lit(atw, 1.0).dev(q, qmr::GMU, 1);
-      if (lincomb(c, q, l1, n, st)) return 56;
-      LinComb l2; l2.dev(q, qmr::KINV);
-      if (lincomb(c, q, l2, n, st)) return 56;
-    }
+    if (qmr1_pq(c, p, q, vt, atw, n)) return 56;
     if (int rc = run_mv(c, 'N', p, ap, nc)) return rc;
     // v~ = Ap - (tau/gamma) v~ ; w~ = q - (tau/ksi) w~   (cu:438)
-    { LinComb l; l.lit(ap, 1.0).dev(vt, qmr::TG, 1); if (lincomb(c, vt, l, n, st)) return 56; }
-    { LinComb l; l.lit(q, 1.0).dev(wt, qmr::TK, 1); if (lincomb(c, wt, l, n, st)) return 56; }
+    if (qmr1_vw(c, vt, wt, ap, q, n)) return 56;
     if (int rc = run_mv(c, 'N', wt, atw, nc)) return rc;
     DotSet ds;
     ds.add(vt, vt, DOT_N2).add(wt, wt, DOT_N2).add(vt, wt, DOT_U).add(vt, atw, DOT_U);
     if (dots(c, ds, n, SL_D, st)) return 57;
     hipLaunchKernelGGL(qmr::k_recur, dim3(1), dim3(1), 0, st, c.scal);
     // d = theta d + kappa p ; s = theta s + kappa Ap ; x += d ; r -= s   (cu:443)
-    {
-      LinComb ld; ld.dev(d, qmr::THETA).dev(p, qmr::KAPPA);
-      LinComb ls; ls.dev(sv, qmr::THETA).dev(ap, qmr::KAPPA);
-      LinComb lx; lx.lit(xs, 1.0).lit(d, 1.0);
-      LinComb lr; lr.lit(r, 1.0).lit(sv, -1.0);
-      if (lincomb(c, d, ld, n, st) || lincomb(c, sv, ls, n, st) || lincomb(c, xs, lx, n, st) ||
-          lincomb(c, r, lr, n, st))
-        return 59;
-    }
-    DotSet dr; dr.add(r, r, DOT_N2);
-    if (dots(c, dr, n, SL_D + 8, st)) return 59;
+    if (qmr1_dsxr(c, d, sv, xs, r, p, ap, n, SL_D + 8)) return 59;
     if (fetch_scalars(c, 0, SL_D + 10, st)) return 59;
     if (c.hscal[SL_STATUS] != 0.0) {
       char msg[80];

@@ -46,6 +46,17 @@ int solver_end(Context& c, const SolverArgs& a, const float2* xs, int it, double
 int reliable_check(Context& c, const float2* rhs, const float2* xcur, const float2* rrec, double rhsnorm,
                    double rec_res, double tol, int& nc, Reliable& rr);
 
+// Single-RHS QMRCCG vector steps (cu:437, cu:438, cu:443) on the fused
+// kernels of the batched solver with one RHS; coefficients from c.scal.
+//   pq:   p = v~/gamma - mu p ; q = (A w~ - gamma mu q)/ksi
+//   vw:   v~ = Ap - (tau/gamma) v~ ; w~ = q - (tau/ksi) w~
+//   dsxr: d = theta d + kappa p ; s = theta s + kappa Ap ; x += d ; r -= s,
+//         with |r|^2 into c.scal[dst] (imaginary part 0 at dst + 1)
+int qmr1_pq(Context& c, float2* p, float2* q, const float2* vt, const float2* atw, size_t n);
+int qmr1_vw(Context& c, float2* vt, float2* wt, const float2* ap, const float2* q, size_t n);
+int qmr1_dsxr(Context& c, float2* d, float2* sv, float2* x, float2* r, const float2* p, const float2* ap, size_t n,
+              int dst);
+
 int solve_pbcgs2(Context& c, const SolverArgs& a);
 int solve_qmrccg(Context& c, const SolverArgs& a);
 int solve_petrkp(Context& c, const SolverArgs& a);
