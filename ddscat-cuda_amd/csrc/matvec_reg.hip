@@ -50,27 +50,49 @@ __global__ __launch_bounds__(R1 * C) void kr_pass1(const float2* __restrict__ x,
   for (int i = threadIdx.x; i < L; i += RT) twL[i] = twg[i];
   float2 v[VN];
   const bool live = g < nrows;
-  int lo = 0, q0 = 0, cnt = 0;
-  if (compact && sm.contiguous && live) {
-    lo = sm.rowlo[g]; q0 = sm.rowptr[g]; cnt = sm.rowptr[g + 1] - q0;
+  // site of each unpadded input n = t + R1 n2 (n2 < H), resolved with the
+  // layout test outside the element loop; the loads then run unconditionally
+  // (an empty site or a row past the end reads entry 0 and is zeroed), so
+  // they are in flight together instead of waiting on per-element branches
+  constexpr int H = (R2 + 1) / 2;
+  size_t o[H];
+  bool ok[H];
+  if (!compact) {
+#pragma unroll
+    for (int n2 = 0; n2 < H; ++n2) {
+      const int n = t + R1 * n2;
+      ok[n2] = live && lower_half<R2>(n2, n, nx);
+      o[n2] = ok[n2] ? (size_t)g * nx + n : 0;
+    }
+  } else if (sm.contiguous) {
+    int lo = 0, q0 = 0, cnt = 0;
+    if (live) { lo = sm.rowlo[g]; q0 = sm.rowptr[g]; cnt = sm.rowptr[g + 1] - q0; }
+#pragma unroll
+    for (int n2 = 0; n2 < H; ++n2) {
+      const int n = t + R1 * n2, d = n - lo;
+      ok[n2] = live && lower_half<R2>(n2, n, nx) && d >= 0 && d < cnt;
+      o[n2] = ok[n2] ? (size_t)(q0 + d) : 0;
+    }
+  } else {
+#pragma unroll
+    for (int n2 = 0; n2 < H; ++n2) {
+      const int n = t + R1 * n2;
+      const int q = live && lower_half<R2>(n2, n, nx) ? sm.qmap[(size_t)g * nx + n] : -1;
+      ok[n2] = q >= 0;
+      o[n2] = ok[n2] ? (size_t)q : 0;
+    }
+  }
+  const float2* xc = x + (size_t)c * (compact ? nat0 : nat);
+#pragma unroll
+  for (int n2 = 0; n2 < H; ++n2) {
+    const float2 val = xc[o[n2]];
+    v[n2] = ok[n2] ? val : make_float2(0.f, 0.f);
   }
 #pragma unroll
-  for (int n2 = 0; n2 < R2; ++n2) {
-    const int n = t + R1 * n2;
-    float2 val = make_float2(0.f, 0.f);
-    if (live && lower_half<R2>(n2, n, nx)) {
-      if (!compact) {
-        val = x[(size_t)c * nat + (size_t)g * nx + n];
-      } else if (sm.contiguous) {
-        const int d = n - lo;
-        if (d >= 0 && d < cnt) val = x[(size_t)c * nat0 + q0 + d];
-      } else {
-        const int q = sm.qmap[(size_t)g * nx + n];
-        if (q >= 0) val = x[(size_t)c * nat0 + q];
-      }
-      if (cj) val.y = -val.y;
-    }
-    v[n2] = val;
+  for (int n2 = H; n2 < R2; ++n2) v[n2] = make_float2(0.f, 0.f);
+  if (cj) {
+#pragma unroll
+    for (int n2 = 0; n2 < H; ++n2) v[n2].y = -v[n2].y;
   }
   __syncthreads();
   fft2_fwd_s<R1, R2, +1, 1>(v, buf, twL, t, s, P);
