@@ -338,6 +338,133 @@ __global__ __launch_bounds__(3 * R1 * C) void kr_pass3c(float2* __restrict__ a2,
   }
 }
 
+// ------------------------------------------------ pass 3, L = M * 256
+// e^{DIR 2 pi i q / M} for M = 2, 3
+template <int M, int DIR>
+__device__ __forceinline__ float2 root_m(int q) {
+  if (M == 2) return make_float2(q ? -1.f : 1.f, 0.f);
+  constexpr float h = 0.8660254037844386f;  // sqrt(3) / 2
+  return q == 0 ? make_float2(1.f, 0.f) : make_float2(-0.5f, ((q == 1) == (DIR > 0)) ? h : -h);
+}
+// output m of an M-point DFT (sign DIR) of z[0..M)
+template <int M, int DIR>
+__device__ __forceinline__ float2 dft_m_out(const float2* z, int m) {
+  float2 r = z[0];
+#pragma unroll
+  for (int q = 1; q < M; ++q) r = cadd(r, cmul(root_m<M, DIR>((q * m) % M), z[q]));
+  return r;
+}
+
+// Pass 3 for L = M * 256 (M = 2: 512, M = 3: 768 points) by decimation in
+// time over 256-point register FFTs (16 x 16).  Thread group m of M
+// transforms x[M n1 + m] (zero for n1 >= 128, pruned at compile time as in
+// the 256-point kernels); the twiddle w_L^{m k1} and an M-point DFT through
+// LDS leave group m holding X[k1 + 256 m] for its k1 = t + 16 j.  The
+// inverse runs the same steps backwards.  A thread keeps 16 values per
+// component, so the three components and the Green multiply stay in
+// registers; two-level shapes of 512 or 768 points hold 32 values per
+// component and have to split the components over thread groups.
+template <int M, int C>
+__global__ __launch_bounds__(M * 16 * C) void kr_pass3m(float2* __restrict__ a2, const float2* __restrict__ green,
+                                                    const float2* __restrict__ twg, int nx, int ny, int nz, int gx,
+                                                    int gy, size_t plane, BatchArgs ba, GreenGeo gg) {
+  constexpr int R = 16, S = R * R, L = M * S, GT = R * C, RT = M * GT;
+  constexpr int PS = M * C + 1;  // 256-point exchange: M C sequences per workgroup
+  constexpr int PC = C + 1;      // M-point exchange layout (q S + k1) PC + s
+  constexpr int XB = S * PS > M * S * PC ? S * PS : M * S * PC;
+  __shared__ float2 twL[L];
+  __shared__ float2 twS[S];
+  __shared__ float2 xb[XB];
+  const int rb = blockIdx.z;
+  if (ba.active && !ba.active[rb]) return;
+  a2 += rb * ba.a2s;
+  const int m = threadIdx.x / GT, lt = threadIdx.x % GT;
+  const int s = lt % C, t = lt / C, sv = m * C + s;
+  // XCD-aware grid order as in kr_pass3
+  const int j = blockIdx.y;
+  const int ky = j == 0 ? 0 : j == 1 ? ny : ((j & 1) ? gy - ((j - 2) / 2 + 1) : (j - 2) / 2 + 1);
+  const int tile = xcd_tile<C>((int)blockIdx.x, (int)gridDim.x, true);
+  const int kx = tile * C + s;
+  for (int i = threadIdx.x; i < L; i += RT) twL[i] = twg[i];
+  for (int i = threadIdx.x; i < S; i += RT) twS[i] = twg[M * i];
+  const bool live = kx < gx;
+  const size_t cs = (size_t)nz * plane;
+  float2* col = a2 + (size_t)ky * gx + kx;
+  float2 v[3][R];
+#pragma unroll
+  for (int n2 = 0; n2 < R; ++n2) {
+    const bool ok = live && n2 < R / 2;
+    const size_t o = (size_t)(M * (t + R * n2) + m) * plane;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) v[c][n2] = ok ? col[o + c * cs] : make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    if (c) __syncthreads();
+    fft2_fwd<R, R, +1>(v[c], xb, twS, t, sv, PS);  // v[c][i] = Y_m[t + 16 i]
+#pragma unroll
+    for (int i = 0; i < R; ++i) v[c][i] = cmul(v[c][i], twL[m * (t + R * i)]);
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < R; ++i) xb[(m * S + t + R * i) * PC + s] = v[c][i];
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+      float2 z[M];
+#pragma unroll
+      for (int q = 0; q < M; ++q) z[q] = xb[(q * S + t + R * i) * PC + s];
+      v[c][i] = dft_m_out<M, +1>(z, m);  // X[t + 16 i + 256 m]
+    }
+  }
+  // 3x3 symmetric Green multiply at kz = t + 16 i + 256 m (octant parity
+  // reconstruction for IPBC=0, cu:109-115; whole grid for IPBC=1)
+  const int ox = gg.ox, oy = gg.oy;
+  const int ir = live ? min(kx, gg.mx - kx) : 0;
+  const int jr = min(ky, gg.my - ky);
+  const float sxy = ((kx <= gg.tx) == (ky <= gg.ty)) ? 1.f : -1.f;
+  const float sx = kx <= gg.tx ? 1.f : -1.f, sy = ky <= gg.ty ? 1.f : -1.f;
+  const size_t pb = (size_t)ir + (size_t)ox * jr;
+#pragma unroll
+  for (int i = 0; i < R; ++i) {
+    const int kz = t + R * i + S * m;
+    const int kr = min(kz, gg.mz - kz);
+    const float sz = kz <= gg.tz ? 1.f : -1.f;
+    const Green6 G = green_at(green, pb + (size_t)ox * oy * kr);
+    const float2 g12 = cscale(G.xy, sxy), g13 = cscale(G.xz, sx * sz), g23 = cscale(G.yz, sy * sz);
+    const float2 p0 = v[0][i], p1 = v[1][i], p2 = v[2][i];
+    v[0][i] = cadd(cadd(cmul(G.xx, p0), cmul(g12, p1)), cmul(g13, p2));
+    v[1][i] = cadd(cadd(cmul(g12, p0), cmul(G.yy, p1)), cmul(g23, p2));
+    v[2][i] = cadd(cadd(cmul(g13, p0), cmul(g23, p1)), cmul(G.zz, p2));
+  }
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < R; ++i) xb[(m * S + t + R * i) * PC + s] = v[c][i];
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+      float2 z[M];
+#pragma unroll
+      for (int q = 0; q < M; ++q) z[q] = xb[(q * S + t + R * i) * PC + s];
+      float2 w = twL[m * (t + R * i)];
+      w.y = -w.y;
+      v[c][i] = cmul(dft_m_out<M, -1>(z, m), w);  // W_m[t + 16 i]
+    }
+    __syncthreads();
+    fft2_inv<R, R, -1>(v[c], xb, twS, t, sv, PS);  // v[c][n2] = x[M (t + 16 n2) + m]
+  }
+  if (live) {
+#pragma unroll
+    for (int n2 = 0; n2 < R / 2; ++n2) {
+      const size_t o = (size_t)(M * (t + R * n2) + m) * plane;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) col[o + c * cs] = v[c][n2];
+    }
+  }
+}
+
 // Pass 3 with the mirror rows ky and gy - ky in the two halves of one
 // workgroup: both halves read the same Green octant entries at the same
 // time, so one of each pair of loads is served by the CU's L1.
@@ -725,6 +852,17 @@ static void launch_reg1(Context& c, int cj, int compact, const float2* x, float2
                        bt.a1, m.twy, m.gx, m.gy, c.ny, c.nz, m.p2, m.p1, bt.k);
 }
 
+// pass 3 by kr_pass3m; serves pass 3 only, listed for L = gz = 2 NZ = M * 256
+template <int M, int C>
+static void launch_p3m(Context& c, int cj, int compact, const float2* x, float2* y, const float2* adia,
+                       const float2* aoff, hipStream_t st, const MvBatch& bt, int which) {
+  (void)cj; (void)compact; (void)x; (void)y; (void)adia; (void)aoff;
+  if (which != 3) return;
+  const MatvecPlan& m = c.mv;
+  hipLaunchKernelGGL((kr_pass3m<M, C>), dim3((m.gx + C - 1) / C, m.gy, (unsigned)bt.nrhs), dim3(M * 16 * C), 0, st,
+                     bt.a2, c.green, m.twz, c.nx, c.ny, c.nz, m.gx, m.gy, m.p2, bt.k, green_geo(c));
+}
+
 // pass 3 by component-split thread groups (kr_pass3c); serves pass 3 only
 template <int R1, int R2, int C>
 static void launch_split3(Context& c, int cj, int compact, const float2* x, float2* y, const float2* adia,
@@ -764,9 +902,12 @@ static const RegShape kShapes[] = {
     // 512 and 768 (256^3 and 384^3 targets): 32 values per thread and component;
     // single-component passes in registers, pass 3 split over component groups
     DDH_SHAPE1(512, 32, 16, 8), DDH_SHAPE1(512, 32, 16, 4),
+    RegShape{512, 16, 16, 8, &launch_p3m<2, 8>, 1 << 3}, RegShape{512, 16, 16, 16, &launch_p3m<2, 16>, 1 << 3},
+    RegShape{512, 16, 16, 4, &launch_p3m<2, 4>, 1 << 3},
     RegShape{512, 32, 16, 8, &launch_split3<32, 16, 8>, 1 << 3},
     RegShape{512, 32, 16, 4, &launch_split3<32, 16, 4>, 1 << 3},
     DDH_SHAPE1(768, 32, 24, 8), DDH_SHAPE1(768, 32, 24, 4),
+    RegShape{768, 16, 16, 8, &launch_p3m<3, 8>, 1 << 3}, RegShape{768, 16, 16, 4, &launch_p3m<3, 4>, 1 << 3},
     RegShape{768, 32, 24, 8, &launch_split3<32, 24, 8>, 1 << 3},
     RegShape{768, 32, 24, 4, &launch_split3<32, 24, 4>, 1 << 3},
 };
