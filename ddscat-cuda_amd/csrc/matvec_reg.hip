@@ -161,6 +161,77 @@ __global__ __launch_bounds__(R1 * C) void kr_passy(const float2* __restrict__ in
 }
 
 // ------------------------------------------------------------------ pass 3
+// One column of the Green convolution pass.  FFT3D: column (kx, ky) of a2,
+// transform along z; SLICES: column (kx, kz = k0 + plane) of a slice batch
+// [planes][3][NY][2NX], transform along y.  Element n of component c is at
+// col[n es + c cs]; the transform has nf nonzero inputs and kept outputs.
+// The Green entry of point k along the transform is at pb + step min(k, mf - k);
+// the off-diagonal signs are c12, c13, c23 times the parity of k
+// (k <= tf) for the components that involve the transform axis
+// (octant mirror rules for IPBC=0, cu:109-115 and cu:166-186; IPBC=1 has
+// no mirrors and all signs +1).
+struct P3Col {
+  float2* col;
+  size_t es, cs;
+  int nf;
+  size_t pb, step;
+  int mf, tf;
+  float c12, c13, c23;
+};
+template <bool SLY>
+__device__ __forceinline__ P3Col p3_col(float2* a2, int kx, int row, int nx, int ny, int nz, int gx, size_t plane,
+                                        const GreenGeo& gg) {
+  P3Col q;
+  const int ir = kx < gx ? min(kx, gg.mx - kx) : 0;
+  const float sx = kx <= gg.tx ? 1.f : -1.f;
+  if (!SLY) {  // row = ky
+    const int jr = min(row, gg.my - row);
+    const float sy = row <= gg.ty ? 1.f : -1.f;
+    q.col = a2 + (size_t)row * gx + kx;
+    q.es = plane;
+    q.cs = (size_t)nz * plane;
+    q.nf = nz;
+    q.pb = (size_t)ir + (size_t)gg.ox * jr;
+    q.step = (size_t)gg.ox * gg.oy;
+    q.mf = gg.mz; q.tf = gg.tz;
+    q.c12 = sx * sy; q.c13 = sx; q.c23 = sy;
+  } else {  // row = kz; a2 points at plane 0 of the batch, plane = index of the batch's first kz
+    const int b = row - (int)plane, kr = min(row, gg.mz - row);
+    const float sz = row <= gg.tz ? 1.f : -1.f;
+    q.col = a2 + (size_t)b * 3 * ny * gx + kx;
+    q.es = gx;
+    q.cs = (size_t)ny * gx;
+    q.nf = ny;
+    q.pb = (size_t)ir + (size_t)gg.ox * gg.oy * kr;
+    q.step = gg.ox;
+    q.mf = gg.my; q.tf = gg.ty;
+    q.c12 = sx; q.c13 = sx * sz; q.c23 = sz;
+  }
+  return q;
+}
+// Green multiply of point k along the transform: p0..p2 in, G p out
+template <bool SLY>
+__device__ __forceinline__ void p3_green(const P3Col& q, const float2* __restrict__ green, int k, float2& p0,
+                                         float2& p1, float2& p2) {
+  const int kr = min(k, q.mf - k);
+  const float sf = k <= q.tf ? 1.f : -1.f;
+  const Green6 G = green_at(green, q.pb + q.step * kr);
+  const float2 g12 = cscale(G.xy, SLY ? q.c12 * sf : q.c12);
+  const float2 g13 = cscale(G.xz, SLY ? q.c13 : q.c13 * sf);
+  const float2 g23 = cscale(G.yz, q.c23 * sf);
+  const float2 a = p0, b = p1, c = p2;
+  p0 = cadd(cadd(cmul(G.xx, a), cmul(g12, b)), cmul(g13, c));
+  p1 = cadd(cadd(cmul(g12, a), cmul(G.yy, b)), cmul(g23, c));
+  p2 = cadd(cadd(cmul(g13, a), cmul(g23, b)), cmul(G.zz, c));
+}
+// row of block row index j: FFT3D orders ky as 0, NY, 1, 2NY-1, 2, ... so
+// mirror rows (same Green octant entries) run back to back on one XCD
+template <bool SLY>
+__device__ __forceinline__ int p3_row(int j, int ny, int gy, int k0) {
+  if (SLY) return k0 + j;
+  return j == 0 ? 0 : j == 1 ? ny : ((j & 1) ? gy - ((j - 2) / 2 + 1) : (j - 2) / 2 + 1);
+}
+
 template <int R1, int R2, int C>
 __global__ __launch_bounds__(R1 * C) void kr_pass3(float2* __restrict__ a2, const float2* __restrict__ green,
                                                const float2* __restrict__ twg, int nx, int ny, int nz, int gx,
@@ -248,6 +319,80 @@ __global__ __launch_bounds__(R1 * C) void kr_pass3(float2* __restrict__ a2, cons
         col[o] = v0[n2];
         col[o + cs] = v1[n2];
         col[o + 2 * cs] = v2[n2];
+      }
+    }
+  }
+}
+
+// SLICES y pass with the register FFT of kr_pass3: y-FFT, Green multiply at
+// kz, inverse y-FFT over a batch of kz planes starting at k0 (a2 = slice
+// buffer [planes][3][NY][2NX], plane = k0, grid rows = planes of the batch).
+// A kernel of its own so that kr_pass3 keeps its register count.
+template <int R1, int R2, int C>
+__global__ __launch_bounds__(R1 * C) void kr_sly(float2* __restrict__ a2, const float2* __restrict__ green,
+                                               const float2* __restrict__ twg, int nx, int ny, int nz, int gx,
+                                               int gy, size_t plane, BatchArgs ba, GreenGeo gg) {
+  constexpr int L = R1 * R2, RT = R1 * C, P = C + 1, G = Reg2<R1, R2>::G, VN = Reg2<R1, R2>::VN;
+  __shared__ float2 twL[L];
+  __shared__ float2 buf[L * P];
+  const int rb = blockIdx.z;
+  if (ba.active && !ba.active[rb]) return;
+  a2 += rb * ba.a2s;
+  const int s = threadIdx.x % C, t = threadIdx.x / C;
+  // Grid order puts Green-sharing blocks on one XCD (workgroups go round-robin
+  // over the 8 XCDs by linear id): mirror rows one grid row apart (p3_row),
+  // and column tile b and its mirror NT-1-b h = ceil(NT/2) grid columns
+  // apart (NT = column tiles).
+  const int NT = gridDim.x, h = (NT + 1) / 2;
+  const int tile = (int)blockIdx.x < h ? (int)blockIdx.x : NT - 1 - ((int)blockIdx.x - h);
+  const int kx = tile * C + s;
+  for (int i = threadIdx.x; i < L; i += RT) twL[i] = twg[i];
+  const bool live = kx < gx;
+  const P3Col q = p3_col<true>(a2, kx, p3_row<true>((int)blockIdx.y, ny, gy, (int)plane), nx, ny, nz, gx, plane, gg);
+  float2 v0[VN], v1[VN], v2[VN];
+#pragma unroll
+  for (int n2 = 0; n2 < R2; ++n2) {
+    const int n = t + R1 * n2;
+    const bool ok = live && lower_half<R2>(n2, n, q.nf);
+    const size_t o = (size_t)n * q.es;
+    v0[n2] = ok ? q.col[o] : make_float2(0.f, 0.f);
+    v1[n2] = ok ? q.col[o + q.cs] : make_float2(0.f, 0.f);
+    v2[n2] = ok ? q.col[o + 2 * q.cs] : make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  fft2_fwd<R1, R2, +1>(v0, buf, twL, t, s, P);
+  __syncthreads();
+  fft2_fwd<R1, R2, +1>(v1, buf, twL, t, s, P);
+  __syncthreads();
+  fft2_fwd<R1, R2, +1>(v2, buf, twL, t, s, P);
+  // 3x3 symmetric Green multiply.  Each octant entry serves up to eight
+  // mirror points; the mirrors along the transform are in this block and the
+  // others run on the same XCD at about the same time (grid order above), so
+  // repeats hit in L2.
+#pragma unroll
+  for (int i = 0; i < G; ++i) {
+    if (!reg2_owns<R1, R2>(t, i)) continue;
+#pragma unroll
+    for (int k1 = 0; k1 < R1; ++k1) {
+      const int e = i * R1 + k1;
+      p3_green<true>(q, green, (t + R1 * i) + R2 * k1, v0[e], v1[e], v2[e]);
+    }
+  }
+  __syncthreads();
+  fft2_inv<R1, R2, -1>(v0, buf, twL, t, s, P);
+  __syncthreads();
+  fft2_inv<R1, R2, -1>(v1, buf, twL, t, s, P);
+  __syncthreads();
+  fft2_inv<R1, R2, -1>(v2, buf, twL, t, s, P);
+  if (live) {
+#pragma unroll
+    for (int n2 = 0; n2 < R2; ++n2) {
+      const int n = t + R1 * n2;
+      if (lower_half<R2>(n2, n, q.nf)) {
+        const size_t o = (size_t)n * q.es;
+        q.col[o] = v0[n2];
+        q.col[o + q.cs] = v1[n2];
+        q.col[o + 2 * q.cs] = v2[n2];
       }
     }
   }
@@ -364,7 +509,7 @@ __device__ __forceinline__ float2 dft_m_out(const float2* z, int m) {
 // component, so the three components and the Green multiply stay in
 // registers; two-level shapes of 512 or 768 points hold 32 values per
 // component and have to split the components over thread groups.
-template <int M, int C>
+template <int M, int C, bool SLY = false>
 __global__ __launch_bounds__(M * 16 * C) void kr_pass3m(float2* __restrict__ a2, const float2* __restrict__ green,
                                                     const float2* __restrict__ twg, int nx, int ny, int nz, int gx,
                                                     int gy, size_t plane, BatchArgs ba, GreenGeo gg) {
@@ -381,22 +526,19 @@ __global__ __launch_bounds__(M * 16 * C) void kr_pass3m(float2* __restrict__ a2,
   const int m = threadIdx.x / GT, lt = threadIdx.x % GT;
   const int s = lt % C, t = lt / C, sv = m * C + s;
   // XCD-aware grid order as in kr_pass3
-  const int j = blockIdx.y;
-  const int ky = j == 0 ? 0 : j == 1 ? ny : ((j & 1) ? gy - ((j - 2) / 2 + 1) : (j - 2) / 2 + 1);
   const int tile = xcd_tile<C>((int)blockIdx.x, (int)gridDim.x, true);
   const int kx = tile * C + s;
   for (int i = threadIdx.x; i < L; i += RT) twL[i] = twg[i];
   for (int i = threadIdx.x; i < S; i += RT) twS[i] = twg[M * i];
   const bool live = kx < gx;
-  const size_t cs = (size_t)nz * plane;
-  float2* col = a2 + (size_t)ky * gx + kx;
+  const P3Col q = p3_col<SLY>(a2, kx, p3_row<SLY>((int)blockIdx.y, ny, gy, (int)plane), nx, ny, nz, gx, plane, gg);
   float2 v[3][R];
 #pragma unroll
   for (int n2 = 0; n2 < R; ++n2) {
     const bool ok = live && n2 < R / 2;
-    const size_t o = (size_t)(M * (t + R * n2) + m) * plane;
+    const size_t o = (size_t)(M * (t + R * n2) + m) * q.es;
 #pragma unroll
-    for (int c = 0; c < 3; ++c) v[c][n2] = ok ? col[o + c * cs] : make_float2(0.f, 0.f);
+    for (int c = 0; c < 3; ++c) v[c][n2] = ok ? q.col[o + c * q.cs] : make_float2(0.f, 0.f);
   }
   __syncthreads();
 #pragma unroll
@@ -417,26 +559,9 @@ __global__ __launch_bounds__(M * 16 * C) void kr_pass3m(float2* __restrict__ a2,
       v[c][i] = dft_m_out<M, +1>(z, m);  // X[t + 16 i + 256 m]
     }
   }
-  // 3x3 symmetric Green multiply at kz = t + 16 i + 256 m (octant parity
-  // reconstruction for IPBC=0, cu:109-115; whole grid for IPBC=1)
-  const int ox = gg.ox, oy = gg.oy;
-  const int ir = live ? min(kx, gg.mx - kx) : 0;
-  const int jr = min(ky, gg.my - ky);
-  const float sxy = ((kx <= gg.tx) == (ky <= gg.ty)) ? 1.f : -1.f;
-  const float sx = kx <= gg.tx ? 1.f : -1.f, sy = ky <= gg.ty ? 1.f : -1.f;
-  const size_t pb = (size_t)ir + (size_t)ox * jr;
+  // 3x3 symmetric Green multiply at k = t + 16 i + 256 m along the transform
 #pragma unroll
-  for (int i = 0; i < R; ++i) {
-    const int kz = t + R * i + S * m;
-    const int kr = min(kz, gg.mz - kz);
-    const float sz = kz <= gg.tz ? 1.f : -1.f;
-    const Green6 G = green_at(green, pb + (size_t)ox * oy * kr);
-    const float2 g12 = cscale(G.xy, sxy), g13 = cscale(G.xz, sx * sz), g23 = cscale(G.yz, sy * sz);
-    const float2 p0 = v[0][i], p1 = v[1][i], p2 = v[2][i];
-    v[0][i] = cadd(cadd(cmul(G.xx, p0), cmul(g12, p1)), cmul(g13, p2));
-    v[1][i] = cadd(cadd(cmul(g12, p0), cmul(G.yy, p1)), cmul(g23, p2));
-    v[2][i] = cadd(cadd(cmul(g13, p0), cmul(g23, p1)), cmul(G.zz, p2));
-  }
+  for (int i = 0; i < R; ++i) p3_green<SLY>(q, green, t + R * i + S * m, v[0][i], v[1][i], v[2][i]);
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     __syncthreads();
@@ -458,9 +583,9 @@ __global__ __launch_bounds__(M * 16 * C) void kr_pass3m(float2* __restrict__ a2,
   if (live) {
 #pragma unroll
     for (int n2 = 0; n2 < R / 2; ++n2) {
-      const size_t o = (size_t)(M * (t + R * n2) + m) * plane;
+      const size_t o = (size_t)(M * (t + R * n2) + m) * q.es;
 #pragma unroll
-      for (int c = 0; c < 3; ++c) col[o + c * cs] = v[c][n2];
+      for (int c = 0; c < 3; ++c) q.col[o + c * q.cs] = v[c][n2];
     }
   }
 }
@@ -923,6 +1048,47 @@ static const RegShape kPairShapes[] = {
 };
 #undef DDH_PAIR
 #undef DDH_SHAPE
+
+// ------------------------------------------------- SLICES y pass (register)
+// y-FFT, Green multiply at kz and inverse y-FFT of nb slice planes starting
+// at kz = k0, with the register kernels of pass 3 (column setup p3_col<true>)
+template <int R1, int R2, int C>
+static void launch_sly(Context& c, float2* sl, int k0, int nb, hipStream_t st) {
+  const MatvecPlan& m = c.mv;
+  hipLaunchKernelGGL((kr_sly<R1, R2, C>), dim3((m.gx + C - 1) / C, nb, 1), dim3(R1 * C), 0, st, sl, c.green, m.twy,
+                     c.nx, c.ny, c.nz, m.gx, m.gy, (size_t)k0, BatchArgs{}, green_geo(c));
+}
+template <int M, int C>
+static void launch_sly_m(Context& c, float2* sl, int k0, int nb, hipStream_t st) {
+  const MatvecPlan& m = c.mv;
+  hipLaunchKernelGGL((kr_pass3m<M, C, true>), dim3((m.gx + C - 1) / C, nb, 1), dim3(M * 16 * C), 0, st, sl, c.green,
+                     m.twy, c.nx, c.ny, c.nz, m.gx, m.gy, (size_t)k0, BatchArgs{}, green_geo(c));
+}
+using SlyLaunch = void (*)(Context&, float2*, int, int, hipStream_t);
+struct SlyShape {
+  int L;
+  SlyLaunch fn;
+};
+// the default pass-3 shape of each register length
+static const SlyShape kSlyShapes[] = {
+    {32, &launch_sly<4, 8, 64>},     {48, &launch_sly<8, 6, 32>},     {60, &launch_sly<10, 6, 32>},
+    {64, &launch_sly<8, 8, 32>},     {72, &launch_sly<12, 6, 16>},    {80, &launch_sly<10, 8, 32>},
+    {96, &launch_sly<12, 8, 16>},    {100, &launch_sly<10, 10, 32>},  {108, &launch_sly<12, 9, 16>},
+    {120, &launch_sly<12, 10, 16>},  {128, &launch_sly<8, 16, 32>},   {144, &launch_sly<12, 12, 16>},
+    {150, &launch_sly<15, 10, 16>},  {160, &launch_sly<16, 10, 16>},  {180, &launch_sly<15, 12, 16>},
+    {192, &launch_sly<16, 12, 16>},  {240, &launch_sly<16, 15, 16>},  {256, &launch_sly<16, 16, 16>},
+    {512, &launch_sly_m<2, 16>},     {768, &launch_sly_m<3, 8>},
+};
+
+int slices_y_green_reg(Context& c, float2* sl, int k0, int nb, hipStream_t st) {
+  if (c.fft_generic) return 0;
+  for (const SlyShape& r : kSlyShapes)
+    if (r.L == c.mv.gy) {
+      r.fn(c, sl, k0, nb, st);
+      return 1;
+    }
+  return 0;
+}
 
 // shape `cand` (0 = default) of length L serving pass `which`, or nullptr
 static const RegShape* find_shape(int L, int which, int cand) {

@@ -375,13 +375,13 @@ int slices_matvec(Context& c, int cwhat, const float2* x, float2* y, const float
                          nb, s.rx_f);
       DDH_LAUNCH_OK("SLICES x FFT", 23);
     }
-    {
+    if (!slices_y_green_reg(c, sl, k0, nb, ss)) {
       dim3 grid((m.gx + s.cy - 1) / s.cy, nb);
       const size_t lds = (size_t)(m.gy + 3 * s.cy * m.gy) * sizeof(float2);
       hipLaunchKernelGGL(k_sl_y_green, grid, dim3(SL_THREADS), lds, ss, sl, c.green, m.twy, m.py, c.nx, c.ny, c.nz,
                          k0, lg2(s.cy), green_geo(c));
-      DDH_LAUNCH_OK("SLICES y FFT + Green", 24);
     }
+    DDH_LAUNCH_OK("SLICES y FFT + Green", 24);
     {
       dim3 grid((nrows + s.rx_i - 1) / s.rx_i);
       const size_t lds = (size_t)(m.gx + s.rx_i * m.gx) * sizeof(float2);

@@ -101,6 +101,9 @@ int reg_candidates(int L, int which);
 // register-FFT pass `which` (1..5) with shape `cand` of its axis length; 1 = launched
 int reg_pass(Context& c, int which, int cand, int cj, int compact, const float2* x, float2* y, const float2* adia,
              const float2* aoff, hipStream_t st, const MvBatch& bt);
+// SLICES y-FFT + Green + inverse y-FFT of nb slice planes from kz = k0 with
+// the register kernels when 2NY has a register shape; 1 = launched
+int slices_y_green_reg(Context& c, float2* sl, int k0, int nb, hipStream_t st);
 // times every FFT kernel candidate of each pass and stores the fastest in c.mv.kind
 int matvec_autotune(Context& c);
 

@@ -95,6 +95,29 @@ def test_pbc_matvec_matches_oracle(lib, backend, pyd, pzd):
         lib.release()
 
 
+def test_pbc_slices_register_y_pass(lib):
+    """SLICES with a register-FFT y pass (2 NY = 32) on the whole-grid
+    Green of a target periodic in y and z gives the oracle MATVEC."""
+    from ddscat_hip import BACKEND_FFT3D, BACKEND_SLICES
+    nx, ny, nz = 4, 16, 3
+    rng = np.random.default_rng(1643)
+    iocc, x, adia, aoff = random_case(rng, nx, ny, nz)
+    lib.set_backend(BACKEND_SLICES)
+    try:
+        prepare_pbc(lib, nx, ny, nz, iocc, 16.0, 3.0)
+        g = lib.get_green().astype(np.complex128).reshape(6, 2 * nz, 2 * ny, 2 * nx)
+        gref = O.green_full_pbc(nx, ny, nz, AK, GAMMA, 16.0, 3.0)
+        assert np.max(np.abs(g - gref)) / np.max(np.abs(gref)) < 1e-5
+        for cw in ("N", "C"):
+            y = lib.apply(cw, x, adia, aoff)
+            err = rel_l2(y, O.matvec_fft(cw, x, adia, aoff, gref, nx, ny, nz, iocc, ipbc=1))
+            print(f"SLICES register y pass, PBC MATVEC {cw}: {err:.2e}")
+            assert err < MATVEC_TOL
+    finally:
+        lib.set_backend(BACKEND_FFT3D)
+        lib.release()
+
+
 def test_pbc_every_kernel_candidate(lib):
     """At 16^3 (padded axes 32) the register kernels run; every pass-3
     candidate and the LDS-staged kernel give the oracle MATVEC on the

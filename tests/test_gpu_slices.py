@@ -33,7 +33,11 @@ def slib(lib):
     lib.set_backend(BACKEND_FFT3D)
 
 
-@pytest.mark.parametrize("dims", [(2, 2, 2), (3, 5, 4), (8, 8, 8), (12, 15, 16), (24, 20, 18), (10, 6, 36)])
+# the last seven have a register-FFT y pass (2 NY = 32, 48, 128, 256, 256,
+# 512 = 2 x 256, 768 = 3 x 256); the others run the LDS-staged y pass
+@pytest.mark.parametrize("dims", [(2, 2, 2), (3, 5, 4), (8, 8, 8), (12, 15, 16), (24, 20, 18), (10, 6, 36),
+                                  (5, 16, 6), (3, 24, 5), (4, 64, 3), (3, 128, 4), (7, 128, 3), (4, 256, 3),
+                                  (3, 384, 4)])
 @pytest.mark.parametrize("cwhat", ["N", "C"])
 def test_slices_matvec_matches_oracle(slib, dims, cwhat):
     nx, ny, nz = dims
