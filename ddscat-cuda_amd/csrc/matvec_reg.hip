@@ -1049,6 +1049,98 @@ static const RegShape kPairShapes[] = {
 #undef DDH_PAIR
 #undef DDH_SHAPE
 
+// ------------------------------------------------ SLICES x passes (register)
+// Rows of a batch of kz planes, row r = (b 3 + c) NY + iy for plane
+// kz = k0 + b and component c.  DIR = +1: x-FFT of zwork row
+// [c][kz][iy] (NX values, zero-padded to 2NX) into slice row r (2NX
+// spectral values).  DIR = -1: inverse x-FFT of slice row r back into the
+// zwork row (its NX unpadded outputs).  A row past the end is clamped to
+// the last row for loading and never stored.
+template <int R1, int R2, int C, int DIR>
+__global__ __launch_bounds__(R1 * C) void kr_slx(float2* __restrict__ zw, float2* __restrict__ sl,
+                                             const float2* __restrict__ twg, int nx, int ny, int gz, int k0,
+                                             int nrows) {
+  constexpr int L = R1 * R2, RT = R1 * C, P = row_stride<R1, R2>(), G = Reg2<R1, R2>::G, VN = Reg2<R1, R2>::VN;
+  __shared__ float2 twL[L];
+  __shared__ float buf[C * P];  // split exchange, row layout
+  const int t = threadIdx.x % R1, s = threadIdx.x / R1;
+  const int r = blockIdx.x * C + s;
+  for (int i = threadIdx.x; i < L; i += RT) twL[i] = twg[i];
+  const bool live = r < nrows;
+  const int rr = live ? r : nrows - 1;
+  const int iy = rr % ny, bc = rr / ny, c = bc % 3, b = bc / 3;
+  float2* zrow = zw + (((size_t)c * gz + k0 + b) * ny + iy) * nx;
+  float2* srow = sl + (size_t)rr * L;
+  float2 v[VN];
+  if (DIR > 0) {
+#pragma unroll
+    for (int n2 = 0; n2 < R2; ++n2) {
+      const int n = t + R1 * n2;
+      v[n2] = lower_half<R2>(n2, n, nx) ? zrow[n] : make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+    fft2_fwd_s<R1, R2, +1, 1>(v, buf, twL, t, s, P);
+    if (live) {
+#pragma unroll
+      for (int i = 0; i < G; ++i)
+#pragma unroll
+        for (int k1 = 0; k1 < R1; ++k1)
+          if (reg2_owns<R1, R2>(t, i)) srow[(t + R1 * i) + R2 * k1] = v[i * R1 + k1];
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < G; ++i)
+#pragma unroll
+      for (int k1 = 0; k1 < R1; ++k1)
+        v[i * R1 + k1] = reg2_owns<R1, R2>(t, i) ? srow[(t + R1 * i) + R2 * k1] : make_float2(0.f, 0.f);
+    __syncthreads();
+    fft2_inv_s<R1, R2, -1, 1>(v, buf, twL, t, s, P);
+    if (live) {
+#pragma unroll
+      for (int n2 = 0; n2 < R2; ++n2) {
+        const int n = t + R1 * n2;
+        if (lower_half<R2>(n2, n, nx)) zrow[n] = v[n2];
+      }
+    }
+  }
+}
+
+template <int R1, int R2, int C>
+static void launch_slx(Context& c, int dir, float2* zw, float2* sl, int k0, int nb, hipStream_t st) {
+  const MatvecPlan& m = c.mv;
+  const int nrows = nb * 3 * c.ny;
+  const dim3 grid((nrows + C - 1) / C);
+  if (dir > 0)
+    hipLaunchKernelGGL((kr_slx<R1, R2, C, +1>), grid, dim3(R1 * C), 0, st, zw, sl, m.twx, c.nx, c.ny, m.gz, k0, nrows);
+  else
+    hipLaunchKernelGGL((kr_slx<R1, R2, C, -1>), grid, dim3(R1 * C), 0, st, zw, sl, m.twx, c.nx, c.ny, m.gz, k0, nrows);
+}
+using SlxLaunch = void (*)(Context&, int, float2*, float2*, int, int, hipStream_t);
+struct SlxShape {
+  int L;
+  SlxLaunch fn;
+};
+// the default row shape of each register length (passes 1 and 5 of FFT3D)
+static const SlxShape kSlxShapes[] = {
+    {32, &launch_slx<4, 8, 64>},     {48, &launch_slx<8, 6, 32>},     {60, &launch_slx<10, 6, 32>},
+    {64, &launch_slx<8, 8, 32>},     {72, &launch_slx<12, 6, 16>},    {80, &launch_slx<10, 8, 32>},
+    {96, &launch_slx<12, 8, 16>},    {100, &launch_slx<10, 10, 32>},  {108, &launch_slx<12, 9, 16>},
+    {120, &launch_slx<12, 10, 16>},  {128, &launch_slx<8, 16, 32>},   {144, &launch_slx<12, 12, 16>},
+    {150, &launch_slx<15, 10, 16>},  {160, &launch_slx<16, 10, 16>},  {180, &launch_slx<15, 12, 16>},
+    {192, &launch_slx<16, 12, 16>},  {240, &launch_slx<16, 15, 16>},  {256, &launch_slx<16, 16, 16>},
+    {512, &launch_slx<32, 16, 8>},   {768, &launch_slx<32, 24, 8>},
+};
+
+int slices_x_reg(Context& c, int dir, float2* zw, float2* sl, int k0, int nb, hipStream_t st) {
+  if (c.fft_generic) return 0;
+  for (const SlxShape& r : kSlxShapes)
+    if (r.L == c.mv.gx) {
+      r.fn(c, dir, zw, sl, k0, nb, st);
+      return 1;
+    }
+  return 0;
+}
+
 // ------------------------------------------------- SLICES y pass (register)
 // y-FFT, Green multiply at kz and inverse y-FFT of nb slice planes starting
 // at kz = k0, with the register kernels of pass 3 (column setup p3_col<true>)

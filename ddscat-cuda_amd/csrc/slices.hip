@@ -368,13 +368,13 @@ int slices_matvec(Context& c, int cwhat, const float2* x, float2* y, const float
     hipStream_t ss = s.side[k & 1];
     float2* sl = s.slice[k & 1];
     const int nrows = nb * 3 * c.ny;
-    {
+    if (!slices_x_reg(c, +1, s.zwork, sl, k0, nb, ss)) {
       dim3 grid((nrows + s.rx_f - 1) / s.rx_f);
       const size_t lds = (size_t)(m.gx + s.rx_f * m.gx) * sizeof(float2);
       hipLaunchKernelGGL(k_sl_xfwd, grid, dim3(SL_THREADS), lds, ss, s.zwork, sl, m.twx, m.px, c.nx, c.ny, m.gz, k0,
                          nb, s.rx_f);
-      DDH_LAUNCH_OK("SLICES x FFT", 23);
     }
+    DDH_LAUNCH_OK("SLICES x FFT", 23);
     if (!slices_y_green_reg(c, sl, k0, nb, ss)) {
       dim3 grid((m.gx + s.cy - 1) / s.cy, nb);
       const size_t lds = (size_t)(m.gy + 3 * s.cy * m.gy) * sizeof(float2);
@@ -382,13 +382,13 @@ int slices_matvec(Context& c, int cwhat, const float2* x, float2* y, const float
                          k0, lg2(s.cy), green_geo(c));
     }
     DDH_LAUNCH_OK("SLICES y FFT + Green", 24);
-    {
+    if (!slices_x_reg(c, -1, s.zwork, sl, k0, nb, ss)) {
       dim3 grid((nrows + s.rx_i - 1) / s.rx_i);
       const size_t lds = (size_t)(m.gx + s.rx_i * m.gx) * sizeof(float2);
       hipLaunchKernelGGL(k_sl_xinv, grid, dim3(SL_THREADS), lds, ss, sl, s.zwork, m.twx, m.px, c.nx, c.ny, m.gz, k0,
                          nb, s.rx_i);
-      DDH_LAUNCH_OK("SLICES x inverse FFT", 26);
     }
+    DDH_LAUNCH_OK("SLICES x inverse FFT", 26);
   }
   for (int i = 0; i < 2; ++i) {
     DDH_TRY(hipEventRecord(s.ev_done[i], s.side[i]), "SLICES record", 25);

@@ -104,6 +104,10 @@ int reg_pass(Context& c, int which, int cand, int cj, int compact, const float2*
 // SLICES y-FFT + Green + inverse y-FFT of nb slice planes from kz = k0 with
 // the register kernels when 2NY has a register shape; 1 = launched
 int slices_y_green_reg(Context& c, float2* sl, int k0, int nb, hipStream_t st);
+// SLICES x-FFT (dir +1: zwork rows -> slice) or inverse x-FFT (dir -1:
+// slice -> zwork rows) of nb planes from kz = k0 with the register kernels
+// when 2NX has a register shape; 1 = launched
+int slices_x_reg(Context& c, int dir, float2* zw, float2* sl, int k0, int nb, hipStream_t st);
 // times every FFT kernel candidate of each pass and stores the fastest in c.mv.kind
 int matvec_autotune(Context& c);
 
