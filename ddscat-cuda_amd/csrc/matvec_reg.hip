@@ -1141,6 +1141,56 @@ int slices_x_reg(Context& c, int dir, float2* zw, float2* sl, int k0, int nb, hi
   return 0;
 }
 
+// ------------------------------------------------ SLICES z passes (register)
+// Column FFTs along z over the NX NY columns of a plane (kr_passy with one
+// "row", columns at element stride NX NY).  dir = +1: in [3][NZ][NY][NX]
+// (the full-layout vector) -> out = zwork [3][2NZ][NY][NX]; dir = -1: zwork ->
+// out [3][NZ][NY][NX] (the NZ unpadded outputs).
+template <int R1, int R2, int C>
+static void launch_slz(Context& c, int dir, const float2* in, float2* out, hipStream_t st) {
+  const MatvecPlan& m = c.mv;
+  const int plane = c.nx * c.ny;
+  const size_t full = (size_t)m.gz * plane, half = (size_t)c.nz * plane;
+  const dim3 grid((plane + C - 1) / C, 1, 3);
+  if (dir > 0)
+    hipLaunchKernelGGL((kr_passy<R1, R2, C, +1>), grid, dim3(R1 * C), 0, st, in, out, m.twz, plane, c.nz, m.gz, 1, half,
+                       full, BatchArgs{});
+  else
+    hipLaunchKernelGGL((kr_passy<R1, R2, C, -1>), grid, dim3(R1 * C), 0, st, in, out, m.twz, plane, m.gz, c.nz, 1, full,
+                       half, BatchArgs{});
+}
+using SlzLaunch = void (*)(Context&, int, const float2*, float2*, hipStream_t);
+struct SlzShape {
+  int L;
+  SlzLaunch fn;
+};
+// the default column shape of each register length (passes 2 and 4 of FFT3D)
+static const SlzShape kSlzShapes[] = {
+    {32, &launch_slz<4, 8, 64>},     {48, &launch_slz<8, 6, 32>},     {60, &launch_slz<10, 6, 32>},
+    {64, &launch_slz<8, 8, 32>},     {72, &launch_slz<12, 6, 16>},    {80, &launch_slz<10, 8, 32>},
+    {96, &launch_slz<12, 8, 16>},    {100, &launch_slz<10, 10, 32>},  {108, &launch_slz<12, 9, 16>},
+    {120, &launch_slz<12, 10, 16>},  {128, &launch_slz<8, 16, 32>},   {144, &launch_slz<12, 12, 16>},
+    {150, &launch_slz<15, 10, 16>},  {160, &launch_slz<16, 10, 16>},  {180, &launch_slz<15, 12, 16>},
+    {192, &launch_slz<16, 12, 16>},  {240, &launch_slz<16, 15, 16>},  {256, &launch_slz<16, 16, 16>},
+    {512, &launch_slz<32, 16, 8>},   {768, &launch_slz<32, 24, 8>},
+};
+
+int slices_z_reg_length(int L) {
+  for (const SlzShape& r : kSlzShapes)
+    if (r.L == L) return 1;
+  return 0;
+}
+
+int slices_z_reg(Context& c, int dir, const float2* in, float2* out, hipStream_t st) {
+  if (c.fft_generic) return 0;
+  for (const SlzShape& r : kSlzShapes)
+    if (r.L == c.mv.gz) {
+      r.fn(c, dir, in, out, st);
+      return 1;
+    }
+  return 0;
+}
+
 // ------------------------------------------------- SLICES y pass (register)
 // y-FFT, Green multiply at kz and inverse y-FFT of nb slice planes starting
 // at kz = k0, with the register kernels of pass 3 (column setup p3_col<true>)

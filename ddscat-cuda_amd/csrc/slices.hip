@@ -23,6 +23,7 @@ static constexpr int SL_PERW = 16;  // single-component kernels
 
 struct SliceState {
   float2* zwork = nullptr;
+  float2* zin = nullptr;  // [3][NZ][NY][NX] (full layout): z-pass input / inverse output (register z path)
   float2* slice[2] = {nullptr, nullptr};
   hipStream_t side[2] = {nullptr, nullptr};
   hipEvent_t ev_in = nullptr, ev_done[2] = {nullptr, nullptr};
@@ -265,6 +266,63 @@ __global__ __launch_bounds__(SL_THREADS) void k_sl_zinv_finish(
   }
 }
 
+// ------------------------------------ register z path: gather and finish
+// x (compact via the site map, or full) into zin [3][NAT], conj for 'C'
+__global__ __launch_bounds__(SL_THREADS) void k_sl_gather(const float2* __restrict__ x, const int* __restrict__ qmap,
+                                                         float2* __restrict__ zin, int nat, int nat0, int compact,
+                                                         int cj) {
+  const int c = blockIdx.y;
+  const size_t stride = (size_t)gridDim.x * SL_THREADS;
+  for (size_t j = (size_t)blockIdx.x * SL_THREADS + threadIdx.x; j < (size_t)nat; j += stride) {
+    float2 v;
+    if (compact) {
+      const int q = qmap[j];
+      v = q >= 0 ? x[(size_t)c * nat0 + q] : make_float2(0.f, 0.f);
+    } else {
+      v = x[(size_t)c * nat + j];
+    }
+    if (cj) v.y = -v.y;
+    zin[(size_t)c * nat + j] = v;
+  }
+}
+
+// finish on the inverse-transformed w [3][NAT] (cu:116-118): y = adia x - w/G
+// + offdiag(aoff) x at every occupied site, IOCC mask, conj for 'C'
+__global__ __launch_bounds__(SL_THREADS) void k_sl_finish(const float2* __restrict__ w, const float2* __restrict__ x,
+                                                         float2* __restrict__ y, const float2* __restrict__ adia,
+                                                         const float2* __restrict__ aoff,
+                                                         const int16_t* __restrict__ iocc,
+                                                         const int* __restrict__ qmap, int nat, int nat0, int compact,
+                                                         int cj, float inv) {
+  const size_t gs = (size_t)gridDim.x * SL_THREADS;
+  for (size_t j = (size_t)blockIdx.x * SL_THREADS + threadIdx.x; j < (size_t)nat; j += gs) {
+    size_t o = j, sd = (size_t)nat;
+    bool st = true, zr = false;
+    if (compact) {
+      const int q = qmap[j];
+      st = q >= 0;
+      o = st ? (size_t)q : 0;
+      sd = (size_t)nat0;
+    } else {
+      zr = iocc && iocc[j] == 0;
+    }
+    const float2 w0 = w[j], w1 = w[j + nat], w2 = w[j + 2 * (size_t)nat];
+    float2 x1 = x[o], x2 = x[o + sd], x3 = x[o + 2 * sd];
+    const float2 d1 = adia[o], d2 = adia[o + sd], d3 = adia[o + 2 * sd];
+    const float2 a23 = aoff[o], a31 = aoff[o + sd], a12 = aoff[o + 2 * sd];
+    if (cj) { x1.y = -x1.y; x2.y = -x2.y; x3.y = -x3.y; }
+    float2 r1 = csub(cmul(d1, x1), cscale(w0, inv));
+    float2 r2 = csub(cmul(d2, x2), cscale(w1, inv));
+    float2 r3 = csub(cmul(d3, x3), cscale(w2, inv));
+    r1 = cadd(r1, cadd(cmul(a31, x3), cmul(a12, x2)));
+    r2 = cadd(r2, cadd(cmul(a12, x1), cmul(a23, x3)));
+    r3 = cadd(r3, cadd(cmul(a23, x2), cmul(a31, x1)));
+    if (cj) { r1.y = -r1.y; r2.y = -r2.y; r3.y = -r3.y; }
+    if (zr) r1 = r2 = r3 = make_float2(0.f, 0.f);
+    if (st) { y[o] = r1; y[o + sd] = r2; y[o + 2 * sd] = r3; }
+  }
+}
+
 // ------------------------------------------------------------------ host --
 static int p2floor(int v) {
   int p = 1;
@@ -313,6 +371,8 @@ int slices_setup(Context& c) {
   const size_t zcount = (size_t)3 * m.gz * c.nx * c.ny;
   const size_t scount = (size_t)s.batch * 3 * c.ny * m.gx;
   DDH_TRY(hipMalloc((void**)&s.zwork, zcount * sizeof(float2)), "malloc SLICES z workspace", 204);
+  if (slices_z_reg_length(m.gz))
+    DDH_TRY(hipMalloc((void**)&s.zin, (size_t)3 * c.nat * sizeof(float2)), "malloc SLICES z input", 204);
   for (int i = 0; i < 2; ++i) {
     DDH_TRY(hipMalloc((void**)&s.slice[i], scount * sizeof(float2)), "malloc SLICES slice batch", 204);
     DDH_TRY(hipStreamCreateWithFlags(&s.side[i], hipStreamNonBlocking), "create SLICES stream", 206);
@@ -331,6 +391,7 @@ void slices_release(Context& c) {
     }
   }
   if (s.zwork) hipFree(s.zwork);
+  if (s.zin) hipFree(s.zin);
   for (int i = 0; i < 2; ++i) {
     if (s.slice[i]) hipFree(s.slice[i]);
     if (s.side[i]) hipStreamDestroy(s.side[i]);
@@ -353,13 +414,24 @@ int slices_matvec(Context& c, int cwhat, const float2* x, float2* y, const float
   if (!s.zwork) { set_error("SLICES MATVEC", "backend not prepared"); return 20; }
   const int cj = (cwhat == 'C') ? 1 : 0;
   const int compact = layout ? 1 : 0;
-  {
+  // register z path: a full-layout 'N' input is already [3][NZ][NY][NX]
+  const bool regz = s.zin && !c.fft_generic;
+  const int ewg = (int)std::min<size_t>(8192, ((size_t)c.nat + SL_THREADS * 4 - 1) / (SL_THREADS * 4));
+  if (regz) {
+    const float2* zsrc = x;
+    if (compact || cj) {
+      hipLaunchKernelGGL(k_sl_gather, dim3(ewg, 3), dim3(SL_THREADS), 0, st, x, c.qmap, s.zin, c.nat, c.nat0,
+                         compact, cj);
+      zsrc = s.zin;
+    }
+    slices_z_reg(c, +1, zsrc, s.zwork, st);
+  } else {
     dim3 grid((c.nx + s.cz - 1) / s.cz, c.ny, 3);
     const size_t lds = (size_t)(m.gz + s.cz * m.gz) * sizeof(float2);
     hipLaunchKernelGGL(k_sl_zfwd, grid, dim3(SL_THREADS), lds, st, x, c.qmap, s.zwork, m.twz, m.pz, c.nx, c.ny,
                        c.nz, c.nat, c.nat0, compact, cj, lg2(s.cz));
-    DDH_LAUNCH_OK("SLICES z FFT", 21);
   }
+  DDH_LAUNCH_OK("SLICES z FFT", 21);
   DDH_TRY(hipEventRecord(s.ev_in, st), "SLICES record", 25);
   for (int i = 0; i < 2; ++i) DDH_TRY(hipStreamWaitEvent(s.side[i], s.ev_in, 0), "SLICES wait", 25);
   int k = 0;
@@ -394,14 +466,18 @@ int slices_matvec(Context& c, int cwhat, const float2* x, float2* y, const float
     DDH_TRY(hipEventRecord(s.ev_done[i], s.side[i]), "SLICES record", 25);
     DDH_TRY(hipStreamWaitEvent(st, s.ev_done[i], 0), "SLICES wait", 25);
   }
-  {
+  if (regz) {
+    slices_z_reg(c, -1, s.zwork, s.zin, st);
+    hipLaunchKernelGGL(k_sl_finish, dim3(ewg), dim3(SL_THREADS), 0, st, s.zin, x, y, adia, aoff,
+                       compact ? nullptr : c.iocc, c.qmap, c.nat, c.nat0, compact, cj, 1.f / (float)m.ngrid);
+  } else {
     dim3 grid((c.nx + s.cz - 1) / s.cz, c.ny);
     const size_t lds = (size_t)(m.gz + 3 * s.cz * m.gz) * sizeof(float2);
     hipLaunchKernelGGL(k_sl_zinv_finish, grid, dim3(SL_THREADS), lds, st, s.zwork, m.twz, m.pz, x, y, adia, aoff,
                        compact ? nullptr : c.iocc, c.qmap, c.nx, c.ny, c.nz, c.nat, c.nat0, compact, cj,
                        1.f / (float)m.ngrid, lg2(s.cz));
-    DDH_LAUNCH_OK("SLICES z inverse FFT + finish", 29);
   }
+  DDH_LAUNCH_OK("SLICES z inverse FFT + finish", 29);
   c.matvec_count++;
   return 0;
 }
