@@ -315,8 +315,10 @@ def large_target(lib, edge=384, kd=0.25, m=1.7 + 0.05j, steps=10, warmup=2, seed
     """The large single target: sphere of diameter ``edge`` dipoles (384:
     29.6 M dipoles, padded grid 768^3 x 3 complex64), m = 1.7+0.05i, kd = 0.25,
     isotropic LDR operators.  MATVEC/s on compact device vectors for the
-    FFT3D and SLICES backends (the padded axis 768 runs the LDS-staged FFT
-    kernels), with the algorithmic bytes 696 NAT + 48 (N+1)^3 per MATVEC."""
+    FFT3D and SLICES backends (768-point axes: 32 x 24 register shapes for the
+    single-component passes, 3 x 256 decimation for the Green pass, the
+    LDS-staged kernel for FFT3D's pass 5), with the algorithmic bytes
+    696 NAT + 48 (N+1)^3 per MATVEC."""
     import numpy as np
     import torch
     from ddscat_hip import targets as T, BACKEND_SLICES, BACKEND_FFT3D
@@ -390,7 +392,10 @@ def adjoint_design(lib, edge=64, xsize=6.0, tol=1e-5, h=1e-3, fd_tol=2e-7, seed=
     v = ndi.gaussian_filter(rng.normal(size=(edge, edge, edge)), sigma=3.0).reshape(-1)[tg.occ]
     v = (v + 1j * np.roll(v, 7)) / np.abs(v).max()
     lib.prepare_green(tg.nx, tg.ny, tg.nz, tg.iocc, khat * kd)
-    J = design_q_abs(lib, tg, torch.tensor(eps0), kd, khat, pols, tol=tol)   # warm-up (graph capture)
+    # warm-up forward and backward (graph capture, workspaces): an inverse-design
+    # loop takes many gradient steps, the timed one is a steady-state step
+    ew = torch.tensor(eps0, dtype=torch.complex128, requires_grad=True)
+    design_q_abs(lib, tg, ew, kd, khat, pols, tol=tol).backward()
     eps = torch.tensor(eps0, dtype=torch.complex128, requires_grad=True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -550,6 +555,8 @@ def main():
     ap.add_argument("--backend", choices=["fft3d", "slices"], default="fft3d")
     ap.add_argument("--pmc-calib", action="store_true", help="run one torch copy of known size (rocprof scale)")
     ap.add_argument("--no-large", action="store_true", help="skip the 384^3 sphere MATVEC leg")
+    ap.add_argument("--large-only", action="store_true",
+                    help="run only the 384^3 sphere MATVEC leg (FFT3D and SLICES) and print its JSON")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build measures one GPU only (single-GPU box); --gpus must be 1", file=sys.stderr)
@@ -560,6 +567,9 @@ def main():
     from ddscat_hip import Library, BACKEND_SLICES, BACKEND_FFT3D
     assert torch.cuda.is_available(), "bench.py needs an MI355X"
     lib = Library()
+    if args.large_only:
+        print(json.dumps(dict(large_target=large_target(lib, steps=args.steps, warmup=args.warmup))), flush=True)
+        return
     lib.set_backend(BACKEND_SLICES if args.backend == "slices" else BACKEND_FFT3D)
     tg, kd, bufs, host = setup_workload(lib, edge=args.edge)
     info = lib.info()
