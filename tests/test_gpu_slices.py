@@ -33,11 +33,14 @@ def slib(lib):
     lib.set_backend(BACKEND_FFT3D)
 
 
-# the last seven have a register-FFT y pass (2 NY = 32, 48, 128, 256, 256,
-# 512 = 2 x 256, 768 = 3 x 256); the others run the LDS-staged y pass
+# register-FFT y passes (2 NY = 32, 48, 128, 256, 256, 512 = 2 x 256,
+# 768 = 3 x 256), register z passes with the gather and the finish (2 NZ =
+# 32, 128, 256, 512, 768), and all three axes on register kernels
+# (16, 24, 32); the others, and the remaining axes, run the LDS-staged kernels
 @pytest.mark.parametrize("dims", [(2, 2, 2), (3, 5, 4), (8, 8, 8), (12, 15, 16), (24, 20, 18), (10, 6, 36),
                                   (5, 16, 6), (3, 24, 5), (4, 64, 3), (3, 128, 4), (7, 128, 3), (4, 256, 3),
-                                  (3, 384, 4)])
+                                  (3, 384, 4), (6, 5, 16), (5, 7, 64), (3, 4, 128), (3, 3, 256), (2, 3, 384),
+                                  (16, 24, 32)])
 @pytest.mark.parametrize("cwhat", ["N", "C"])
 def test_slices_matvec_matches_oracle(slib, dims, cwhat):
     nx, ny, nz = dims
@@ -56,8 +59,11 @@ def test_slices_matvec_matches_oracle(slib, dims, cwhat):
 
 
 def test_slices_compact_layout_and_solver(slib):
+    """D=16 sphere: every padded axis is 32 points, so the x, y and z passes
+    run the register kernels and the compact layout goes through the site-map
+    gather and finish of the z passes."""
     import torch
-    tg = T.sphere(20)
+    tg = T.sphere(16)
     kd = T.kd_for_size_parameter(tg, 1.5)
     khat = np.array([0.0, 0.0, 1.0])
     e0 = np.array([1, 0, 0], complex)
@@ -66,15 +72,16 @@ def test_slices_compact_layout_and_solver(slib):
     slib.prepare_green(tg.nx, tg.ny, tg.nz, tg.iocc, khat * kd)
     rng = np.random.default_rng(1)
     x = ((rng.normal(size=3 * tg.nat) + 1j * rng.normal(size=3 * tg.nat)) * np.tile(tg.iocc != 0, 3)).astype(np.complex64)
-    yfull = slib.apply("N", x, adia, aoff)
     comp = lambda v: np.ascontiguousarray(v.reshape(3, -1)[:, tg.occ].reshape(-1))
     dev = lambda a: torch.from_numpy(np.ascontiguousarray(a).view(np.float32)).cuda()
     xc, ac, oc = dev(comp(x)), dev(comp(adia)), dev(comp(aoff))
     yc = torch.empty_like(xc)
-    torch.cuda.synchronize()
-    slib.apply_device("N", xc.data_ptr(), yc.data_ptr(), ac.data_ptr(), oc.data_ptr(), layout=1)
-    slib.sync()
-    assert rel_l2(yc.cpu().numpy().view(np.complex64), comp(yfull).astype(np.complex128)) < 1e-6
+    for cw in ("N", "C"):
+        yfull = slib.apply(cw, x, adia, aoff)
+        torch.cuda.synchronize()
+        slib.apply_device(cw, xc.data_ptr(), yc.data_ptr(), ac.data_ptr(), oc.data_ptr(), layout=1)
+        slib.sync()
+        assert rel_l2(yc.cpu().numpy().view(np.complex64), comp(yfull).astype(np.complex128)) < 1e-6, cw
     xs, it, tole, nc, rc = slib.solve("qmrccg", b, adia, aoff, maxit=300, tol=1e-5)
     slib.release()
     slib.set_backend(BACKEND_FFT3D)
