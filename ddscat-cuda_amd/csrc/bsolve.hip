@@ -512,7 +512,7 @@ __global__ void kb_pack(const double* __restrict__ scal, double* __restrict__ ou
 
 }  // namespace
 
-// ------------------------------------------------ single-RHS QMRCCG steps --
+// ------------------------------------- single-RHS QMRCCG and PBCGS2 steps --
 // One RHS: grid.y = 1, no activity mask, scalars at c.scal.  The |r|^2
 // partials of dsxr go to c.red with the stride finalize_partials reads.
 static_assert(BMAX_BLOCKS == RED_MAX_BLOCKS, "single-RHS partials share the reduction buffer layout");
@@ -549,6 +549,39 @@ int qmr1_dsxr(Context& c, float2* d, float2* sv, float2* x, float2* r, const flo
                        RED_MAX_BLOCKS);
   DDH_LAUNCH_OK("QMRCCG d, s, x, r update", 1);
   return finalize_partials(c, (int)g.x, 2, dst, c.stream);
+}
+
+int p21_xr(Context& c, int K, float2* x, float2* r0, float2* r1, const float2* u0, const float2* u1, const float2* u2,
+           size_t n) {
+  const dim3 g = single_grid(n);
+  const bool f64 = c.precision != PREC_FP32;
+  if (K == 1 && f64) hipLaunchKernelGGL((kb_p2_xr<double, 1>), g, dim3(BT), 0, c.stream, x, r0, r1, u0, u1, u2, c.scal, n, nullptr);
+  else if (K == 1) hipLaunchKernelGGL((kb_p2_xr<float, 1>), g, dim3(BT), 0, c.stream, x, r0, r1, u0, u1, u2, c.scal, n, nullptr);
+  else if (f64) hipLaunchKernelGGL((kb_p2_xr<double, 2>), g, dim3(BT), 0, c.stream, x, r0, r1, u0, u1, u2, c.scal, n, nullptr);
+  else hipLaunchKernelGGL((kb_p2_xr<float, 2>), g, dim3(BT), 0, c.stream, x, r0, r1, u0, u1, u2, c.scal, n, nullptr);
+  DDH_LAUNCH_OK("PBCGS2 x, r update", 1);
+  return 0;
+}
+
+int p21_u2(Context& c, float2* u0, float2* u1, const float2* r0, const float2* r1, size_t n) {
+  if (c.precision != PREC_FP32)
+    hipLaunchKernelGGL(kb_p2_u2<double>, single_grid(n), dim3(BT), 0, c.stream, u0, u1, r0, r1, c.scal, n, nullptr);
+  else
+    hipLaunchKernelGGL(kb_p2_u2<float>, single_grid(n), dim3(BT), 0, c.stream, u0, u1, r0, r1, c.scal, n, nullptr);
+  DDH_LAUNCH_OK("PBCGS2 u update", 1);
+  return 0;
+}
+
+int p21_comb(Context& c, float2* u0, float2* x, float2* r0, const float2* u1, const float2* u2, const float2* r1,
+             const float2* r2, size_t n) {
+  if (c.precision != PREC_FP32)
+    hipLaunchKernelGGL(kb_p2_comb<double>, single_grid(n), dim3(BT), 0, c.stream, u0, x, r0, u1, u2, r1, r2, c.scal,
+                       n, nullptr);
+  else
+    hipLaunchKernelGGL(kb_p2_comb<float>, single_grid(n), dim3(BT), 0, c.stream, u0, x, r0, u1, u2, r1, r2, c.scal,
+                       n, nullptr);
+  DDH_LAUNCH_OK("PBCGS2 convex combination", 1);
+  return 0;
 }
 
 namespace {

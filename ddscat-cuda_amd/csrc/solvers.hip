@@ -170,31 +170,27 @@ int solve_pbcgs2(Context& c, const SolverArgs& a) {
       if (k == 2) d.add(r0, r0, DOT_N2);
       if (dots(c, d, n, SL_D, st)) return 94;
       hipLaunchKernelGGL(p2::k_beta, dim3(1), dim3(1), 0, st, c.scal, k == 2 ? 1 : 0);
-      for (int j = 0; j < k; ++j) {
-        LinComb l; l.lit(rk[j], 1.0).dev(uk[j], p2::BETA, 1);
-        if (lincomb(c, uk[j], l, n, st)) return 96;
+      // u_j = r_j - beta u_j, j < k (fused for k = 2)
+      if (k == 1) {
+        LinComb l; l.lit(r0, 1.0).dev(u0, p2::BETA, 1);
+        if (lincomb(c, u0, l, n, st)) return 96;
+      } else if (p21_u2(c, u0, u1, r0, r1, n)) {
+        return 96;
       }
       if (int rc = run_mv(c, 'N', uk[k - 1], uk[k], nc)) return rc;
       DotSet ds; ds.add(rr, uk[k], DOT_C);
       if (dots(c, ds, n, SL_D, st)) return 97;
       hipLaunchKernelGGL(p2::k_alpha, dim3(1), dim3(1), 0, st, c.scal);
-      { LinComb l; l.lit(x, 1.0).dev(u0, p2::ALPHA); if (lincomb(c, x, l, n, st)) return 99; }
-      for (int j = 0; j < k; ++j) {
-        LinComb l; l.lit(rk[j], 1.0).dev(uk[j + 1], p2::ALPHA, 1);
-        if (lincomb(c, rk[j], l, n, st)) return 99;
-      }
+      // x += alpha u0 ; r_j -= alpha u_{j+1}, j < k
+      if (p21_xr(c, k, x, r0, r1, u0, u1, u2, n)) return 99;
       if (int rc = run_mv(c, 'N', rk[k - 1], rk[k], nc)) return rc;
     }
     DotSet g;
     g.add(r0, r0, DOT_C).add(r1, r0, DOT_C).add(r1, r1, DOT_C).add(r2, r0, DOT_C).add(r2, r1, DOT_C).add(r2, r2, DOT_C);
     if (dots(c, g, n, SL_D, st)) return 101;
     hipLaunchKernelGGL(p2::k_convex, dim3(1), dim3(1), 0, st, c.scal);
-    {
-      LinComb lu; lu.lit(u0, 1.0).dev(u1, p2::Y1, 1).dev(u2, p2::Y2, 1);
-      LinComb lx; lx.lit(x, 1.0).dev(r0, p2::Y1).dev(r1, p2::Y2);
-      LinComb lr; lr.lit(r0, 1.0).dev(r1, p2::Y1, 1).dev(r2, p2::Y2, 1);
-      if (lincomb(c, u0, lu, n, st) || lincomb(c, x, lx, n, st) || lincomb(c, r0, lr, n, st)) return 103;
-    }
+    // u0 -= y1 u1 + y2 u2 ; x += y1 r0 + y2 r1 ; r0 -= y1 r1 + y2 r2   (cu:472-473)
+    if (p21_comb(c, u0, x, r0, u1, u2, r1, r2, n)) return 103;
     if (fetch_scalars(c, 0, 4, st)) return 103;
     const int status = (int)c.hscal[SL_STATUS];
     if (status) {

@@ -56,6 +56,16 @@ int qmr1_pq(Context& c, float2* p, float2* q, const float2* vt, const float2* at
 int qmr1_vw(Context& c, float2* vt, float2* wt, const float2* ap, const float2* q, size_t n);
 int qmr1_dsxr(Context& c, float2* d, float2* sv, float2* x, float2* r, const float2* p, const float2* ap, size_t n,
               int dst);
+// Single-RHS PBCGS2 vector steps (cu:466-473) on the batched solver's fused
+// kernels, coefficients from c.scal:
+//   xr (K = 1, 2): x += alpha u0 ; r_j -= alpha u_{j+1}, j < K
+//   u2:            u_j = r_j - beta u_j, j = 0, 1
+//   comb:          u0 -= y1 u1 + y2 u2 ; x += y1 r0 + y2 r1 ; r0 -= y1 r1 + y2 r2
+int p21_xr(Context& c, int K, float2* x, float2* r0, float2* r1, const float2* u0, const float2* u1, const float2* u2,
+           size_t n);
+int p21_u2(Context& c, float2* u0, float2* u1, const float2* r0, const float2* r1, size_t n);
+int p21_comb(Context& c, float2* u0, float2* x, float2* r0, const float2* u1, const float2* u2, const float2* r1,
+             const float2* r2, size_t n);
 
 int solve_pbcgs2(Context& c, const SolverArgs& a);
 int solve_qmrccg(Context& c, const SolverArgs& a);
