@@ -58,6 +58,25 @@ def test_slices_matvec_matches_oracle(slib, dims, cwhat):
     assert np.all(y.reshape(3, -1)[:, iocc == 0] == 0)
 
 
+def test_slices_lds_kernels_on_register_lengths(slib):
+    """ddscat_hip_set_fft_kernels(1) keeps the LDS-staged SLICES kernels on a
+    grid whose axes all have register shapes (the register z path's buffer
+    is allocated but unused)."""
+    nx, ny, nz = 16, 24, 32
+    rng = np.random.default_rng(162432)
+    iocc, x, adia, aoff = random_case(rng, nx, ny, nz)
+    goct = O.green_octant(nx, ny, nz, np.float64(np.float32(0.29)))
+    try:
+        slib.set_fft_kernels(1)
+        slib.prepare_green(nx, ny, nz, iocc, (0.29, 0.0, 0.0))
+        for cw in ("N", "C"):
+            err = rel_l2(slib.apply(cw, x, adia, aoff), O.matvec_fft(cw, x, adia, aoff, goct, nx, ny, nz, iocc))
+            print(f"SLICES LDS-staged kernels {cw}: {err:.2e}")
+            assert err < 1e-5
+    finally:
+        slib.set_fft_kernels(0)
+
+
 def test_slices_compact_layout_and_solver(slib):
     """D=16 sphere: every padded axis is 32 points, so the x, y and z passes
     run the register kernels and the compact layout goes through the site-map
