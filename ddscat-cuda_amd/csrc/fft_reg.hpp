@@ -377,6 +377,22 @@ __device__ __forceinline__ void fft2_inv_s(float2* v, float* __restrict__ lds, c
   dft_reg<R2, DIR>(v);
 }
 
+// e^{DIR 2 pi i q / M} for M = 2, 3
+template <int M, int DIR>
+__device__ __forceinline__ float2 root_m(int q) {
+  if (M == 2) return make_float2(q ? -1.f : 1.f, 0.f);
+  constexpr float h = 0.8660254037844386f;  // sqrt(3) / 2
+  return q == 0 ? make_float2(1.f, 0.f) : make_float2(-0.5f, ((q == 1) == (DIR > 0)) ? h : -h);
+}
+// output m of an M-point DFT (sign DIR) of z[0..M)
+template <int M, int DIR>
+__device__ __forceinline__ float2 dft_m_out(const float2* z, int m) {
+  float2 r = z[0];
+#pragma unroll
+  for (int q = 1; q < M; ++q) r = cadd(r, cmul(root_m<M, DIR>((q * m) % M), z[q]));
+  return r;
+}
+
 // ---- lane-pair split DFTs: a Q-point DFT (Q even) held by two adjacent
 // lanes (h = lane & 1), Q/2 values each; one radix-2 step crosses the pair
 // through a lane shuffle.

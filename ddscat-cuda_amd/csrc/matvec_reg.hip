@@ -11,6 +11,7 @@
 // otherwise a full-site -> compact map (qmap, -1 for void sites) is used.
 #include "state.hpp"
 #include "fft_reg.hpp"
+#include <type_traits>
 #include "solver_common.hpp"
 
 namespace ddhip {
@@ -155,6 +156,100 @@ __global__ __launch_bounds__(R1 * C) void kr_passy(const float2* __restrict__ in
       for (int n2 = 0; n2 < R2; ++n2) {
         const int n = t + R1 * n2;
         if (lower_half<R2>(n2, n, nout)) dst[(size_t)n * gx] = v[n2];
+      }
+    }
+  }
+}
+
+// ------------------------------------------------ passes 2, 4, L = M * 256
+// Column pass of kr_passy for L = M * 256 (M = 2, 3) by the decimation of
+// kr_pass3m: one component per thread, 16 values each, exchanges of real and
+// imaginary parts in turn (256-point FFTs and the M-point combine), so a
+// 768-point column tile of C = 8 needs about 36 KB of LDS instead of the
+// 61 KB of the two-level 32 x 24 shape.  DIR = +1: nin (<= L/2, pruned)
+// inputs -> L spectral outputs; DIR = -1: L spectral inputs -> nout (<= L/2).
+template <int M, int C, int DIR>
+__global__ __launch_bounds__(M * 16 * C) void kr_passym(const float2* __restrict__ in, float2* __restrict__ out,
+                                                    const float2* __restrict__ twg, int gx, int nin, int nout, int nz,
+                                                    size_t ins, size_t outs, BatchArgs ba) {
+  constexpr int R = 16, S = R * R, L = M * S, GT = R * C, RT = M * GT;
+  constexpr int PS = M * C + 1;  // 256-point exchange: M C sequences per workgroup (floats)
+  constexpr int PC = C + 1;      // M-point exchange layout (u S + k1) PC + s, u < M (floats)
+  constexpr int XB = S * PS > M * S * PC ? S * PS : M * S * PC;
+  __shared__ float2 twL[L];
+  __shared__ float2 twS[S];
+  __shared__ float xf[XB];
+  const int rb = blockIdx.z / 3;
+  if (ba.active && !ba.active[rb]) return;
+  in += rb * (DIR > 0 ? ba.a1s : ba.a2s);
+  out += rb * (DIR > 0 ? ba.a2s : ba.a1s);
+  const int m = threadIdx.x / GT, lt = threadIdx.x % GT;
+  const int s = lt % C, t = lt / C, sv = m * C + s;
+  const int kx = xcd_tile<C>((int)blockIdx.x, (int)gridDim.x, false) * C + s, iz = blockIdx.y, c = blockIdx.z % 3;
+  for (int i = threadIdx.x; i < L; i += RT) twL[i] = twg[i];
+  for (int i = threadIdx.x; i < S; i += RT) twS[i] = twg[M * i];
+  const bool live = kx < gx;
+  const float2* src = in + ((size_t)c * nz + iz) * ins + kx;
+  float2* dst = out + ((size_t)c * nz + iz) * outs + kx;
+  float2 v[R];
+  // M-point DFT across the groups, through LDS in two halves: the real
+  // parts cross first and are held (M R floats), then each output reads the
+  // imaginary parts at its k1 and combines (DIR: sign of the M-point DFT;
+  // the inverse then applies the conjugate twiddle w_L^{-m k1})
+  auto combine = [&](auto dirc) {
+    constexpr int D = decltype(dirc)::value;
+    float zr[M][R];
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < R; ++i) xf[(m * S + t + R * i) * PC + s] = v[i].x;
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < R; ++i)
+#pragma unroll
+      for (int u = 0; u < M; ++u) zr[u][i] = xf[(u * S + t + R * i) * PC + s];
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < R; ++i) xf[(m * S + t + R * i) * PC + s] = v[i].y;
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+      float2 z[M];
+#pragma unroll
+      for (int u = 0; u < M; ++u) z[u] = make_float2(zr[u][i], xf[(u * S + t + R * i) * PC + s]);
+      v[i] = dft_m_out<M, D>(z, m);
+      if (D < 0) {
+        float2 w = twL[m * (t + R * i)];
+        w.y = -w.y;
+        v[i] = cmul(v[i], w);
+      }
+    }
+  };
+  if (DIR > 0) {
+#pragma unroll
+    for (int n2 = 0; n2 < R; ++n2) {
+      const int n = M * (t + R * n2) + m;
+      v[n2] = (live && n2 < R / 2 && n < nin) ? src[(size_t)n * gx] : make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+    fft2_fwd_s<R, R, +1>(v, xf, twS, t, sv, PS);  // v[i] = Y_m[t + 16 i]
+#pragma unroll
+    for (int i = 0; i < R; ++i) v[i] = cmul(v[i], twL[m * (t + R * i)]);
+    combine(std::integral_constant<int, +1>{});  // v[i] = X[t + 16 i + 256 m]
+    if (live) {
+#pragma unroll
+      for (int i = 0; i < R; ++i) dst[(size_t)(t + R * i + S * m) * gx] = v[i];
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < R; ++i) v[i] = live ? src[(size_t)(t + R * i + S * m) * gx] : make_float2(0.f, 0.f);
+    combine(std::integral_constant<int, -1>{});  // v[i] = W_m[t + 16 i]
+    __syncthreads();
+    fft2_inv_s<R, R, -1>(v, xf, twS, t, sv, PS);  // v[n2] = x[M (t + 16 n2) + m]
+    if (live) {
+#pragma unroll
+      for (int n2 = 0; n2 < R / 2; ++n2) {
+        const int n = M * (t + R * n2) + m;
+        if (n < nout) dst[(size_t)n * gx] = v[n2];
       }
     }
   }
@@ -484,22 +579,6 @@ __global__ __launch_bounds__(3 * R1 * C) void kr_pass3c(float2* __restrict__ a2,
 }
 
 // ------------------------------------------------ pass 3, L = M * 256
-// e^{DIR 2 pi i q / M} for M = 2, 3
-template <int M, int DIR>
-__device__ __forceinline__ float2 root_m(int q) {
-  if (M == 2) return make_float2(q ? -1.f : 1.f, 0.f);
-  constexpr float h = 0.8660254037844386f;  // sqrt(3) / 2
-  return q == 0 ? make_float2(1.f, 0.f) : make_float2(-0.5f, ((q == 1) == (DIR > 0)) ? h : -h);
-}
-// output m of an M-point DFT (sign DIR) of z[0..M)
-template <int M, int DIR>
-__device__ __forceinline__ float2 dft_m_out(const float2* z, int m) {
-  float2 r = z[0];
-#pragma unroll
-  for (int q = 1; q < M; ++q) r = cadd(r, cmul(root_m<M, DIR>((q * m) % M), z[q]));
-  return r;
-}
-
 // Pass 3 for L = M * 256 (M = 2: 512, M = 3: 768 points) by decimation in
 // time over 256-point register FFTs (16 x 16).  Thread group m of M
 // transforms x[M n1 + m] (zero for n1 >= 128, pruned at compile time as in
@@ -515,7 +594,7 @@ __global__ __launch_bounds__(M * 16 * C) void kr_pass3m(float2* __restrict__ a2,
                                                     int gy, size_t plane, BatchArgs ba, GreenGeo gg) {
   constexpr int R = 16, S = R * R, L = M * S, GT = R * C, RT = M * GT;
   constexpr int PS = M * C + 1;  // 256-point exchange: M C sequences per workgroup
-  constexpr int PC = C + 1;      // M-point exchange layout (q S + k1) PC + s
+  constexpr int PC = C + 1;      // M-point exchange layout (u S + k1) PC + s, u < M
   constexpr int XB = S * PS > M * S * PC ? S * PS : M * S * PC;
   __shared__ float2 twL[L];
   __shared__ float2 twS[S];
@@ -555,7 +634,7 @@ __global__ __launch_bounds__(M * 16 * C) void kr_pass3m(float2* __restrict__ a2,
     for (int i = 0; i < R; ++i) {
       float2 z[M];
 #pragma unroll
-      for (int q = 0; q < M; ++q) z[q] = xb[(q * S + t + R * i) * PC + s];
+      for (int u = 0; u < M; ++u) z[u] = xb[(u * S + t + R * i) * PC + s];
       v[c][i] = dft_m_out<M, +1>(z, m);  // X[t + 16 i + 256 m]
     }
   }
@@ -572,7 +651,7 @@ __global__ __launch_bounds__(M * 16 * C) void kr_pass3m(float2* __restrict__ a2,
     for (int i = 0; i < R; ++i) {
       float2 z[M];
 #pragma unroll
-      for (int q = 0; q < M; ++q) z[q] = xb[(q * S + t + R * i) * PC + s];
+      for (int u = 0; u < M; ++u) z[u] = xb[(u * S + t + R * i) * PC + s];
       float2 w = twL[m * (t + R * i)];
       w.y = -w.y;
       v[c][i] = cmul(dft_m_out<M, -1>(z, m), w);  // W_m[t + 16 i]
@@ -977,6 +1056,21 @@ static void launch_reg1(Context& c, int cj, int compact, const float2* x, float2
                        bt.a1, m.twy, m.gx, m.gy, c.ny, c.nz, m.p2, m.p1, bt.k);
 }
 
+// passes 2 and 4 by kr_passym; listed for L = gy = M * 256
+template <int M, int C>
+static void launch_ym(Context& c, int cj, int compact, const float2* x, float2* y, const float2* adia,
+                      const float2* aoff, hipStream_t st, const MvBatch& bt, int which) {
+  (void)cj; (void)compact; (void)x; (void)y; (void)adia; (void)aoff;
+  const MatvecPlan& m = c.mv;
+  const dim3 grid((m.gx + C - 1) / C, c.nz, 3 * (unsigned)bt.nrhs), block(M * 16 * C);
+  if (which == 2)
+    hipLaunchKernelGGL((kr_passym<M, C, +1>), grid, block, 0, st, bt.a1, bt.a2, m.twy, m.gx, c.ny, m.gy, c.nz, m.p1,
+                       m.p2, bt.k);
+  else if (which == 4)
+    hipLaunchKernelGGL((kr_passym<M, C, -1>), grid, block, 0, st, bt.a2, bt.a1, m.twy, m.gx, m.gy, c.ny, c.nz, m.p2,
+                       m.p1, bt.k);
+}
+
 // pass 3 by kr_pass3m; serves pass 3 only, listed for L = gz = 2 NZ = M * 256
 template <int M, int C>
 static void launch_p3m(Context& c, int cj, int compact, const float2* x, float2* y, const float2* adia,
@@ -1038,13 +1132,18 @@ static const RegShape kShapes[] = {
 };
 #define DDH_PAIR(L_, Q_, CR_, CP_) RegShape{L_, Q_, Q_, CR_, &launch_pair<Q_, CR_, CP_>, 1 << 3}
 // Autotuner candidates listed after the defaults of their lengths: lane-pair
-// and mirror-row pass-3 kernels, and a narrower pass-1 tile.
+// and mirror-row pass-3 kernels, a narrower pass-1 tile, and the 512- and
+// 768-point y passes by decimation over 256-point FFTs.
 static const RegShape kPairShapes[] = {
     DDH_PAIR(64, 8, 32, 32), DDH_PAIR(144, 12, 16, 16), DDH_PAIR(256, 16, 16, 16), DDH_PAIR(256, 16, 16, 8),
     DDH_SHAPE(256, 16, 16, 32),
     RegShape{256, 16, 16, 16, &launch_ypair<16, 16, 16>, 1 << 3}, RegShape{128, 8, 16, 32, &launch_ypair<8, 16, 32>, 1 << 3},
     RegShape{64, 8, 8, 32, &launch_ypair<8, 8, 32>, 1 << 3},
     RegShape{256, 16, 16, 8, &launch_reg1<16, 16, 8>, 1 << 1},
+    RegShape{512, 16, 16, 8, &launch_ym<2, 8>, (1 << 2) | (1 << 4)},
+    RegShape{512, 16, 16, 16, &launch_ym<2, 16>, (1 << 2) | (1 << 4)},
+    RegShape{768, 16, 16, 8, &launch_ym<3, 8>, (1 << 2) | (1 << 4)},
+    RegShape{768, 16, 16, 16, &launch_ym<3, 16>, (1 << 2) | (1 << 4)},
 };
 #undef DDH_PAIR
 #undef DDH_SHAPE
@@ -1159,12 +1258,27 @@ static void launch_slz(Context& c, int dir, const float2* in, float2* out, hipSt
     hipLaunchKernelGGL((kr_passy<R1, R2, C, -1>), grid, dim3(R1 * C), 0, st, in, out, m.twz, plane, m.gz, c.nz, 1, full,
                        half, BatchArgs{});
 }
+template <int M, int C>
+static void launch_slz_m(Context& c, int dir, const float2* in, float2* out, hipStream_t st) {
+  const MatvecPlan& m = c.mv;
+  const int plane = c.nx * c.ny;
+  const size_t full = (size_t)m.gz * plane, half = (size_t)c.nz * plane;
+  const dim3 grid((plane + C - 1) / C, 1, 3), block(M * 16 * C);
+  if (dir > 0)
+    hipLaunchKernelGGL((kr_passym<M, C, +1>), grid, block, 0, st, in, out, m.twz, plane, c.nz, m.gz, 1, half, full,
+                       BatchArgs{});
+  else
+    hipLaunchKernelGGL((kr_passym<M, C, -1>), grid, block, 0, st, in, out, m.twz, plane, m.gz, c.nz, 1, full, half,
+                       BatchArgs{});
+}
 using SlzLaunch = void (*)(Context&, int, const float2*, float2*, hipStream_t);
 struct SlzShape {
   int L;
   SlzLaunch fn;
 };
-// the default column shape of each register length (passes 2 and 4 of FFT3D)
+// the default column shape of each register length (passes 2 and 4 of FFT3D);
+// 512 and 768 points by decimation over 256-point FFTs (kr_passym, the faster
+// y pass of FFT3D at those lengths)
 static const SlzShape kSlzShapes[] = {
     {32, &launch_slz<4, 8, 64>},     {48, &launch_slz<8, 6, 32>},     {60, &launch_slz<10, 6, 32>},
     {64, &launch_slz<8, 8, 32>},     {72, &launch_slz<12, 6, 16>},    {80, &launch_slz<10, 8, 32>},
@@ -1172,7 +1286,7 @@ static const SlzShape kSlzShapes[] = {
     {120, &launch_slz<12, 10, 16>},  {128, &launch_slz<8, 16, 32>},   {144, &launch_slz<12, 12, 16>},
     {150, &launch_slz<15, 10, 16>},  {160, &launch_slz<16, 10, 16>},  {180, &launch_slz<15, 12, 16>},
     {192, &launch_slz<16, 12, 16>},  {240, &launch_slz<16, 15, 16>},  {256, &launch_slz<16, 16, 16>},
-    {512, &launch_slz<32, 16, 8>},   {768, &launch_slz<32, 24, 8>},
+    {512, &launch_slz_m<2, 16>},     {768, &launch_slz_m<3, 16>},
 };
 
 int slices_z_reg_length(int L) {
