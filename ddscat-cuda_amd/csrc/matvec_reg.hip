@@ -11,8 +11,8 @@
 // otherwise a full-site -> compact map (qmap, -1 for void sites) is used.
 #include "state.hpp"
 #include "fft_reg.hpp"
-#include <type_traits>
 #include "solver_common.hpp"
+#include <type_traits>
 
 namespace ddhip {
 
@@ -850,26 +850,26 @@ __global__ __launch_bounds__(2 * Q * C) void kr_pass3p(float2* __restrict__ a2, 
 
 // ------------------------------------------------------------------ pass 5
 // finish of row g (cu:116-118): y = adia x - w/G + offdiag(aoff) x on the
-// unpadded outputs ix = t + R1 n2 < nx; w0..w2 hold the inverse-transformed
-// components at index n2 (compact or full layout, conj for 'C').  Site
-// indices are resolved first with the layout test outside the element loop;
-// the loads then run unconditionally in groups of FG outputs (an unoccupied
-// site reads entry 0 and stores nothing), so each group's 9 FG loads are in
-// flight together instead of waiting on per-output branches.
-template <int R1, int R2, int FG>
-__device__ __forceinline__ void p5_finish(const float2* w0, const float2* w1, const float2* w2, int t, int g,
-                                          const float2* __restrict__ x, float2* __restrict__ y,
-                                          const float2* __restrict__ adia, const float2* __restrict__ aoff,
-                                          const int16_t* __restrict__ iocc, const SiteMap& sm, int nx, int nat,
-                                          int nat0, int compact, int cj, float inv) {
-  constexpr int H = (R2 + 1) / 2;  // outputs n2 >= H lie in the padding
+// unpadded outputs; w0..w2[n2] hold the inverse-transformed components at
+// row index ix(n2) for n2 < H, kept(n2, ix) says whether ix < nx (compact
+// or full layout, conj for 'C').  Site indices are resolved first with the
+// layout test outside the element loop; the loads then run unconditionally
+// in groups of FG outputs (an unoccupied site reads entry 0 and stores
+// nothing), so each group's 9 FG loads are in flight together instead of
+// waiting on per-output branches.
+template <int H, int FG, typename IX, typename KEPT>
+__device__ __forceinline__ void p5_finish_at(const float2* w0, const float2* w1, const float2* w2, IX ixof, KEPT kept,
+                                             int g, const float2* __restrict__ x, float2* __restrict__ y,
+                                             const float2* __restrict__ adia, const float2* __restrict__ aoff,
+                                             const int16_t* __restrict__ iocc, const SiteMap& sm, int nx, int nat,
+                                             int nat0, int compact, int cj, float inv) {
   size_t o[H];
   bool st[H], zr[H];
   if (!compact) {
 #pragma unroll
     for (int n2 = 0; n2 < H; ++n2) {
-      const int ix = t + R1 * n2;
-      st[n2] = lower_half<R2>(n2, ix, nx);
+      const int ix = ixof(n2);
+      st[n2] = kept(n2, ix);
       o[n2] = st[n2] ? (size_t)g * nx + ix : 0;
       zr[n2] = iocc && iocc[o[n2]] == 0;
     }
@@ -877,16 +877,16 @@ __device__ __forceinline__ void p5_finish(const float2* w0, const float2* w1, co
     const int lo = sm.rowlo[g], q0 = sm.rowptr[g], cnt = sm.rowptr[g + 1] - q0;
 #pragma unroll
     for (int n2 = 0; n2 < H; ++n2) {
-      const int ix = t + R1 * n2, d = ix - lo;
-      st[n2] = lower_half<R2>(n2, ix, nx) && d >= 0 && d < cnt;
+      const int ix = ixof(n2), d = ix - lo;
+      st[n2] = kept(n2, ix) && d >= 0 && d < cnt;
       o[n2] = st[n2] ? (size_t)(q0 + d) : 0;
       zr[n2] = false;
     }
   } else {
 #pragma unroll
     for (int n2 = 0; n2 < H; ++n2) {
-      const int ix = t + R1 * n2;
-      const int q = lower_half<R2>(n2, ix, nx) ? sm.qmap[(size_t)g * nx + ix] : -1;
+      const int ix = ixof(n2);
+      const int q = kept(n2, ix) ? sm.qmap[(size_t)g * nx + ix] : -1;
       st[n2] = q >= 0;
       o[n2] = st[n2] ? (size_t)q : 0;
       zr[n2] = false;
@@ -925,6 +925,20 @@ __device__ __forceinline__ void p5_finish(const float2* w0, const float2* w1, co
       }
     }
   }
+}
+
+// finish in the register order of fft2_inv: output n2 < (R2 + 1) / 2 of
+// thread t is row index t + R1 n2
+template <int R1, int R2, int FG>
+__device__ __forceinline__ void p5_finish(const float2* w0, const float2* w1, const float2* w2, int t, int g,
+                                          const float2* __restrict__ x, float2* __restrict__ y,
+                                          const float2* __restrict__ adia, const float2* __restrict__ aoff,
+                                          const int16_t* __restrict__ iocc, const SiteMap& sm, int nx, int nat,
+                                          int nat0, int compact, int cj, float inv) {
+  p5_finish_at<(R2 + 1) / 2, FG>(
+      w0, w1, w2, [=](int n2) { return t + R1 * n2; },
+      [=](int n2, int ix) { return lower_half<R2>(n2, ix, nx); }, g, x, y, adia, aoff, iocc, sm, nx, nat, nat0,
+      compact, cj, inv);
 }
 
 // pass-4 output of one component of row g in the register order of fft2_inv;
@@ -973,6 +987,130 @@ __global__ __launch_bounds__(R1 * C) void kr_pass5(const float2* __restrict__ a1
   fft2_inv<R1, R2, -1, 1>(v2, buf, twL, t, s, P);
   if (!live) return;
   p5_finish<R1, R2, 4>(v0, v1, v2, t, g, x, y, adia, aoff, iocc, sm, nx, nat, nat0, compact, cj, inv);
+}
+
+// ------------------------------------------------ pass 5, L = M * 256
+// Pass 5 for L = M * 256 (M = 2, 3) by the decimation of kr_passym along
+// the contiguous x rows: thread t of group m reads spectral elements
+// t + 16 i + 256 m of row g (16 lanes cover 128 contiguous bytes).  The
+// three components are transformed in turn.  The kept outputs
+// x[M (t + 16 n2) + m], n2 < 8 (n < L/2), cross LDS once more into row order,
+// so that thread q = 16 m + t of the row finishes row indices q + 16 M j,
+// j < 8, with the coalesced loads and stores of the register kernels.
+// KEEPL parks components 0 and 1 in LDS in row order until the finish
+// instead of in registers: 167 instead of 194 VGPRs at M = 3, C = 4 (3
+// waves per SIMD; 48 KB of LDS per workgroup).
+// M-point DFT across the M groups of a decimated L = M * 256 transform, as
+// in kr_passym
+// (thread t of group m holds v[i], element t + 16 i of its 256-point
+// sequence s), through LDS in two halves: the real parts cross first and are
+// held (M 16 floats), then each output reads the imaginary parts at its
+// element and combines.  D: sign of the M-point DFT; the inverse (D < 0)
+// then applies the conjugate twiddle w_L^{-m (t + 16 i)}.  Layout of xf:
+// (u 256 + t + 16 i) (C + 1) + s for group u; the caller's exchange buffer
+// holds at least M 256 (C + 1) floats.
+template <int M, int C, int D>
+__device__ __forceinline__ void dec_combine(float2* v, float* xf, const float2* twL, int m,
+                                            int t, int s) {
+  constexpr int R = 16, S = R * R, PC = C + 1;
+  float zr[M][R];
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < R; ++i) xf[(m * S + t + R * i) * PC + s] = v[i].x;
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < R; ++i)
+#pragma unroll
+    for (int u = 0; u < M; ++u) zr[u][i] = xf[(u * S + t + R * i) * PC + s];
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < R; ++i) xf[(m * S + t + R * i) * PC + s] = v[i].y;
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < R; ++i) {
+    float2 z[M];
+#pragma unroll
+    for (int u = 0; u < M; ++u) z[u] = make_float2(zr[u][i], xf[(u * S + t + R * i) * PC + s]);
+    v[i] = dft_m_out<M, D>(z, m);
+    if (D < 0) {
+      float2 w = twL[m * (t + R * i)];
+      w.y = -w.y;
+      v[i] = cmul(v[i], w);
+    }
+  }
+}
+
+template <int M, int C, bool KEEPL>
+__global__ __launch_bounds__(M * 16 * C) void kr_pass5m(const float2* __restrict__ a1, const float2* __restrict__ twg,
+                                                    const float2* __restrict__ x, float2* __restrict__ y,
+                                                    const float2* __restrict__ adia, const float2* __restrict__ aoff,
+                                                    const int16_t* __restrict__ iocc, SiteMap sm, int nx, int nrows,
+                                                    int nat, int nat0, int compact, int cj, float inv, BatchArgs ba) {
+  constexpr int R = 16, S = R * R, L = M * S, H = L / 2, GT = R * C, RT = M * GT;
+  constexpr int PS = M * C + 1;       // 256-point exchange (floats)
+  constexpr int PC = C + 1;           // M-point exchange (floats)
+  constexpr int PR = H + 1;           // row-order exchange: real at s PR + n, imaginary at (C + s) PR + n
+  constexpr int XA = S * PS > M * S * PC ? S * PS : M * S * PC;
+  constexpr int XB = XA > 2 * C * PR ? XA : 2 * C * PR;
+  constexpr int RB = 2 * C * PR;      // one component in row order
+  __shared__ float2 twL[L];
+  __shared__ float2 twS[S];
+  __shared__ float xf[XB];
+  __shared__ float rowk[KEEPL ? 2 * RB : 1];  // KEEPL: components 0, 1 in row order
+  const int rb = blockIdx.y;
+  if (ba.active && !ba.active[rb]) return;
+  a1 += rb * ba.a1s;
+  x += rb * ba.xs;
+  y += rb * ba.xs;
+  adia += rb * ba.os;
+  aoff += rb * ba.os;
+  const int m = threadIdx.x / GT, lt = threadIdx.x % GT;
+  const int t = lt % R, s = lt / R, sv = m * C + s, q = R * m + t;
+  const int g = blockIdx.x * C + s;
+  for (int i = threadIdx.x; i < L; i += RT) twL[i] = twg[i];
+  for (int i = threadIdx.x; i < S; i += RT) twS[i] = twg[M * i];
+  const bool live = g < nrows;
+  const size_t cs = (size_t)nrows * L;
+  const float2* src = a1 + (size_t)(live ? g : nrows - 1) * L + t + S * m;
+  float2 w[3][R / 2];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    float2 v[R];
+#pragma unroll
+    for (int i = 0; i < R; ++i) v[i] = src[c * cs + R * i];
+    dec_combine<M, C, -1>(v, xf, twL, m, t, s);  // v[i] = W_m[t + 16 i]
+    __syncthreads();
+    fft2_inv_s<R, R, -1>(v, xf, twS, t, sv, PS);  // v[n2] = x[M (t + 16 n2) + m]
+    float* ro = KEEPL && c < 2 ? rowk + c * RB : xf;
+    if (!KEEPL || c == 2) __syncthreads();
+#pragma unroll
+    for (int n2 = 0; n2 < R / 2; ++n2) {
+      const int n = M * (t + R * n2) + m;
+      ro[s * PR + n] = v[n2].x;
+      ro[(C + s) * PR + n] = v[n2].y;
+    }
+    if (!KEEPL || c == 2) {
+      __syncthreads();
+#pragma unroll
+      for (int j = 0; j < R / 2; ++j) {
+        const int n = q + R * M * j;
+        w[c][j] = make_float2(xf[s * PR + n], xf[(C + s) * PR + n]);
+      }
+    }
+  }
+  if (KEEPL) {  // written before the barriers of component 2
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+      for (int j = 0; j < R / 2; ++j) {
+        const int n = q + R * M * j;
+        w[c][j] = make_float2(rowk[c * RB + s * PR + n], rowk[c * RB + (C + s) * PR + n]);
+      }
+  }
+  if (!live) return;
+  p5_finish_at<R / 2, 4>(
+      w[0], w[1], w[2], [=](int j) { return q + R * M * j; }, [=](int, int ix) { return ix < nx; }, g, x, y, adia,
+      aoff, iocc, sm, nx, nat, nat0, compact, cj, inv);
 }
 
 // ------------------------------------------------------------------ host --
@@ -1071,6 +1209,18 @@ static void launch_ym(Context& c, int cj, int compact, const float2* x, float2* 
                        m.p1, bt.k);
 }
 
+// pass 5 by kr_pass5m; serves pass 5 only, listed for L = gx = M * 256
+template <int M, int C, bool KEEPL>
+static void launch_p5m(Context& c, int cj, int compact, const float2* x, float2* y, const float2* adia,
+                       const float2* aoff, hipStream_t st, const MvBatch& bt, int which) {
+  const MatvecPlan& m = c.mv;
+  const int nrows = c.ny * c.nz;
+  SiteMap sm{c.rowptr, c.rowlo, c.qmap, c.rows_contiguous};
+  hipLaunchKernelGGL((kr_pass5m<M, C, KEEPL>), dim3((nrows + C - 1) / C, (unsigned)bt.nrhs), dim3(M * 16 * C), 0, st,
+                     bt.a1, m.twx, x, y, adia, aoff, compact ? nullptr : c.iocc, sm, c.nx, nrows, c.nat, c.nat0,
+                     compact, cj, 1.f / (float)m.ngrid, bt.k);
+}
+
 // pass 3 by kr_pass3m; serves pass 3 only, listed for L = gz = 2 NZ = M * 256
 template <int M, int C>
 static void launch_p3m(Context& c, int cj, int compact, const float2* x, float2* y, const float2* adia,
@@ -1133,7 +1283,7 @@ static const RegShape kShapes[] = {
 #define DDH_PAIR(L_, Q_, CR_, CP_) RegShape{L_, Q_, Q_, CR_, &launch_pair<Q_, CR_, CP_>, 1 << 3}
 // Autotuner candidates listed after the defaults of their lengths: lane-pair
 // and mirror-row pass-3 kernels, a narrower pass-1 tile, and the 512- and
-// 768-point y passes by decimation over 256-point FFTs.
+// 768-point y passes and pass 5 by decimation over 256-point FFTs.
 static const RegShape kPairShapes[] = {
     DDH_PAIR(64, 8, 32, 32), DDH_PAIR(144, 12, 16, 16), DDH_PAIR(256, 16, 16, 16), DDH_PAIR(256, 16, 16, 8),
     DDH_SHAPE(256, 16, 16, 32),
@@ -1144,6 +1294,9 @@ static const RegShape kPairShapes[] = {
     RegShape{512, 16, 16, 16, &launch_ym<2, 16>, (1 << 2) | (1 << 4)},
     RegShape{768, 16, 16, 8, &launch_ym<3, 8>, (1 << 2) | (1 << 4)},
     RegShape{768, 16, 16, 16, &launch_ym<3, 16>, (1 << 2) | (1 << 4)},
+    RegShape{512, 16, 16, 4, &launch_p5m<2, 4, false>, 1 << 5}, RegShape{512, 16, 16, 8, &launch_p5m<2, 8, false>, 1 << 5},
+    RegShape{768, 16, 16, 4, &launch_p5m<3, 4, true>, 1 << 5}, RegShape{768, 16, 16, 4, &launch_p5m<3, 4, false>, 1 << 5},
+    RegShape{768, 16, 16, 8, &launch_p5m<3, 8, false>, 1 << 5},
 };
 #undef DDH_PAIR
 #undef DDH_SHAPE

@@ -114,11 +114,11 @@ def test_apply_batch_shared_operator_matches_oracle(lib, shape):
         assert err < 1e-5, (r, err)
 
 
-@pytest.mark.parametrize("shape", [(128, 6, 4), (4, 6, 256), (4, 6, 384), (6, 256, 4), (384, 4, 6)])
+@pytest.mark.parametrize("shape", [(128, 6, 4), (4, 6, 256), (4, 6, 384), (6, 256, 4), (384, 4, 6), (256, 5, 4)])
 def test_apply_batch_register_shapes_bit_identical(lib, shape):
     """Batched MATVEC with the 256-, 512- and 768-point register kernels
-    (pass 3 by 2 x 256 and 3 x 256 decimation, the RHS index in the grid)
-    bit-identical to the single-RHS MATVEC of each RHS."""
+    (passes 3 and 5 by 2 x 256 and 3 x 256 decimation, the RHS index in the
+    grid) bit-identical to the single-RHS MATVEC of each RHS."""
     import torch
     nx, ny, nz = shape
     rng = np.random.default_rng(nx * 7 + ny * 3 + nz)
@@ -131,23 +131,27 @@ def test_apply_batch_register_shapes_bit_identical(lib, shape):
     x, ad, ao = cx(B, n0), cx(B, n0), 0.3 * cx(B, n0)
     xd, add, aod = dev(x), dev(ad), dev(ao)
     yd = torch.empty_like(xd)
-    kind, checked = -1, 0
-    while lib.set_pass_kernel(3, kind, check=False) == 0:  # every pass-3 kernel, batched and single
-        torch.cuda.synchronize()
-        lib.apply_batch("N", B, xd.data_ptr(), yd.data_ptr(), add.data_ptr(), aod.data_ptr(), op_per_rhs=True)
-        lib.sync()
-        yb = host(yd).reshape(B, -1)
-        for r in range(B):
-            xs, ys = dev(x[r]), torch.empty(2 * n0, dtype=torch.float32, device="cuda")
-            a1, a2 = dev(ad[r]), dev(ao[r])
-            torch.cuda.synchronize()
-            lib.apply_device("N", xs.data_ptr(), ys.data_ptr(), a1.data_ptr(), a2.data_ptr(), layout=1)
-            lib.sync()
-            assert np.array_equal(host(ys), yb[r]), (shape, kind, r)
-        checked += 1
-        kind += 1
     info = lib.info()
-    print(shape, "pass-3 kernels checked:", checked, "tuned", [info[f"kernel_pass{p}"] for p in range(1, 6)])
+    base = {p: info[f"kernel_pass{p}"] for p in (3, 5)}
+    checked = 0
+    for p in (3, 5):  # every pass-3 and pass-5 kernel, batched and single
+        kind = -1
+        while lib.set_pass_kernel(p, kind, check=False) == 0:
+            torch.cuda.synchronize()
+            lib.apply_batch("N", B, xd.data_ptr(), yd.data_ptr(), add.data_ptr(), aod.data_ptr(), op_per_rhs=True)
+            lib.sync()
+            yb = host(yd).reshape(B, -1)
+            for r in range(B):
+                xs, ys = dev(x[r]), torch.empty(2 * n0, dtype=torch.float32, device="cuda")
+                a1, a2 = dev(ad[r]), dev(ao[r])
+                torch.cuda.synchronize()
+                lib.apply_device("N", xs.data_ptr(), ys.data_ptr(), a1.data_ptr(), a2.data_ptr(), layout=1)
+                lib.sync()
+                assert np.array_equal(host(ys), yb[r]), (shape, p, kind, r)
+            checked += 1
+            kind += 1
+        lib.set_pass_kernel(p, base[p])
+    print(shape, "pass-3/5 kernels checked:", checked, "tuned", [info[f"kernel_pass{p}"] for p in range(1, 6)])
     lib.release()
 
 

@@ -317,3 +317,54 @@ def test_every_kernel_candidate_matches_oracle(lib, dims):
         lib.set_pass_kernel(p, base[p - 1])
     print(f"{dims}: {checked} kernel choices checked, tuned {base}")
     assert checked >= 6  # at least one register choice besides the LDS kernels
+
+
+@pytest.mark.parametrize("dims", [(384, 5, 4), (256, 4, 5)])
+@pytest.mark.parametrize("rows", ["interval", "sitemap"])
+@pytest.mark.parametrize("cwhat", ["N", "C"])
+def test_pass5_kernels_on_compact_vectors(lib, dims, rows, cwhat):
+    """Every pass-5 kernel of a 768- or 512-point x axis (the 3 x 256 and
+    2 x 256 decimated register kernels and the LDS-staged one) on compact
+    [3][NAT0] device vectors: occupied sites forming one interval per row
+    (index-free addressing) or scattered (site map), 'N' and 'C'."""
+    import torch
+    nx, ny, nz = dims
+    rng = np.random.default_rng(nx + 5 * ny + 11 * nz + (rows == "sitemap"))
+    if rows == "interval":
+        lo = rng.integers(0, nx // 2, size=ny * nz)
+        ln = rng.integers(0, nx - lo)
+        lo[0], ln[0] = 0, max(int(ln[0]), 1)  # row 0 starts at site 0, as NAT0 > 0 needs a site
+        ix = np.arange(nx)
+        iocc = ((ix >= lo[:, None]) & (ix < (lo + ln)[:, None])).astype(np.int16).reshape(-1)
+    else:
+        iocc = (rng.random(nx * ny * nz) < 0.6).astype(np.int16)
+        iocc[0] = 1
+    occ = np.flatnonzero(iocc)
+    n0 = 3 * occ.size
+    cx = lambda s=1.0: (s * (rng.normal(size=n0) + 1j * rng.normal(size=n0))).astype(np.complex64)
+    x, ad, ao = cx(), cx(), cx(0.3)
+    akd = (0.27, 0.05, 0.1)
+    lib.prepare_green(nx, ny, nz, iocc, akd)
+    full = lambda v: np.zeros(3 * nx * ny * nz, complex)
+    xf, adf, aof = full(0), full(0), full(0)
+    xf.reshape(3, -1)[:, occ] = x.reshape(3, -1)
+    adf.reshape(3, -1)[:, occ] = ad.reshape(3, -1)
+    aof.reshape(3, -1)[:, occ] = ao.reshape(3, -1)
+    kd = float(np.linalg.norm(np.float32(akd)))
+    yref = O.matvec_fft(cwhat, xf, adf, aof, O.green_octant(nx, ny, nz, kd), nx, ny, nz, iocc)
+    yref = yref.reshape(3, -1)[:, occ].reshape(-1)
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a).view(np.float32)).cuda()
+    xd, add, aod = dev(x), dev(ad), dev(ao)
+    yd = torch.empty_like(xd)
+    base = lib.info()["kernel_pass5"]
+    kind, checked = -1, 0
+    while lib.set_pass_kernel(5, kind, check=False) == 0:
+        torch.cuda.synchronize()
+        lib.apply_device(cwhat, xd.data_ptr(), yd.data_ptr(), add.data_ptr(), aod.data_ptr(), layout=1)
+        lib.sync()
+        err = rel_l2(yd.cpu().numpy().view(np.complex64), yref)
+        assert err < MATVEC_TOL, (dims, rows, cwhat, kind, err)
+        checked += 1
+        kind += 1
+    lib.set_pass_kernel(5, base)
+    assert checked >= 3  # the LDS-staged kernel and two decimated tiles
