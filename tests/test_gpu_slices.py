@@ -35,12 +35,13 @@ def slib(lib):
 
 # register-FFT y passes (2 NY = 32, 48, 128, 256, 256, 512 = 2 x 256,
 # 768 = 3 x 256), register z passes with the gather and the finish (2 NZ =
-# 32, 128, 256, 512, 768), and all three axes on register kernels
+# 32, 128, 256, 512, 768), register x passes at 2 NX = 512 and 768 (32 x 16,
+# 32 x 24), and all three axes on register kernels
 # (16, 24, 32); the others, and the remaining axes, run the LDS-staged kernels
 @pytest.mark.parametrize("dims", [(2, 2, 2), (3, 5, 4), (8, 8, 8), (12, 15, 16), (24, 20, 18), (10, 6, 36),
                                   (5, 16, 6), (3, 24, 5), (4, 64, 3), (3, 128, 4), (7, 128, 3), (4, 256, 3),
                                   (3, 384, 4), (6, 5, 16), (5, 7, 64), (3, 4, 128), (3, 3, 256), (2, 3, 384),
-                                  (16, 24, 32)])
+                                  (16, 24, 32), (256, 3, 4), (384, 4, 3)])
 @pytest.mark.parametrize("cwhat", ["N", "C"])
 def test_slices_matvec_matches_oracle(slib, dims, cwhat):
     nx, ny, nz = dims
