@@ -1257,8 +1257,10 @@ struct RegShape {
 // One register shape per padded axis length 2N (L = R1 R2, R1 threads per
 // sequence, C sequences per workgroup of R1 C threads).  Shapes keep a
 // thread's register array at <= 16 complex values so the three-component
-// passes (3 and 5) stay free of spills; lengths without such a split (162,
-// 200, 216, primes of 7 ...) use the LDS-staged kernels.
+// passes (3 and 5) stay free of spills; 200 = 20 x 10 and 216 = 18 x 12 hold
+// 20 and 18 values per thread (153-217 VGPRs in passes 3 and 5, no spills).
+// Lengths without such a split (162, primes of 7 ...) use the LDS-staged
+// kernels.
 // The first shape of a length is its default; further ones are autotuner
 // candidates.
 static const RegShape kShapes[] = {
@@ -1267,7 +1269,8 @@ static const RegShape kShapes[] = {
     DDH_SHAPE(96, 8, 12, 32),   DDH_SHAPE(100, 10, 10, 32), DDH_SHAPE(108, 12, 9, 16), DDH_SHAPE(120, 12, 10, 16),
     DDH_SHAPE(120, 8, 15, 32),  DDH_SHAPE(128, 8, 16, 32),  DDH_SHAPE(128, 16, 8, 16), DDH_SHAPE(144, 12, 12, 16),
     DDH_SHAPE(144, 16, 9, 16),  DDH_SHAPE(150, 15, 10, 16), DDH_SHAPE(160, 16, 10, 16), DDH_SHAPE(180, 15, 12, 16),
-    DDH_SHAPE(192, 16, 12, 16), DDH_SHAPE(240, 16, 15, 16), DDH_SHAPE(256, 16, 16, 16),
+    DDH_SHAPE(192, 16, 12, 16), DDH_SHAPE(200, 20, 10, 16), DDH_SHAPE(216, 18, 12, 16), DDH_SHAPE(240, 16, 15, 16),
+    DDH_SHAPE(256, 16, 16, 16),
     // 512 and 768 (256^3 and 384^3 targets): 32 values per thread and component;
     // single-component passes in registers, pass 3 split over component groups
     DDH_SHAPE1(512, 32, 16, 8), DDH_SHAPE1(512, 32, 16, 4),

@@ -114,11 +114,13 @@ def test_apply_batch_shared_operator_matches_oracle(lib, shape):
         assert err < 1e-5, (r, err)
 
 
-@pytest.mark.parametrize("shape", [(128, 6, 4), (4, 6, 256), (4, 6, 384), (6, 256, 4), (384, 4, 6), (256, 5, 4)])
+@pytest.mark.parametrize("shape", [(128, 6, 4), (4, 6, 256), (4, 6, 384), (6, 256, 4), (384, 4, 6), (256, 5, 4),
+                                   (4, 6, 108), (100, 5, 4)])
 def test_apply_batch_register_shapes_bit_identical(lib, shape):
-    """Batched MATVEC with the 256-, 512- and 768-point register kernels
-    (passes 3 and 5 by 2 x 256 and 3 x 256 decimation, the RHS index in the
-    grid) bit-identical to the single-RHS MATVEC of each RHS."""
+    """Batched MATVEC with the 200-, 216-, 256-, 512- and 768-point register
+    kernels (passes 3 and 5 by 2 x 256 and 3 x 256 decimation at 512 and
+    768, the RHS index in the grid) bit-identical to the single-RHS MATVEC of
+    each RHS."""
     import torch
     nx, ny, nz = shape
     rng = np.random.default_rng(nx * 7 + ny * 3 + nz)
