@@ -1258,7 +1258,8 @@ struct RegShape {
 // sequence, C sequences per workgroup of R1 C threads).  Shapes keep a
 // thread's register array at <= 16 complex values so the three-component
 // passes (3 and 5) stay free of spills; 200 = 20 x 10 and 216 = 18 x 12 hold
-// 20 and 18 values per thread (153-217 VGPRs in passes 3 and 5, no spills).
+// 20 and 18 values per thread (153-217 VGPRs in passes 3 and 5, no spills;
+// the 10 x 20 pass-3 tile at 200, 191 VGPRs).
 // Lengths without such a split (162, primes of 7 ...) use the LDS-staged
 // kernels.
 // The first shape of a length is its default; further ones are autotuner
@@ -1269,8 +1270,15 @@ static const RegShape kShapes[] = {
     DDH_SHAPE(96, 8, 12, 32),   DDH_SHAPE(100, 10, 10, 32), DDH_SHAPE(108, 12, 9, 16), DDH_SHAPE(120, 12, 10, 16),
     DDH_SHAPE(120, 8, 15, 32),  DDH_SHAPE(128, 8, 16, 32),  DDH_SHAPE(128, 16, 8, 16), DDH_SHAPE(144, 12, 12, 16),
     DDH_SHAPE(144, 16, 9, 16),  DDH_SHAPE(150, 15, 10, 16), DDH_SHAPE(160, 16, 10, 16), DDH_SHAPE(180, 15, 12, 16),
-    DDH_SHAPE(192, 16, 12, 16), DDH_SHAPE(200, 20, 10, 16), DDH_SHAPE(216, 18, 12, 16), DDH_SHAPE(240, 16, 15, 16),
-    DDH_SHAPE(256, 16, 16, 16),
+    DDH_SHAPE(192, 16, 12, 16), DDH_SHAPE(240, 16, 15, 16), DDH_SHAPE(256, 16, 16, 16),
+    // 200 and 216: pass 3 defaults to the fastest tile measured on 200^3 and
+    // 240 x 240 x 216 grids (10 x 20 by 16 columns: 121 us; 18 x 12 by 32
+    // columns: 171 us); the 16-column tiles of 20 x 10 and 18 x 12 ran at
+    // 207 and 282 us, behind the LDS-staged kernel's 175 us at 200
+    RegShape{200, 20, 10, 16, &launch_reg<20, 10, 16>, ALL_PASSES & ~(1 << 3)},
+    RegShape{200, 10, 20, 16, &launch_reg<10, 20, 16>, 1 << 3},
+    RegShape{216, 18, 12, 16, &launch_reg<18, 12, 16>, ALL_PASSES & ~(1 << 3)},
+    RegShape{216, 18, 12, 32, &launch_reg<18, 12, 32>, 1 << 3},
     // 512 and 768 (256^3 and 384^3 targets): 32 values per thread and component;
     // single-component passes in registers, pass 3 split over component groups
     DDH_SHAPE1(512, 32, 16, 8), DDH_SHAPE1(512, 32, 16, 4),
@@ -1293,6 +1301,9 @@ static const RegShape kPairShapes[] = {
     RegShape{256, 16, 16, 16, &launch_ypair<16, 16, 16>, 1 << 3}, RegShape{128, 8, 16, 32, &launch_ypair<8, 16, 32>, 1 << 3},
     RegShape{64, 8, 8, 32, &launch_ypair<8, 8, 32>, 1 << 3},
     RegShape{256, 16, 16, 8, &launch_reg1<16, 16, 8>, 1 << 1},
+    RegShape{216, 18, 12, 8, &launch_reg<18, 12, 8>, 1 << 3}, RegShape{216, 12, 18, 16, &launch_reg<12, 18, 16>, 1 << 3},
+    RegShape{200, 20, 10, 8, &launch_reg<20, 10, 8>, 1 << 3}, RegShape{200, 20, 10, 32, &launch_reg<20, 10, 32>, 1 << 3},
+    RegShape{200, 20, 10, 16, &launch_reg<20, 10, 16>, 1 << 3},
     RegShape{512, 16, 16, 8, &launch_ym<2, 8>, (1 << 2) | (1 << 4)},
     RegShape{512, 16, 16, 16, &launch_ym<2, 16>, (1 << 2) | (1 << 4)},
     RegShape{768, 16, 16, 8, &launch_ym<3, 8>, (1 << 2) | (1 << 4)},
