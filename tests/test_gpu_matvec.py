@@ -291,7 +291,8 @@ def test_nearfield_matches_oracle_on_all_sites(lib):
 
 @pytest.mark.parametrize("dims", [(128, 8, 6), (6, 128, 4), (4, 6, 128), (32, 32, 32), (72, 6, 72), (20, 30, 40),
                                   (64, 6, 10), (384, 6, 4), (6, 384, 4), (4, 6, 384), (256, 4, 6), (6, 256, 4), (4, 6, 256),
-                                  (100, 6, 4), (4, 100, 6), (6, 4, 100), (108, 4, 6), (6, 108, 4), (4, 6, 108)])
+                                  (100, 6, 4), (4, 100, 6), (6, 4, 100), (108, 4, 6), (6, 108, 4), (4, 6, 108),
+                                  (6, 4, 48), (4, 6, 60), (6, 4, 75), (4, 6, 80), (6, 4, 90), (4, 6, 96), (6, 4, 120)])
 def test_every_kernel_candidate_matches_oracle(lib, dims):
     """Each FFT kernel the autotuner may pick (every register shape of the
     pass's axis length, the lane-pair pass 3, and the LDS-staged kernel),
