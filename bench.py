@@ -203,17 +203,6 @@ def batched_small_target(lib, n_orient=16, method="pbcgs2", tol=1e-5, maxit=300,
                 qext_mean=float(np.mean(qb)))
 
 
-def random_rotations(rng, n):
-    """n rotation matrices uniform on SO(3) (unit quaternions)."""
-    import numpy as np
-    q = rng.normal(size=(n, 4))
-    q /= np.linalg.norm(q, axis=1, keepdims=True)
-    w, x, y, z = q.T
-    return np.stack([np.stack([1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)], -1),
-                     np.stack([2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)], -1),
-                     np.stack([2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)], -1)], 1)
-
-
 def batched_targets(lib, n_targets=48, n_orient=8, kd=0.1, method="pbcgs2", tol=1e-5, maxit=500,
                     single_sample=3, seed=20261202):
     """Uneven batch: n_targets seeded unions of 1-4 ellipsoids (semi-axes
@@ -232,7 +221,7 @@ def batched_targets(lib, n_targets=48, n_orient=8, kd=0.1, method="pbcgs2", tol=
     for _ in range(n_targets):
         tg = T.ellipsoid_union(rng)
         m = complex(rng.uniform(1.2, 2.0), rng.uniform(0.0, 0.3))
-        tgts.append((tg, m, random_rotations(rng, n_orient)))
+        tgts.append((tg, m, T.random_rotations(rng, n_orient)))
     rhs_total, iters, sizes = 0, [], []
     stage = dict(prepare=0.0, setup=0.0, solve=0.0, evalq=0.0)
     t_batch_sample, t_single, rhs_single = 0.0, 0.0, 0
@@ -449,7 +438,7 @@ def cpu_small_target(cores, budget_s=5.0):
     khat = np.array([1.0, 0.0, 0.0])
     e0 = np.array([0, 1, 0], complex)
     adia, aoff = T.operators(tg, 1.33 + 0.01j, kd, khat, e0)
-    goct = _green_octant_fast(tg.nx, tg.ny, tg.nz, float(np.float32(kd)))
+    goct = O.green_octant_separable(tg.nx, tg.ny, tg.nz, float(np.float32(kd)))
     mv = O.CpuMatvecFP32(goct, tg.nx, tg.ny, tg.nz, workers=cores)
     occ3 = np.flatnonzero(np.tile(tg.iocc != 0, 3))
     full = np.zeros(3 * tg.nat, np.complex64)
@@ -482,7 +471,7 @@ def cpu_baseline(tg, kd, host, budget_s=20.0):
     x, adia, aoff = host
     xf = np.zeros(3 * tg.nat, np.complex64)
     xf.reshape(3, -1)[:, tg.occ] = x.reshape(3, -1)
-    goct = _green_octant_fast(tg.nx, tg.ny, tg.nz, float(np.float32(kd)))
+    goct = O.green_octant_separable(tg.nx, tg.ny, tg.nz, float(np.float32(kd)))
     # the GPU box's host share is 16 cores whatever os.cpu_count() reports
     cores = max(1, min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "16") or 16), 16))
     mv = O.CpuMatvecFP32(goct, tg.nx, tg.ny, tg.nz, workers=cores)
@@ -500,31 +489,6 @@ def cpu_baseline(tg, kd, host, budget_s=20.0):
                        f"workers={cores} on the full padded grid, Green expanded once) in {dt:.1f} s",
                 cpu_model=_cpu_model(), nproc=os.cpu_count(), fft_backend="scipy.fft (pocketfft), complex64",
                 sphere32=cpu_small_target(cores))
-
-
-def _green_octant_fast(nx, ny, nz, akd):
-    """complex128 spectral octant via separable even/odd transforms (same
-    result as oracle.green_octant, (N+1)^3 closed-form evaluations)."""
-    import numpy as np
-    from oracle import ddscat_oracle as O
-    ls = [np.arange(n + 1) for n in (nx, ny, nz)]
-    Z, Y, X = np.meshgrid(ls[2], ls[1], ls[0], indexing="ij")
-    g = O.green_elements(X * 1.0, Y * 1.0, Z * 1.0, akd)
-    g[:, nz] = 0; g[:, :, ny] = 0; g[:, :, :, nx] = 0
-
-    def tmat(N, odd):
-        k = np.arange(N + 1)[:, None]; l = np.arange(N + 1)[None, :]
-        M = 2 * (np.sin if odd else np.cos)(np.pi * k * l / N)
-        M[:, 0] = 0 if odd else 1
-        M[:, N] = 0
-        return M
-    out = np.empty_like(g)
-    for m, par in enumerate(O.COMP_PARITY):
-        a = np.einsum("kl,zyl->zyk", tmat(nx, par[0]), g[m], optimize=True)
-        a = np.einsum("kl,zlx->zkx", tmat(ny, par[1]), a, optimize=True)
-        a = np.einsum("kl,lyx->kyx", tmat(nz, par[2]), a, optimize=True)
-        out[m] = (-1 if m in (1, 2, 4) else 1) * a
-    return out
 
 
 TRAFFIC_FILE = ROOT / "profiles" / "matvec_fft3d_block128_traffic.json"

@@ -225,6 +225,22 @@ extern "C" DDSCAT_HIP_API int ddscat_hip_info(long long* out, int n) {
   return 0;
 }
 
+extern "C" DDSCAT_HIP_API int ddscat_hip_solve_log(int rhs, int* restarts, double* resid, int n, double* checks,
+                                                   int nchecks, int* nchecks_out) {
+  const SolveLog& L = ctx().log;
+  if (rhs < 0 || rhs >= (int)L.resid.size()) return -1;
+  const std::vector<double>& h = L.resid[rhs];
+  const std::vector<double>& k = L.checks[rhs];
+  if (restarts) *restarts = L.restarts[rhs];
+  if (resid)
+    for (int i = 0; i < n && i < (int)h.size(); ++i) resid[i] = h[i];
+  const int nk = (int)(k.size() / 4);
+  if (checks)
+    for (int i = 0; i < 4 * nchecks && i < 4 * nk; ++i) checks[i] = k[i];
+  if (nchecks_out) *nchecks_out = nk;
+  return (int)h.size();
+}
+
 extern "C" DDSCAT_HIP_API int ddscat_hip_force_restart(void) {
   ctx().forced_gap = 1.0;
   return 0;

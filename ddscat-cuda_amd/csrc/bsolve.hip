@@ -746,6 +746,7 @@ int bsolve_qmrccg(Context& c, BatchState& S, const float2* b, float2* x, const f
   const char* nm = "QMRCCG batch";
   Ops o(c, S, adia, aoff, os);
   const int B = S.B;
+  c.log.reset(B);
   enum { R_ = 0, P_, AP_, Q_, D_, SV_, VT_, WT_, ATW_, X_, SY_ };
   float2 *r = o.V(R_), *p = o.V(P_), *ap = o.V(AP_), *q = o.V(Q_), *d = o.V(D_), *sv = o.V(SV_), *vt = o.V(VT_),
          *wt = o.V(WT_), *atw = o.V(ATW_), *xs = o.V(X_), *sy = o.V(SY_);
@@ -814,6 +815,7 @@ int bsolve_qmrccg(Context& c, BatchState& S, const float2* b, float2* x, const f
       const double gap = tr > 0.0 ? gp / tr : (gp == 0.0 ? 0.0 : INFINITY);
       conv[k] = tres[k] <= tol;
       rst[k] = !conv[k] && (gap >= RELIABLE_GAP || resid[k] <= tol);
+      c.log.check(k, tres[k], gap, rst[k] && it[k] < maxit);
     }
     return 0;
   };
@@ -855,6 +857,7 @@ int bsolve_qmrccg(Context& c, BatchState& S, const float2* b, float2* x, const f
         continue;
       }
       resid[k] = S.hpacked[3 * k + 1];
+      c.log.iter(k, resid[k]);
       if ((it[k] % RELIABLE_PERIOD) == 0 || resid[k] <= tol || it[k] >= maxit) sel[k] = 1;
     }
     if (any_set(sel)) {
@@ -910,6 +913,7 @@ int bsolve_pbcgs2(Context& c, BatchState& S, const float2* b, float2* x, const f
   const char* nm = "PBCGS2 batch";
   Ops o(c, S, adia, aoff, os);
   const int B = S.B;
+  c.log.reset(B);
   // work layout per RHS as the single-RHS solver (cu:1173-1174) plus A x scratch
   enum { RR_ = 0, R0_, R1_, R2_, U0_, U1_, U2_, XP_, BP_, X_, SY_ };
   float2 *rr = o.V(RR_), *r0 = o.V(R0_), *r1 = o.V(R1_), *r2 = o.V(R2_), *u0 = o.V(U0_), *u1 = o.V(U1_),
@@ -1003,6 +1007,7 @@ int bsolve_pbcgs2(Context& c, BatchState& S, const float2* b, float2* x, const f
         continue;
       }
       resid[k] = S.hpacked[3 * k + 1];
+      c.log.iter(k, resid[k]);
       if ((it[k] % RELIABLE_PERIOD) == 0 || resid[k] <= tol || it[k] >= maxit) sel[k] = 1;
     }
     if (!any_set(sel)) {
@@ -1028,6 +1033,7 @@ int bsolve_pbcgs2(Context& c, BatchState& S, const float2* b, float2* x, const f
       } else if (gap >= RELIABLE_GAP || resid[k] <= tol) {
         restart[k] = 1;
       }
+      c.log.check(k, tres, gap, restart[k] != 0);
       resid[k] = tres;
     }
     if (any_set(restart)) {

@@ -8,10 +8,18 @@
 // are Fortran-side inputs of the ABI (EVALE, ALPHADIAG / LATTDR); here they
 // are built where the batched solver needs them.  Phases are FP64.
 //
-// Lattice dispersion relation (Draine & Goodman 1993, ApJ 405, 685):
-//   1/alpha = 1/alpha_CM + (b1 + b2 eps + b3 S eps) (kd)^2 - (2/3) i (kd)^3,
-//   alpha_CM = (3/4pi)(eps - 1)/(eps + 2),  S = sum_j (khat_j ehat_j)^2,
+// Lattice dispersion relation, DDSCAT's two LDR prescriptions (CALPHA,
+// ddscat_cuda_unified/fortran/reapar_cuda.f90:497-513):
+//   LATTDR (Draine & Goodman 1993, ApJ 405, 685), isotropic:
+//     1/alpha = 1/alpha_CM + (b1 + b2 eps + b3 S eps) (kd)^2 - (2/3) i (kd)^3,
+//     S = sum_j (khat_j ehat_j)^2;
+//   GKDLDR (Gutkowicz-Krusin & Draine 2004, arXiv:astro-ph/0403082), a
+//     diagonal tensor whose jj element takes S_j = khat_j^2 (direction of
+//     propagation only) in place of S;
+//   alpha_CM = (3/4pi)(eps - 1)/(eps + 2),
 //   b1 = -1.8915316, b2 = 0.1648469, b3 = -1.7700004.
+// For a real unit polarization ehat, sum_j ehat_j^2 (1/alpha_jj) of GKDLDR
+// equals the LATTDR 1/alpha, since sum_j ehat_j^2 khat_j^2 = S.
 #include "solver_common.hpp"
 #include "../../include/ddscat_matvec_hip.h"
 #include <cmath>
@@ -25,9 +33,11 @@ constexpr double LDR_B1 = -1.8915316, LDR_B2 = 0.1648469, LDR_B3 = -1.7700004;
 struct RhsParams {
   double k[3];      // k d
   double e0[6];     // complex e0 (re, im) x 3
-  double S;         // sum (khat_j ehat_j)^2
+  double S[3];      // S of component j: LATTDR sum (khat_i ehat_i)^2 for all j, GKDLDR khat_j^2
   double kd;
 };
+
+int g_prescription = DDSCAT_HIP_LATTDR;
 
 __global__ __launch_bounds__(256) void k_setup_rhs(const RhsParams* __restrict__ prm, const int* __restrict__ occ,
                                                    int nx, int ny, int nat0, double x00, double x01, double x02,
@@ -55,7 +65,7 @@ __global__ __launch_bounds__(256) void k_setup_rhs(const RhsParams* __restrict__
       const double2 inv_cm = zscale(zdiv(make_double2(e.x + 2.0, e.y), make_double2(e.x - 1.0, e.y)),
                                     4.0 * M_PI / 3.0);
       const double kd2 = P.kd * P.kd;
-      const double f = LDR_B2 + LDR_B3 * P.S;
+      const double f = LDR_B2 + LDR_B3 * P.S[c];
       const double2 corr = make_double2((LDR_B1 + f * e.x) * kd2, f * e.y * kd2 - (2.0 / 3.0) * kd2 * P.kd);
       const double2 ai = zadd(inv_cm, corr);
       adia[off + (size_t)c * nat0 + q] = make_float2((float)ai.x, (float)ai.y);
@@ -68,6 +78,15 @@ __global__ __launch_bounds__(256) void k_setup_rhs(const RhsParams* __restrict__
 }  // namespace ddhip
 
 using namespace ddhip;
+
+extern "C" DDSCAT_HIP_API int ddscat_hip_set_polarizability(int prescription) {
+  if (prescription != DDSCAT_HIP_LATTDR && prescription != DDSCAT_HIP_GKDLDR) {
+    set_error("set_polarizability", "prescription must be DDSCAT_HIP_LATTDR or DDSCAT_HIP_GKDLDR");
+    return 1;
+  }
+  g_prescription = prescription;
+  return 0;
+}
 
 extern "C" DDSCAT_HIP_API int ddscat_hip_setup_rhs(int nrhs, const float* ak, const float* e0, const float* x0,
                                                    const float* dx, const void* eps, int eps_per_comp, void* b,
@@ -98,10 +117,14 @@ extern "C" DDSCAT_HIP_API int ddscat_hip_setup_rhs(int nrhs, const float* ak, co
     }
     en = std::sqrt(en);
     if (!(en > 0.0)) { set_error("setup_rhs", "zero polarization vector"); return 1; }
-    P.S = 0.0;
+    double S = 0.0;
     for (int i = 0; i < 3; ++i) {
       const double t = P.k[i] / P.kd * ea[i] / en;
-      P.S += t * t;
+      S += t * t;
+    }
+    for (int i = 0; i < 3; ++i) {
+      const double a = P.k[i] / P.kd;
+      P.S[i] = g_prescription == DDSCAT_HIP_GKDLDR ? a * a : S;
     }
   }
   hipStream_t st = c.stream;

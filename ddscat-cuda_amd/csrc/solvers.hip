@@ -33,6 +33,7 @@ int solver_begin(Context& c, const SolverArgs& a, int code_arg, int code_io, con
   if (int rc = ensure_solver_workspace(c, 13)) return rc;
   c.solution_valid = false;
   c.restarts = 0;
+  c.log.reset(1);
   const size_t fb = (size_t)3 * c.nat * sizeof(float2);
   hipStream_t st = c.stream;
   if (a.device) {
@@ -96,6 +97,7 @@ int reliable_check(Context& c, const float2* rhs, const float2* xcur, const floa
   rr.converged = rr.true_res <= tol;
   rr.restart = rr.gap >= RELIABLE_GAP || (rec_res <= tol && !rr.converged);
   if (rr.restart) ++c.restarts;
+  c.log.check(0, rr.true_res, rr.gap, rr.restart);
   return 0;
 }
 
@@ -200,6 +202,7 @@ int solve_pbcgs2(Context& c, const SolverArgs& a) {
       return status;
     }
     resid = c.hscal[SL_RESID];
+    c.log.iter(0, resid);
     const int flags = (int)c.hscal[SL_FLAGS];
     const bool rcmp = flags & 1, xpdt = (flags & 2) != 0;
     if ((it % RELIABLE_PERIOD) == 0 || resid <= a.tol) {
@@ -321,6 +324,7 @@ int solve_qmrccg(Context& c, const SolverArgs& a) {
       return 58;
     }
     resid = std::sqrt(std::max(0.0, c.hscal[SL_D + 8])) / bnorm;
+    c.log.iter(0, resid);
     if ((it % RELIABLE_PERIOD) == 0 || resid <= a.tol) {
       Reliable rr;
       if (int rc = reliable_check(c, c.sb, xs, r, bnorm, resid, a.tol, nc, rr)) return rc > 1 ? rc : 59;

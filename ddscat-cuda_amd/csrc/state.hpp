@@ -4,6 +4,7 @@
 // the library keeps one default Context for it.
 #pragma once
 #include "common.hpp"
+#include <vector>
 
 namespace ddhip {
 
@@ -24,6 +25,27 @@ struct MatvecPlan {
   // axis length, or -1 for the LDS-staged kernel (chosen by the autotuner)
   int kind[6] = {-1, -1, -1, -1, -1, -1};
   int tuned = 0;
+};
+
+// Convergence record of the last solve (batched: one entry per RHS;
+// single-RHS solvers: RHS 0), read by ddscat_hip_solve_log.
+struct SolveLog {
+  std::vector<std::vector<double>> resid;   // recursive residual after each iteration
+  std::vector<std::vector<double>> checks;  // (iteration, true residual, gap, restarted) per reliable check
+  std::vector<int> restarts;
+  void reset(int n) {
+    resid.assign(n, {});
+    checks.assign(n, {});
+    restarts.assign(n, 0);
+  }
+  void iter(int r, double v) {
+    if (r >= 0 && r < (int)resid.size()) resid[r].push_back(v);
+  }
+  void check(int r, double true_res, double gap, bool restarted) {
+    if (r < 0 || r >= (int)resid.size()) return;
+    checks[r].insert(checks[r].end(), {(double)resid[r].size(), true_res, gap, restarted ? 1.0 : 0.0});
+    if (restarted) ++restarts[r];
+  }
 };
 
 struct Context {
@@ -61,6 +83,7 @@ struct Context {
   bool solution_valid = false;
   unsigned long long matvec_count = 0;
   int restarts = 0;               // reliable-residual restarts of the last single-RHS solve
+  SolveLog log;                   // convergence record of the last solve
   double forced_gap = 0.0;        // added once to the next reliable check's gap (test hook)
   double green_build_ms = 0.0;
 };

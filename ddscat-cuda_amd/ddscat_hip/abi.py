@@ -32,6 +32,7 @@ EXTENSION_SYMBOLS = [
     "ddscat_hip_solve_batch", "ddscat_hip_apply_batch", "ddscat_hip_evalq_batch",
     "ddscat_hip_cabs_adjoint", "ddscat_hip_setup_rhs", "ddscat_hip_set_autotune",
     "ddscat_hip_nearfield", "ddscat_hip_set_pass_kernel", "ddscat_hip_scat_matrix", "ddscat_hip_force_restart",
+    "ddscat_hip_solve_log", "ddscat_hip_set_polarizability",
 ]
 # method ids of ddscat_hip_solve_device
 SOLVER_IDS = {"pbcgs2": 0, "qmrccg": 1, "petrkp": 2, "gpbicg": 3, "pbcgst": 4, "bicgstab4": 5,
@@ -141,6 +142,10 @@ class Library:
         L.ddscat_hip_scat_matrix.restype = i
         L.ddscat_hip_force_restart.argtypes = []
         L.ddscat_hip_force_restart.restype = i
+        L.ddscat_hip_solve_log.argtypes = [i, vp, vp, i, vp, i, vp]
+        L.ddscat_hip_solve_log.restype = i
+        L.ddscat_hip_set_polarizability.argtypes = [i]
+        L.ddscat_hip_set_polarizability.restype = i
         L.ddscat_hip_set_pass_kernel.argtypes = [i, i]
         L.ddscat_hip_set_pass_kernel.restype = i
         self.nat = self.nat0 = 0
@@ -363,6 +368,25 @@ class Library:
     def force_restart(self):
         """The next reliable-residual check of a single-RHS solve restarts (test hook)."""
         return self._check("force_restart", self.lib.ddscat_hip_force_restart())
+
+    def solve_log(self, rhs=0) -> dict:
+        """Convergence record of RHS `rhs` of the last solve: restarts, the
+        recursive residual after each iteration, and the reliable checks as
+        rows (iteration, true residual, gap, restarted)."""
+        nrst = C.c_int(0)
+        nck = C.c_int(0)
+        n = self.lib.ddscat_hip_solve_log(int(rhs), C.byref(nrst), None, 0, None, 0, C.byref(nck))
+        if n < 0:
+            raise IndexError(f"no RHS {rhs} in the last solve")
+        resid = np.zeros(n, np.float64)
+        checks = np.zeros((nck.value, 4), np.float64)
+        self.lib.ddscat_hip_solve_log(int(rhs), C.byref(nrst), _p(resid), n, _p(checks), nck.value, C.byref(nck))
+        return dict(restarts=nrst.value, resid=resid, checks=checks)
+
+    def set_polarizability(self, prescription):
+        """LDR prescription of setup_rhs: "LATTDR" (default) or "GKDLDR"."""
+        code = {"LATTDR": 0, "GKDLDR": 1}.get(prescription, -1)
+        return self._check("set_polarizability", self.lib.ddscat_hip_set_polarizability(code))
 
     def set_autotune(self, on):
         return self._check("set_autotune", self.lib.ddscat_hip_set_autotune(1 if on else 0))

@@ -9,9 +9,11 @@ same inputs the driver would hand to ``ddscat_cuda_solve_*_f32``:
 * targets: sphere (sites with |r - c| <= D/2, c the lattice centre), block,
   seeded unions of ellipsoids;
 * polarizability: lattice-dispersion relation of Draine & Goodman (1993,
-  ApJ 405, 685), alpha = a_CM / (1 + (a_CM/d^3)[(b1 + b2 m^2 + b3 m^2 S)(kd)^2
-  - (2/3) i (kd)^3]), S = sum_j (a_j e_j)^2, b1=-1.8915316, b2=0.1648469,
-  b3=-1.7700004; CXADIA = 1/alpha, CXAOFF = 0 for isotropic material;
+  ApJ 405, 685; DDSCAT's LATTDR), alpha = a_CM / (1 + (a_CM/d^3)[(b1 + b2 m^2
+  + b3 m^2 S)(kd)^2 - (2/3) i (kd)^3]), S = sum_j (a_j e_j)^2, b1=-1.8915316,
+  b2=0.1648469, b3=-1.7700004; or the diagonal tensor of Gutkowicz-Krusin &
+  Draine (2004; GKDLDR), whose jj element takes S_j = a_j^2; CXADIA = 1/alpha,
+  CXAOFF = 0 for isotropic material;
 * incident field: E_j = E0 exp(i k.r_j) at r_j = ((ix+1)+x0_1)dx_1, ...,
   the same site coordinates the far-field sums use
   (ddscat_cuda_unified/cuda/ddscat_matvec_cuda.cu:800-802).
@@ -96,10 +98,15 @@ def fft_friendly(n: int) -> int:
         m += 1
 
 
-def ellipsoid_union(rng: np.random.Generator, n_min=1, n_max=4, semi_lo=6.0, semi_hi=40.0) -> Target:
+def ellipsoid_union(rng: np.random.Generator, n_min=1, n_max=4, semi_lo=6.0, semi_hi=40.0,
+                    grid_lo=24, grid_hi=96) -> Target:
     """Union of 1..4 randomly oriented ellipsoids with semi-axes U[6,40]
-    (batched-solve workload generator).  The bounding box is padded to
-    2,3,5-smooth sizes."""
+    (batched-solve workload generator).  The first ellipsoid sits at the
+    centre; each further one is offset by up to half the first one's largest
+    semi-axis per axis, clipped so that it stays inside a grid_hi-point box.
+    Each axis of the bounding grid is at least grid_lo points and is extended
+    to a 2,3,5-smooth size, so grids run from 24 to 96 points per axis."""
+    half_max = grid_hi // 2 - 2
     k = int(rng.integers(n_min, n_max + 1))
     ells = []
     for e in range(k):
@@ -110,11 +117,13 @@ def ellipsoid_union(rng: np.random.Generator, n_min=1, n_max=4, semi_lo=6.0, sem
         R = np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
                       [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
                       [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
-        off = rng.uniform(-0.5, 0.5, size=3) * (ax.max() if e else 0.0)
+        off = rng.uniform(-0.5, 0.5, size=3) * (ells[0][0].max() if e else 0.0)
+        lim = max(0.0, half_max - ax.max())
+        off = np.clip(off, -lim, lim)
         ells.append((ax, R, off))
     ext = np.max([np.abs(o) + a.max() for a, R, o in ells], axis=0)
     half = np.ceil(ext).astype(int) + 1
-    dims = [fft_friendly(int(2 * h)) for h in half]
+    dims = [fft_friendly(max(grid_lo, int(2 * h))) for h in half]
     nx, ny, nz = dims
     iz, iy, ix = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
     pts = np.stack([ix - (nx - 1) / 2.0, iy - (ny - 1) / 2.0, iz - (nz - 1) / 2.0], axis=-1)
@@ -126,23 +135,51 @@ def ellipsoid_union(rng: np.random.Generator, n_min=1, n_max=4, semi_lo=6.0, sem
                   meta={"n_ellipsoids": k})
 
 
+def random_rotations(rng: np.random.Generator, n: int) -> np.ndarray:
+    """n rotation matrices [n, 3, 3] uniform on SO(3) (normalised Gaussian
+    quaternions).  Column 0 is the incidence direction in the target frame,
+    columns 1 and 2 the two polarizations."""
+    q = rng.normal(size=(n, 4))
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    w, x, y, z = q.T
+    return np.stack([np.stack([1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)], -1),
+                     np.stack([2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)], -1),
+                     np.stack([2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)], -1)], 1)
+
+
 def kd_for_size_parameter(target: Target, x: float) -> float:
     """k d for size parameter x = k a_eff."""
     return x / target.a_eff()
 
 
-def ldr_alpha(eps, kd: float, khat, ehat) -> np.ndarray:
-    """Lattice-dispersion-relation polarizability (units d^3)."""
+def ldr_s(khat, ehat, prescription="LATTDR") -> np.ndarray:
+    """S of each component [3]: LATTDR sum_j (khat_j |ehat_j|)^2 (|ehat| = 1)
+    for all three, GKDLDR khat_j^2."""
+    khat = np.asarray(khat, float)
+    khat = khat / np.linalg.norm(khat)
+    if prescription == "GKDLDR":
+        return khat ** 2
+    if prescription != "LATTDR":
+        raise ValueError(f"unknown polarizability prescription {prescription!r}")
+    ea = np.abs(np.asarray(ehat, complex))
+    S = float(np.sum((khat * ea / max(np.linalg.norm(ea), 1e-300)) ** 2))
+    return np.full(3, S)
+
+
+def ldr_alpha(eps, kd: float, khat, ehat, prescription="LATTDR") -> np.ndarray:
+    """Lattice-dispersion-relation polarizability (units d^3): shape of eps
+    for LATTDR (isotropic), [3, *eps.shape] (diagonal elements) for GKDLDR."""
     eps = np.asarray(eps, np.complex128)
     a_cm = (3.0 / (4.0 * math.pi)) * (eps - 1.0) / (eps + 2.0)
-    khat = np.asarray(khat, float); khat = khat / np.linalg.norm(khat)
-    ehat = np.asarray(ehat, complex)
-    S = float(np.sum((khat * np.abs(ehat) / max(np.linalg.norm(np.abs(ehat)), 1e-300)) ** 2))
-    corr = (LDR_B1 + LDR_B2 * eps + LDR_B3 * S * eps) * kd ** 2 - (2.0 / 3.0) * 1j * kd ** 3
-    return a_cm / (1.0 + a_cm * corr)
+    S = ldr_s(khat, ehat, prescription)
+    out = []
+    for Sj in (S[:1] if prescription == "LATTDR" else S):
+        corr = (LDR_B1 + LDR_B2 * eps + LDR_B3 * Sj * eps) * kd ** 2 - (2.0 / 3.0) * 1j * kd ** 3
+        out.append(a_cm / (1.0 + a_cm * corr))
+    return out[0] if prescription == "LATTDR" else np.stack(out)
 
 
-def operators(target: Target, m_or_eps, kd: float, khat, ehat, use_eps=False):
+def operators(target: Target, m_or_eps, kd: float, khat, ehat, use_eps=False, prescription="LATTDR"):
     """Full-layout CXADIA, CXAOFF [3*NAT] complex64 (zero on void sites)."""
     nat = target.nat
     occ = target.occ
@@ -152,7 +189,7 @@ def operators(target: Target, m_or_eps, kd: float, khat, ehat, use_eps=False):
         eps = np.full(occ.size, complex(m_or_eps))
     else:
         eps = np.full(occ.size, complex(m_or_eps) ** 2)
-    alpha = ldr_alpha(eps, kd, khat, ehat)
+    alpha = ldr_alpha(eps, kd, khat, ehat, prescription)
     adia = np.zeros((3, nat), dtype=np.complex128)
     adia[:, occ] = 1.0 / alpha
     aoff = np.zeros((3, nat), dtype=np.complex128)

@@ -195,12 +195,22 @@ DDSCAT_HIP_API int ddscat_hip_cabs_adjoint(int method, int nrhs, const void *p, 
                                            int maxit, float tol, void *g_adia, void *g_aoff, int *itno,
                                            float *tole);
 
+/* LDR polarizability prescription used by ddscat_hip_setup_rhs (DDSCAT's
+ * CALPHA, ddscat_cuda_unified/fortran/reapar_cuda.f90:497-513): LATTDR
+ * (Draine & Goodman 1993, default) or GKDLDR (Gutkowicz-Krusin & Draine
+ * 2004: a diagonal tensor, component j with S_j = khat_j^2).  FLTRCD's
+ * filtered interaction is IDIPINT=1 of ddscat_cuda_prepare_green_f32.
+ * Code 1: unknown prescription. */
+enum { DDSCAT_HIP_LATTDR = 0, DDSCAT_HIP_GKDLDR = 1 };
+DDSCAT_HIP_API int ddscat_hip_set_polarizability(int prescription);
+
 /* Per-RHS inputs of a batched solve built on the DEVICE for the prepared
  * target (what DDSCAT's EVALE and ALPHADIAG/LATTDR produce on the host):
  * for r < nrhs with k d = ak[3r..3r+2] and polarization e0[6r..6r+5]
  * (3 interleaved complex64),
  *   b[r]    = e0 exp(i k.x_q), x_q = ((ix+1) + x0) dx per axis (FP64 phase),
- *   adia[r] = 1/alpha of the lattice dispersion relation for eps(q),
+ *   adia[r] = 1/alpha of the lattice dispersion relation for eps(q)
+ *             (per component; see ddscat_hip_set_polarizability),
  *   aoff[r] = 0,
  * all compact [nrhs][3][NAT0] DEVICE arrays.  eps: complex64 DEVICE array,
  * [NAT0] (isotropic, eps_per_comp = 0) or [3][NAT0] (eps_per_comp = 1).
@@ -244,6 +254,17 @@ DDSCAT_HIP_API int ddscat_hip_scat_matrix(int npair, const void *p, int ndir, co
  * 1e-3 restart threshold) and restarts from the true residual.  The restart
  * count of the last single-RHS solve is ddscat_hip_info field 23. */
 DDSCAT_HIP_API int ddscat_hip_force_restart(void);
+
+/* Convergence record of the last solve: RHS `rhs` of the last
+ * ddscat_hip_solve_batch / ddscat_hip_cabs_adjoint, or rhs 0 of the last
+ * single-RHS solve.  *restarts = reliable-residual restarts (cu:543-556
+ * policy); resid[0..n-1] = the recursive residual after each iteration (as
+ * many as fit; PBCGS2 and QMRCCG); checks[4*i..4*i+3] = (iteration, true
+ * residual, gap, restarted) of reliable check i (up to nchecks of them);
+ * *nchecks_out = the number of checks.  Any output pointer may be NULL.
+ * Returns the number of iterations recorded, -1 if there is no such RHS. */
+DDSCAT_HIP_API int ddscat_hip_solve_log(int rhs, int *restarts, double *resid, int n, double *checks,
+                                        int nchecks, int *nchecks_out);
 
 /* Copy the resident spectral Green tensor (ddscat_hip_info field 5 complex64
  * values) to host memory in the CXZC layout ddscat_cuda_prepare_f32 takes. */

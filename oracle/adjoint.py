@@ -50,21 +50,35 @@ def cabs_dense(Ao, e_c, adia_c, aoff_c, akk, e02=1.0):
     return cabs_of(p, adia_c, aoff_c, akk, e02), p
 
 
-def cabs_adjoint(Ao, p_c, adia_c, aoff_c, akk, e02=1.0):
-    """g_adia, g_aoff [3][NAT0] with dCabs = sum Re(g . d entry), for the
-    solution p of Ao p = E (Ao the dense compact operator)."""
+def adjoint_rhs(p_c, adia_c, aoff_c, akk, e02=1.0):
+    """W conj(P) [3*NAT0]: the right-hand side of the adjoint solve A mu = W conj(P)."""
     n0 = np.asarray(p_c).size // 3
     p = np.asarray(p_c, np.complex128).reshape(3, n0)
     cf = 4.0 * PI * akk / e02
-    M = site_matrices(adia_c, aoff_c, akk ** 3 / 1.5)
-    W = -cf * M.imag  # real symmetric per site
-    rhs = np.einsum("nij,nj->ni", W, np.conj(p.T)).T.reshape(-1)
-    mu = np.linalg.solve(Ao, rhs).reshape(3, n0)
+    W = -cf * site_matrices(adia_c, aoff_c, akk ** 3 / 1.5).imag  # real symmetric per site
+    return np.einsum("nij,nj->ni", W, np.conj(p.T)).T.reshape(-1)
+
+
+def gradient_terms(p_c, mu_c, akk, e02=1.0):
+    """g_adia, g_aoff [3*NAT0] from the solution p and the adjoint solution mu."""
+    n0 = np.asarray(p_c).size // 3
+    p = np.asarray(p_c, np.complex128).reshape(3, n0)
+    mu = np.asarray(mu_c, np.complex128).reshape(3, n0)
+    cf = 4.0 * PI * akk / e02
     g_adia = 1j * cf * np.abs(p) ** 2 - 2.0 * mu * p
     g_aoff = np.empty_like(g_adia)
     for s, (a, b) in enumerate(((1, 2), (2, 0), (0, 1))):  # a23, a31, a12
         g_aoff[s] = 1j * cf * 2.0 * np.real(np.conj(p[a]) * p[b]) - 2.0 * (mu[a] * p[b] + mu[b] * p[a])
     return g_adia.reshape(-1), g_aoff.reshape(-1)
+
+
+def cabs_adjoint(Ao, p_c, adia_c, aoff_c, akk, e02=1.0):
+    """g_adia, g_aoff [3][NAT0] with dCabs = sum Re(g . d entry), for the
+    solution p of Ao p = E.  Ao: the dense compact operator, or a callable
+    solving Ao mu = rhs (the iterative oracle solve on large grids)."""
+    rhs = adjoint_rhs(p_c, adia_c, aoff_c, akk, e02)
+    mu = Ao(rhs) if callable(Ao) else np.linalg.solve(Ao, rhs)
+    return gradient_terms(p_c, mu, akk, e02)
 
 
 def compact_dense_operator(dense_operator, nx, ny, nz, akd, adia_c, aoff_c, iocc):

@@ -447,6 +447,115 @@ class CpuMatvecFP32:
         return y.reshape(-1)
 
 
+def green_octant_separable(nx, ny, nz, akd, dx=(1.0, 1.0, 1.0), idipint=0) -> np.ndarray:
+    """The trimmed spectral octant of green_octant (cu:647-652) from (N+1)^3
+    closed-form evaluations: each component has a definite parity per axis
+    (COMP_PARITY), so the +i transform of the lag grid restricted to the
+    octant is a cos sum (even) or i times a sin sum (odd) per axis.  Used for
+    the large-grid oracle runs, where the full FP64 lag grid and its 3-D FFT
+    would be slow; equal to green_octant to 1e-13 (test_oracle_green.py)."""
+    ls = [np.arange(n + 1) for n in (nx, ny, nz)]
+    Z, Y, X = np.meshgrid(ls[2] * dx[2], ls[1] * dx[1], ls[0] * dx[0], indexing="ij")
+    g = green_elements(X, Y, Z, akd, idipint)
+    g[:, nz] = 0
+    g[:, :, ny] = 0
+    g[:, :, :, nx] = 0
+
+    def tmat(n, odd):
+        k = np.arange(n + 1)[:, None]
+        l = np.arange(n + 1)[None, :]
+        m = 2 * (np.sin if odd else np.cos)(np.pi * k * l / n)
+        m[:, 0] = 0 if odd else 1
+        m[:, n] = 0
+        return m
+
+    out = np.empty_like(g)
+    for c, par in enumerate(COMP_PARITY):
+        a = np.einsum("kl,zyl->zyk", tmat(nx, par[0]), g[c], optimize=True)
+        a = np.einsum("kl,zlx->zkx", tmat(ny, par[1]), a, optimize=True)
+        a = np.einsum("kl,lyx->kyx", tmat(nz, par[2]), a, optimize=True)
+        # two odd axes: i * i = -1 (xy, xz, yz)
+        out[c] = -a if sum(par) == 2 else a
+    return out
+
+
+class FftMatvec64:
+    """FP64 FFT MATVEC of matvec_fft on compact [3][NAT0] vectors, for the
+    large-grid oracle runs (128^3 and 64^3 blocks, ellipsoid unions): the same
+    operator (cu:104-118, cp:446-607), with the spectral Green expanded to the
+    full grid once and the zero-padded transforms pruned axis by axis (x on
+    the NZ*NY input rows, y on the NZ input planes, z on everything; the
+    inverse keeps iz < NZ, then iy < NY, then ix < NX).  scipy.fft in
+    complex128 on ``workers`` threads.  Equal to matvec_fft at 1e-13
+    (test_oracle_matvec.py)."""
+
+    def __init__(self, goct, nx, ny, nz, iocc, workers=None, ipbc=0):
+        import scipy.fft as sfft
+        self.sfft = sfft
+        self.n = (nx, ny, nz)
+        self.workers = workers
+        self.occ = occupied_index(iocc)
+        self.gf = np.asarray(goct, np.complex128) if ipbc else green_full_from_octant(goct, nx, ny, nz)
+
+    def full(self, cwhat, x3, adia3, aoff3):
+        """y [3][NAT] from full-layout x, adia, aoff [3][NAT] (no IOCC mask)."""
+        nx, ny, nz = self.n
+        f = self.sfft
+        w = self.workers
+        xs = np.conj(x3) if cwhat == "C" else np.asarray(x3, np.complex128)
+        a = np.zeros((3, nz, ny, 2 * nx), np.complex128)
+        a[..., :nx] = xs.reshape(3, nz, ny, nx)
+        a = f.ifft(a, axis=3, norm="forward", workers=w, overwrite_x=True)
+        b = np.zeros((3, nz, 2 * ny, 2 * nx), np.complex128)
+        b[:, :, :ny] = a
+        del a
+        b = f.ifft(b, axis=2, norm="forward", workers=w, overwrite_x=True)
+        c = np.zeros((3, 2 * nz, 2 * ny, 2 * nx), np.complex128)
+        c[:, :nz] = b
+        del b
+        c = f.ifft(c, axis=1, norm="forward", workers=w, overwrite_x=True)
+        q = self._green_multiply(c)
+        del c
+        q = f.fft(q, axis=1, workers=w, overwrite_x=True)[:, :nz]
+        q = f.fft(q, axis=2, workers=w, overwrite_x=True)[:, :, :ny]
+        q = f.fft(q, axis=3, workers=w, overwrite_x=True)[..., :nx]
+        w3 = q.reshape(3, nx * ny * nz) / float(8 * nx * ny * nz)
+        y = _finish(cwhat, xs, np.asarray(adia3, np.complex128), np.asarray(aoff3, np.complex128), w3, None)
+        return y
+
+    def _green_multiply(self, c):
+        """q = G c per spectral point (symmetric 3x3), in slabs of kz planes
+        spread over ``workers`` threads (numpy releases the GIL)."""
+        from concurrent.futures import ThreadPoolExecutor
+        g = self.gf
+        q = np.empty_like(c)
+        nzz = c.shape[1]
+        step = max(1, nzz // (4 * (self.workers or 1)))
+
+        def slab(z0):
+            z = slice(z0, min(nzz, z0 + step))
+            c0, c1, c2 = c[0, z], c[1, z], c[2, z]
+            for i, (a, b, d) in enumerate(((0, 1, 2), (1, 3, 4), (2, 4, 5))):
+                t = q[i, z]
+                np.multiply(g[a, z], c0, out=t)
+                t += g[b, z] * c1
+                t += g[d, z] * c2
+
+        with ThreadPoolExecutor(max_workers=self.workers or 1) as ex:
+            list(ex.map(slab, range(0, nzz, step)))
+        return q
+
+    def __call__(self, cwhat, xc, adia_c, aoff_c):
+        """y [3*NAT0] = A x on compact vectors with compact operators."""
+        nx, ny, nz = self.n
+        nat = nx * ny * nz
+        occ = self.occ
+        n0 = occ.size
+        full = lambda v: scatter_compact(np.asarray(v, np.complex128), occ, nat).reshape(3, nat)
+        y = self.full(cwhat, full(xc), full(adia_c), full(aoff_c))
+        return np.ascontiguousarray(y[:, occ]).reshape(3 * n0)
+
+
 def dense_green_matrix(nx, ny, nz, akd, dx=(1.0, 1.0, 1.0), idipint=0) -> np.ndarray:
     """Dense interaction matrix Gm[3*NAT, 3*NAT] (component-major ordering),
     Gm[c*NAT+j, d*NAT+l] = c_cd(r_j - r_l), zero on the diagonal blocks j==l.

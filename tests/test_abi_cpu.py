@@ -55,6 +55,8 @@ def test_host_side_error_paths_without_device():
     assert L.evalq((0.1, 0, 0), 1.0, check=False)[3] == 300
     assert L.scat((0.1, 0, 0), (1, 1, 1), (0, 0, 0), (0, 1, 0), 0.5, check=False)["rc"] == 320
     assert L.lib.ddscat_hip_set_backend(7) == 1 and L.lib.ddscat_hip_set_precision(3) == 1
+    assert L.lib.ddscat_hip_set_polarizability(2) == 1 and L.lib.ddscat_hip_set_polarizability(0) == 0
+    assert L.lib.ddscat_hip_solve_log(5, None, None, 0, None, 0, None) == -1
     assert L.info()["nat"] == 0
 
 

@@ -274,9 +274,11 @@ def test_evalq_batch_matches_single_evalq(lib, case):
         assert cext > 0 and cabs > 0
 
 
-def test_setup_rhs_matches_host_operators(lib):
+@pytest.mark.parametrize("prescription", ["LATTDR", "GKDLDR"])
+def test_setup_rhs_matches_host_operators(lib, prescription):
     """Device incident fields and LDR operators equal the host harness's
-    (targets.incident_field / targets.operators) for random incidences."""
+    (targets.incident_field / targets.operators, pinned in test_ldr_cpu.py)
+    for random incidences (S != 0), both LDR prescriptions."""
     import torch
     tg = T.sphere(16)
     kd = T.kd_for_size_parameter(tg, 2.0)
@@ -292,13 +294,18 @@ def test_setup_rhs_matches_host_operators(lib):
     for khat in dirs:
         for e0 in T.polarization_pair(khat):
             aks.append(khat * kd); e0s.append(e0)
-            ad, _ = T.operators(tg, m, kd, khat, e0)
+            ad, _ = T.operators(tg, m, kd, khat, e0, prescription=prescription)
             refs.append((comp(T.incident_field(tg, khat * kd, e0, x0)), comp(ad)))
     B, n = len(aks), 3 * tg.nat0
     eps = torch.full((tg.nat0,), complex(m) ** 2, dtype=torch.complex64, device="cuda")
     b = torch.empty((B, n), dtype=torch.complex64, device="cuda")
     ad = torch.empty_like(b); ao = torch.empty_like(b)
-    lib.setup_rhs(np.asarray(aks), np.asarray(e0s), x0, eps.data_ptr(), b.data_ptr(), ad.data_ptr(), ao.data_ptr())
+    lib.set_polarizability(prescription)
+    try:
+        lib.setup_rhs(np.asarray(aks), np.asarray(e0s), x0, eps.data_ptr(), b.data_ptr(), ad.data_ptr(),
+                      ao.data_ptr())
+    finally:
+        lib.set_polarizability("LATTDR")
     b, ad, ao = b.cpu().numpy(), ad.cpu().numpy(), ao.cpu().numpy()
     for r, (bref, adref) in enumerate(refs):
         eb = np.linalg.norm(b[r] - bref) / np.linalg.norm(bref)

@@ -97,3 +97,20 @@ def test_golden_fixtures_reproduce():
         assert np.allclose(y, f[f"y_{cw}"], rtol=1e-12, atol=1e-12)
     A = O.dense_operator(*dims, akd, f["adia"], f["aoff"], f["iocc"])
     assert np.allclose(A @ f["x"], f["y_N"], rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("dims", [(3, 4, 5), (6, 2, 3), (8, 7, 6)])
+def test_pruned_fp64_operator_equals_matvec_fft(dims):
+    """FftMatvec64 (the large-grid oracle operator: pruned transforms on
+    compact vectors) is matvec_fft, in both modes, on a grid with holes."""
+    rng = np.random.default_rng(sum(dims))
+    iocc, x, adia, aoff = case(rng, *dims)
+    go = O.green_octant(*dims, 0.33)
+    op = O.FftMatvec64(go, *dims, iocc, workers=2)
+    occ = O.occupied_index(iocc)
+    nat = int(np.prod(dims))
+    comp = lambda v: O.gather_compact(v, occ, nat)
+    for cw in ("N", "C"):
+        ref = comp(O.matvec_fft(cw, x, adia, aoff, go, *dims, iocc))
+        got = op(cw, comp(x), comp(adia), comp(aoff))
+        assert np.linalg.norm(got - ref) <= 1e-13 * np.linalg.norm(ref)
