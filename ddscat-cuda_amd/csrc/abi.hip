@@ -112,9 +112,9 @@ extern "C" DDSCAT_HIP_API int ddscat_cuda_prepare_f32(int nx, int ny, int nz, in
     release_context(c);
     return 2;
   }
-  int rc = finish_prepare(c, iocc, 2, 6);
-  if (rc) { release_context(c); return rc == 6 ? 6 : (rc >= 200 ? rc : 2); }
-  // CXZC arrives component-major; the resident copy is point-interleaved
+  // CXZC arrives component-major; the resident copy is point-interleaved.  It
+  // is uploaded before finish_prepare so that the autotuner times the pass-3
+  // candidates on the real tensor.
   float2* tmp = nullptr;
   if (hipMalloc((void**)&tmp, c.green_count * sizeof(float2)) != hipSuccess) {
     set_error("prepare", "malloc green staging failed");
@@ -129,6 +129,8 @@ extern "C" DDSCAT_HIP_API int ddscat_cuda_prepare_f32(int nx, int ny, int nz, in
     release_context(c);
     return 4;
   }
+  int rc = finish_prepare(c, iocc, 2, 6);
+  if (rc) { release_context(c); return rc == 6 ? 6 : (rc >= 200 ? rc : 2); }
   c.ready = true;
   return 0;
 }

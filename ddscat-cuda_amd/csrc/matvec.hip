@@ -495,9 +495,9 @@ int matvec_launch(Context& c, int cwhat, const float2* x, float2* y, const float
 // instance every target orientation of a DDSCAT run) does not re-time.
 namespace {
 struct TuneKey {
-  int nx, ny, nz;
+  int nx, ny, nz, ipbc;  // IPBC=1 reads the whole-grid Green in pass 3, not the octant
   bool operator<(const TuneKey& o) const {
-    return nx != o.nx ? nx < o.nx : ny != o.ny ? ny < o.ny : nz < o.nz;
+    return nx != o.nx ? nx < o.nx : ny != o.ny ? ny < o.ny : nz != o.nz ? nz < o.nz : ipbc < o.ipbc;
   }
 };
 std::map<TuneKey, std::array<int, 6>>& tune_cache() {
@@ -512,7 +512,7 @@ int matvec_autotune(Context& c) {
   for (int p = 1; p <= 5; ++p) m.kind[p] = reg_candidates(lens[p], p) > 0 ? 0 : -1;
   m.tuned = 0;
   if (!c.autotune || c.fft_generic || !c.sx || !c.sy || !c.adia_c || !c.aoff_c) return 0;
-  const TuneKey key{c.nx, c.ny, c.nz};
+  const TuneKey key{c.nx, c.ny, c.nz, c.ipbc};
   auto it = tune_cache().find(key);
   if (it != tune_cache().end()) {
     for (int p = 1; p <= 5; ++p) m.kind[p] = it->second[p];

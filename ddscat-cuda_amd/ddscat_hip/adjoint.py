@@ -36,6 +36,21 @@ def ldr_alpha_inv(eps: torch.Tensor, kd: float, khat, ehat) -> torch.Tensor:
     return inv_cm + corr
 
 
+class NotConverged(RuntimeError):
+    """A forward or adjoint solve stopped at its iteration limit above the tolerance."""
+
+
+def _require_converged(which, tole, tol, maxit):
+    """The batched solvers return 0 at the iteration limit with the achieved
+    residual (the reference's convention); a Q_abs or gradient from such a
+    solve must not reach the caller silently."""
+    tole = np.asarray(tole, np.float64)
+    bad = np.flatnonzero(~(tole <= 1.01 * tol))
+    if bad.size:
+        raise NotConverged(f"{which} solve: RHS {bad.tolist()} at residual {tole[bad].tolist()} > tol {tol} "
+                           f"after {maxit} iterations")
+
+
 class AbsorptionEfficiency(torch.autograd.Function):
     """Q_abs[r] for RHS r of the prepared target.
 
@@ -54,8 +69,9 @@ class AbsorptionEfficiency(torch.autograd.Function):
         e = e.contiguous()
         x = torch.zeros_like(e)
         torch.cuda.synchronize()
-        lib.solve_batch(method, B, e.data_ptr(), x.data_ptr(), adia.data_ptr(), aoff.data_ptr(), op_per_rhs=True,
-                        maxit=maxit, tol=tol)
+        _, tole, _, _ = lib.solve_batch(method, B, e.data_ptr(), x.data_ptr(), adia.data_ptr(), aoff.data_ptr(),
+                                        op_per_rhs=True, maxit=maxit, tol=tol)
+        _require_converged("forward", tole, tol, maxit)
         q = lib.evalq_batch(B, e.data_ptr(), x.data_ptr(), adia.data_ptr(), aoff.data_ptr(), ak)
         ctx.save_for_backward(adia, aoff, x)
         ctx.lib, ctx.ak, ctx.geo, ctx.method, ctx.tol, ctx.maxit = lib, ak, geo, method, tol, maxit
@@ -68,8 +84,9 @@ class AbsorptionEfficiency(torch.autograd.Function):
         g_adia = torch.empty_like(adia)
         g_aoff = torch.empty_like(aoff)
         torch.cuda.synchronize()
-        ctx.lib.cabs_adjoint(ctx.method, B, x.data_ptr(), adia.data_ptr(), aoff.data_ptr(), ctx.ak,
-                             g_adia.data_ptr(), g_aoff.data_ptr(), maxit=ctx.maxit, tol=ctx.tol)
+        _, tole, _ = ctx.lib.cabs_adjoint(ctx.method, B, x.data_ptr(), adia.data_ptr(), aoff.data_ptr(), ctx.ak,
+                                          g_adia.data_ptr(), g_aoff.data_ptr(), maxit=ctx.maxit, tol=ctx.tol)
+        _require_converged("adjoint", tole, ctx.tol, ctx.maxit)
         # real loss L: dL = Re(g . dz)  ->  torch's gradient for z is conj(g)
         w = (gq.to(torch.float32) / ctx.geo).reshape(B, 1)
         return (torch.conj(g_adia) * w, torch.conj(g_aoff) * w, None, None, None, None, None, None, None)
