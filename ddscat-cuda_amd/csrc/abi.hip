@@ -166,6 +166,15 @@ extern "C" DDSCAT_HIP_API int ddscat_hip_set_precision(int precision) {
   return 0;
 }
 
+extern "C" DDSCAT_HIP_API int ddscat_hip_set_reliable_mode(int mode) {
+  if (mode != RELIABLE_REPLACE && mode != RELIABLE_RESTART) {
+    set_error("set_reliable_mode", "expected DDSCAT_HIP_RELIABLE_REPLACE or DDSCAT_HIP_RELIABLE_RESTART");
+    return 1;
+  }
+  ctx().reliable_mode = mode;
+  return 0;
+}
+
 extern "C" DDSCAT_HIP_API int ddscat_hip_set_fft_kernels(int generic) {
   if (generic != 0 && generic != 1) { set_error("set_fft_kernels", "expected 0 or 1"); return 1; }
   ctx().fft_generic = generic;

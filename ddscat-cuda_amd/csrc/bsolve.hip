@@ -815,7 +815,9 @@ int bsolve_qmrccg(Context& c, BatchState& S, const float2* b, float2* x, const f
       const double gap = tr > 0.0 ? gp / tr : (gp == 0.0 ? 0.0 : INFINITY);
       conv[k] = tres[k] <= tol;
       rst[k] = !conv[k] && (gap >= RELIABLE_GAP || resid[k] <= tol);
-      c.log.check(k, tres[k], gap, rst[k] && it[k] < maxit);
+      const int action = !rst[k] || it[k] >= maxit ? 0
+                         : c.reliable_mode == RELIABLE_REPLACE ? SolveLog::REPLACE : SolveLog::RESTART;
+      c.log.check(k, tres[k], gap, action);
     }
     return 0;
   };
@@ -862,16 +864,22 @@ int bsolve_qmrccg(Context& c, BatchState& S, const float2* b, float2* x, const f
     }
     if (any_set(sel)) {
       if (int rc = reliable(conv, rst, tres)) return rc;
-      std::vector<int> restart(B, 0);
+      std::vector<int> restart(B, 0), replace(B, 0);
       for (int k = 0; k < B; ++k) {
         if (!sel[k]) continue;
         if (conv[k] || it[k] >= maxit) {
           resid[k] = tres[k];
           act[k] = 0;
         } else if (rst[k]) {
-          restart[k] = 1;
+          (c.reliable_mode == RELIABLE_REPLACE ? replace : restart)[k] = 1;
           resid[k] = tres[k];
         }
+      }
+      if (any_set(replace)) {
+        // r = b - A x (A x in sy from the check); the Lanczos recurrence goes on
+        if (o.upload(S.sel, replace)) return 59;
+        LinComb l; l.lit(b, 1.0).lit(sy, -1.0);
+        if (o.lin(r, l, Sm)) return 59;
       }
       if (any_set(restart)) {
         sel = restart;
@@ -1033,7 +1041,7 @@ int bsolve_pbcgs2(Context& c, BatchState& S, const float2* b, float2* x, const f
       } else if (gap >= RELIABLE_GAP || resid[k] <= tol) {
         restart[k] = 1;
       }
-      c.log.check(k, tres, gap, restart[k] != 0);
+      c.log.check(k, tres, gap, restart[k] ? SolveLog::RESTART : 0);
       resid[k] = tres;
     }
     if (any_set(restart)) {

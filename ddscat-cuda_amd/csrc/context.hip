@@ -60,9 +60,10 @@ void release_context(Context& c) {
   dfree(c.red); dfree(c.scal);
   if (c.hscal) hipHostFree(c.hscal);
   hipStream_t keep = c.stream;
-  int backend = c.backend, prec = c.precision, gen = c.fft_generic, tune = c.autotune;
+  int backend = c.backend, prec = c.precision, gen = c.fft_generic, tune = c.autotune, rmode = c.reliable_mode;
   c = Context{};
   c.stream = keep;
+  c.reliable_mode = rmode;
   c.backend = backend;
   c.precision = prec;
   c.fft_generic = gen;

@@ -18,8 +18,14 @@
 #include <vector>
 #include <cmath>
 #include <complex>
+#include <cstdio>
 
 namespace ddhip {
+
+struct FarfieldRegion {
+  unsigned char* base = nullptr;
+  size_t bytes = 0;
+};
 
 __global__ void k_evalq_q(const float2* __restrict__ p, float2* __restrict__ q, const float2* __restrict__ ad,
                           const float2* __restrict__ ao, int nat0, float rabs) {
@@ -174,19 +180,129 @@ __global__ __launch_bounds__(FF_DIRS) void k_farfield_rows(
   }
 }
 
-__global__ void k_farfield_reduce(const double2* __restrict__ part, int nchunks, int ndir, double2* __restrict__ S) {
+// Position-weighted far-field sums for the radiative torque:
+// partial[chunk][d][12] = S_b (b = x, y, z) then R_ab = sum r_a P_b e^{-i k.r}
+// (a = x, y, z, b fastest) over the chunk's rows.  Per row the x weight varies
+// along the row (one extra multiply-add set); y and z weights are the row's.
+__global__ __launch_bounds__(FF_DIRS) void k_farfield_rows_pos(
+    const float2* __restrict__ p, const int* __restrict__ occ, const int* __restrict__ rowptr,
+    const double2* __restrict__ tab, int ndir, int nx, int ny, int nz, int nat0, double dx1, double dx2, double dx3,
+    double x01, double x02, double x03, double2* __restrict__ part) {
+  const int chunk = blockIdx.x;
+  const int d = blockIdx.y * FF_DIRS + threadIdx.x;
+  const int nrows = ny * nz;
+  const int g0 = chunk * FF_ROWS, g1 = min(nrows, g0 + FF_ROWS);
+  if (d >= ndir) return;
+  double2 acc[12];
+#pragma unroll
+  for (int i = 0; i < 12; ++i) acc[i] = make_double2(0, 0);
+  const double2* t = tab + d;
+  const size_t sd = (size_t)ndir;
+  for (int g = g0; g < g1; ++g) {
+    const int q0 = rowptr[g], q1 = rowptr[g + 1];
+    if (q0 == q1) continue;
+    double2 r[3] = {make_double2(0, 0), make_double2(0, 0), make_double2(0, 0)};
+    double2 w[3] = {make_double2(0, 0), make_double2(0, 0), make_double2(0, 0)};
+    const int base = g * nx;
+    for (int q = q0; q < q1; ++q) {
+      const int ix = occ[q] - base;
+      const double2 e = t[(size_t)ix * sd];
+      const double xw = ((double)(ix + 1) + x01) * dx1;
+#pragma unroll
+      for (int b = 0; b < 3; ++b) {
+        const float2 a = p[q + (size_t)b * nat0];
+        const double2 ea = make_double2(e.x * a.x - e.y * a.y, e.x * a.y + e.y * a.x);
+        r[b] = zadd(r[b], ea);
+        w[b].x += xw * ea.x;
+        w[b].y += xw * ea.y;
+      }
+    }
+    const int iy = g % ny, iz = g / ny;
+    const double2 eyz = zmul(t[(size_t)(nx + iy) * sd], t[(size_t)(nx + ny + iz) * sd]);
+    const double yw = ((double)(iy + 1) + x02) * dx2, zw = ((double)(iz + 1) + x03) * dx3;
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+      const double2 sb = zmul(eyz, r[b]);
+      acc[b] = zadd(acc[b], sb);
+      acc[3 + b] = zadd(acc[3 + b], zmul(eyz, w[b]));
+      acc[6 + b] = zadd(acc[6 + b], zscale(sb, yw));
+      acc[9 + b] = zadd(acc[9 + b], zscale(sb, zw));
+    }
+  }
+  double2* o = part + ((size_t)chunk * ndir + d) * 12;
+#pragma unroll
+  for (int i = 0; i < 12; ++i) o[i] = acc[i];
+}
+
+// S[d][width] = sum over chunks of partial[chunk][d][width], fixed order
+__global__ void k_farfield_reduce(const double2* __restrict__ part, int nchunks, int ndir, int width,
+                                  double2* __restrict__ S) {
   const size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (id >= (size_t)ndir * 3) return;
+  if (id >= (size_t)ndir * width) return;
   double2 acc = make_double2(0, 0);
-  for (int ch = 0; ch < nchunks; ++ch) acc = zadd(acc, part[(size_t)ch * ndir * 3 + id]);
+  for (int ch = 0; ch < nchunks; ++ch) acc = zadd(acc, part[(size_t)ch * ndir * width + id]);
   S[id] = acc;
 }
 
-// S[d][3] for all directions (host result) of the compact polarization p
+// Incident-beam torque terms, per workgroup: out[blk][0..2] = sum Im(P^* . E) r,
+// out[blk][3..5] = sum Re(P x E^*), in FP64 over the compact sites.
+constexpr int TQ_THREADS = 256;
+__global__ __launch_bounds__(TQ_THREADS) void k_torque_local(const float2* __restrict__ p,
+                                                             const float2* __restrict__ e, const int* __restrict__ occ,
+                                                             int nx, int ny, int nat0, double dx1, double dx2,
+                                                             double dx3, double x01, double x02, double x03,
+                                                             double* __restrict__ out) {
+  double v[6] = {0, 0, 0, 0, 0, 0};
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < nat0; q += gridDim.x * blockDim.x) {
+    const int j = occ[q];
+    const int ix = j % nx, iy = (j / nx) % ny, iz = j / (nx * ny);
+    const double r[3] = {((double)(ix + 1) + x01) * dx1, ((double)(iy + 1) + x02) * dx2,
+                         ((double)(iz + 1) + x03) * dx3};
+    double2 P[3], E[3];
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+      const float2 pb = p[q + (size_t)b * nat0], eb = e[q + (size_t)b * nat0];
+      P[b] = make_double2(pb.x, pb.y);
+      E[b] = make_double2(eb.x, eb.y);
+    }
+    // Im(conj(P) . E)
+    double im = 0.0;
+#pragma unroll
+    for (int b = 0; b < 3; ++b) im += P[b].x * E[b].y - P[b].y * E[b].x;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) v[a] += im * r[a];
+    // Re(P x conj(E))_a = Re(P_b conj(E_c) - P_c conj(E_b)), (a, b, c) cyclic
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const int b = (a + 1) % 3, c = (a + 2) % 3;
+      v[3 + a] += (P[b].x * E[c].x + P[b].y * E[c].y) - (P[c].x * E[b].x + P[c].y * E[b].y);
+    }
+  }
+  __shared__ double red[TQ_THREADS];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    red[threadIdx.x] = v[k];
+    __syncthreads();
+    for (int h = TQ_THREADS / 2; h > 0; h >>= 1) {
+      if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) out[(size_t)blockIdx.x * 6 + k] = red[0];
+    __syncthreads();
+  }
+}
+
+// S[d][3] for all directions (host result) of the compact polarization p.
+// Device workspace (directions, phase tables, row-chunk partials, sums):
+// allocated per call, or carved out of `region` (the allocation-free SCAT
+// entry), where the direction slab shrinks to fit.  Returns 1 on failure, 2
+// when the region cannot hold even one direction's workspace.
 static int farfield_sums(Context& c, const float2* p, const std::vector<float>& kdirs, const float dx[3],
-                         const float x0[3], std::vector<std::complex<double>>& S) {
+                         const float x0[3], std::vector<std::complex<double>>& S, const FarfieldRegion* region,
+                         bool pos = false) {
   const int ndir = (int)(kdirs.size() / 3);
-  S.assign((size_t)ndir * 3, 0.0);
+  const int width = pos ? 12 : 3;  // complex sums per direction: S, or S and the position-weighted R
+  S.assign((size_t)ndir * width, 0.0);
   if (ndir == 0) return 0;
   hipStream_t st = c.stream;
   const int L = c.nx + c.ny + c.nz;
@@ -195,16 +311,40 @@ static int farfield_sums(Context& c, const float2* p, const std::vector<float>& 
   float* dk = nullptr;
   double2 *tab = nullptr, *part = nullptr, *dS = nullptr;
   int rc = 0;
-  auto cleanup = [&]() { hipFree(dk); hipFree(tab); hipFree(part); hipFree(dS); };
-  // bound the partial buffer: process directions in slabs
-  const int slab = std::max(FF_DIRS, std::min(ndir, (int)std::max<size_t>(FF_DIRS, (size_t)(256u << 20) / ((size_t)nchunks * 48))));
-  if (hipMalloc((void**)&dk, kdirs.size() * sizeof(float)) != hipSuccess ||
-      hipMalloc((void**)&tab, (size_t)slab * L * sizeof(double2)) != hipSuccess ||
-      hipMalloc((void**)&part, (size_t)nchunks * slab * 3 * sizeof(double2)) != hipSuccess ||
-      hipMalloc((void**)&dS, (size_t)ndir * 3 * sizeof(double2)) != hipSuccess) {
-    set_error("GPU SCAT", "far-field workspace allocation failed");
-    cleanup();
-    return 1;
+  int slab = 0;
+  auto cleanup = [&]() {
+    if (region) return;
+    hipFree(dk); hipFree(tab); hipFree(part); hipFree(dS);
+  };
+  if (region) {
+    auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const size_t fixed = up(kdirs.size() * sizeof(float)) + up((size_t)ndir * width * sizeof(double2));
+    const size_t per_dir = (size_t)L * sizeof(double2) + (size_t)nchunks * width * sizeof(double2);
+    if (region->bytes < fixed + 512 + per_dir) {
+      char msg[200];
+      std::snprintf(msg, sizeof(msg), "far-field workspace of %d directions on %d lattice rows needs more than the "
+                    "%zu-byte scratch region", ndir, nrows, region->bytes);
+      set_error("GPU SCAT", msg);
+      return 2;
+    }
+    slab = (int)std::min<size_t>((size_t)ndir, (region->bytes - fixed - 512) / per_dir);
+    unsigned char* b = region->base;
+    dk = (float*)b; b += up(kdirs.size() * sizeof(float));
+    dS = (double2*)b; b += up((size_t)ndir * width * sizeof(double2));
+    tab = (double2*)b; b += up((size_t)slab * L * sizeof(double2));
+    part = (double2*)b;
+  } else {
+    // bound the partial buffer: process directions in slabs
+    slab = std::max(FF_DIRS, std::min(ndir, (int)std::max<size_t>(FF_DIRS, (size_t)(256u << 20) /
+                                                                    ((size_t)nchunks * width * sizeof(double2)))));
+    if (hipMalloc((void**)&dk, kdirs.size() * sizeof(float)) != hipSuccess ||
+        hipMalloc((void**)&tab, (size_t)slab * L * sizeof(double2)) != hipSuccess ||
+        hipMalloc((void**)&part, (size_t)nchunks * slab * width * sizeof(double2)) != hipSuccess ||
+        hipMalloc((void**)&dS, (size_t)ndir * width * sizeof(double2)) != hipSuccess) {
+      set_error("GPU SCAT", "far-field workspace allocation failed");
+      cleanup();
+      return 1;
+    }
   }
   if (hipMemcpyAsync(dk, kdirs.data(), kdirs.size() * sizeof(float), hipMemcpyHostToDevice, st) != hipSuccess) {
     set_error("GPU SCAT", "directions H2D failed");
@@ -217,13 +357,18 @@ static int farfield_sums(Context& c, const float2* p, const std::vector<float>& 
     hipLaunchKernelGGL(k_phase_tables, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, st, dk + 3 * d0, nd, c.nx,
                        c.ny, c.nz, (double)dx[0], (double)dx[1], (double)dx[2], (double)x0[0], (double)x0[1],
                        (double)x0[2], tab);
-    hipLaunchKernelGGL(k_farfield_rows, dim3(nchunks, (nd + FF_DIRS - 1) / FF_DIRS), dim3(FF_DIRS), 0, st, p,
-                       c.occ, c.rowptr, tab, nd, c.nx, c.ny, c.nz, c.nat0, part);
-    hipLaunchKernelGGL(k_farfield_reduce, dim3((unsigned)((nd * 3 + 255) / 256)), dim3(256), 0, st, part, nchunks,
-                       nd, dS + (size_t)3 * d0);
+    if (pos)
+      hipLaunchKernelGGL(k_farfield_rows_pos, dim3(nchunks, (nd + FF_DIRS - 1) / FF_DIRS), dim3(FF_DIRS), 0, st, p,
+                         c.occ, c.rowptr, tab, nd, c.nx, c.ny, c.nz, c.nat0, (double)dx[0], (double)dx[1],
+                         (double)dx[2], (double)x0[0], (double)x0[1], (double)x0[2], part);
+    else
+      hipLaunchKernelGGL(k_farfield_rows, dim3(nchunks, (nd + FF_DIRS - 1) / FF_DIRS), dim3(FF_DIRS), 0, st, p,
+                         c.occ, c.rowptr, tab, nd, c.nx, c.ny, c.nz, c.nat0, part);
+    hipLaunchKernelGGL(k_farfield_reduce, dim3((unsigned)((nd * width + 255) / 256)), dim3(256), 0, st, part,
+                       nchunks, nd, width, dS + (size_t)width * d0);
     if (hipGetLastError() != hipSuccess) { set_error("GPU SCAT", "far-field kernel launch failed"); rc = 1; }
   }
-  if (!rc && (hipMemcpyAsync(S.data(), dS, (size_t)ndir * 3 * sizeof(double2), hipMemcpyDeviceToHost, st) != hipSuccess ||
+  if (!rc && (hipMemcpyAsync(S.data(), dS, (size_t)ndir * width * sizeof(double2), hipMemcpyDeviceToHost, st) != hipSuccess ||
               hipStreamSynchronize(st) != hipSuccess)) {
     set_error("GPU SCAT", "far-field sums D2H failed");
     rc = 1;
@@ -234,7 +379,7 @@ static int farfield_sums(Context& c, const float2* p, const std::vector<float>& 
 
 int scat(Context& c, int jpbc, int ndir, const float* ak, const float* aks, const float* dx, const float* em1,
          const float* em2, float e02, float etasca, const void* e01v, const float* x0, double* cbksca, double* csca,
-         double* cscag, double* cscag2, int* navg, void* f1v, void* f2v) {
+         double* cscag, double* cscag2, int* navg, void* f1v, void* f2v, const FarfieldRegion* region) {
   clear_error();
   if (!c.ready || !c.solution_valid || !ak || !dx || !x0 || !cbksca || !csca || !cscag || !cscag2 || !navg ||
       !(e02 > 0.f)) {
@@ -254,7 +399,7 @@ int scat(Context& c, int jpbc, int ndir, const float* ak, const float* aks, cons
     for (size_t i = 0; i < dirs.size(); ++i)
       for (int k = 0; k < 3; ++k) kd[3 * i + k] = dirs[i].k[k];
     std::vector<std::complex<double>> S;
-    if (farfield_sums(c, c.sx, kd, dx, x0, S)) return 325;
+    if (int frc = farfield_sums(c, c.sx, kd, dx, x0, S, region)) return frc == 2 ? 334 : 325;
     double acc[5] = {0, 0, 0, 0, 0}, back = 0.0;
     for (size_t i = 0; i < dirs.size(); ++i) {
       const double nxv = (double)kd[3 * i] / akk, nyv = (double)kd[3 * i + 1] / akk, nzv = (double)kd[3 * i + 2] / akk;
@@ -273,7 +418,7 @@ int scat(Context& c, int jpbc, int ndir, const float* ak, const float* aks, cons
   if (ndir > 0) {
     std::vector<float> kd(aks, aks + 3 * (size_t)ndir);
     std::vector<std::complex<double>> S;
-    if (farfield_sums(c, c.sx, kd, dx, x0, S)) return 330;
+    if (int frc = farfield_sums(c, c.sx, kd, dx, x0, S, region)) return frc == 2 ? 334 : 330;
     const double term = ak3 / std::sqrt((double)e02);
     float2* f1 = (float2*)f1v;
     float2* f2 = (float2*)f2v;
@@ -336,6 +481,98 @@ static void amplitude_to_mueller(const std::complex<double> f[4], const double k
   M[15] = (s12 - s34).real();
 }
 
+// Radiative torque of the resident solution (DDSCAT's DOTORQ sums), as torque
+// cross sections C = Gamma k / u_rad, u_rad = E0^2 / 8 pi, about the origin of
+// the site coordinates r_j = ((i+1) + x0) dx:
+//   ext: the incident wave's torque on the dipoles,
+//        sum_j [ r_j x F_j + (1/2) Re(P_j x E_j^*) ],  F_j = -(1/2) Im(P_j^* . E_j) k;
+//   rad: the recoil of the radiated field.  The radiative part of the
+//        interaction is G_I(r) = (k^3 / 4 pi) int dOmega (I - n n) e^{i k n.r}
+//        (radiation reaction (2/3) k^3 at r = 0), so its torque on the dipoles is
+//        (k^3 / 4 pi) int dOmega [ (1/2) Re(-i S x S_perp^*) - (k/2) Re(conj(R) S_perp) x n ]
+//        with S_b = sum_j P_jb e^{-i k n.r_j}, R_ab = sum_j r_ja P_jb e^{-i k n.r_j},
+//        S_perp = S - n (n.S), integrated over SCAT's ETASCA quadrature.
+// The reactive part of the interaction exerts no net torque, so ext + rad is
+// the torque on the target; for an absorbing sphere in circularly polarised
+// light its component along k is +-C_abs.  E is the resident right-hand side
+// (the incident field at the sites, as EVALQ's resident mode uses it).
+int torque(Context& c, const float* ak, const float2* e01, float e02, float etasca, const float* dx, const float* x0,
+           double* cext, double* crad, int* navg) {
+  clear_error();
+  if (!c.ready || !c.solution_valid || !ak || !e01 || !dx || !x0 || !cext || !crad || !navg || !(e02 > 0.f) ||
+      !(etasca > 0.f)) {
+    set_error("torque", "invalid state/argument");
+    return 350;
+  }
+  const double kv[3] = {ak[0], ak[1], ak[2]};
+  const double k = std::sqrt(kv[0] * kv[0] + kv[1] * kv[1] + kv[2] * kv[2]);
+  if (!(k > 0.0)) { set_error("torque", "zero wave vector"); return 350; }
+  hipStream_t st = c.stream;
+  // incident-beam part: per-workgroup partials in the reduction buffer
+  const int nb = std::min(RED_MAX_BLOCKS, std::max(1, (c.nat0 + TQ_THREADS - 1) / TQ_THREADS));
+  hipLaunchKernelGGL(k_torque_local, dim3(nb), dim3(TQ_THREADS), 0, st, c.sx, c.sb, c.occ, c.nx, c.ny, c.nat0,
+                     (double)dx[0], (double)dx[1], (double)dx[2], (double)x0[0], (double)x0[1], (double)x0[2], c.red);
+  if (hipGetLastError() != hipSuccess) { set_error("torque", "local-sum kernel launch failed"); return 351; }
+  std::vector<double> part((size_t)nb * 6);
+  if (hipMemcpyAsync(part.data(), c.red, part.size() * sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipStreamSynchronize(st) != hipSuccess) {
+    set_error("torque", "local sums D2H failed");
+    return 351;
+  }
+  double v[6] = {0, 0, 0, 0, 0, 0};
+  for (int b = 0; b < nb; ++b)
+    for (int i = 0; i < 6; ++i) v[i] += part[(size_t)b * 6 + i];
+  const double u = (double)e02 / (8.0 * M_PI);
+  const double cf = k / u;
+  // sum_j r_j x F_j = -(1/2) (sum_j Im(P^*.E) r_j) x k
+  cext[0] = cf * (-0.5 * (v[1] * kv[2] - v[2] * kv[1]) + 0.5 * v[3]);
+  cext[1] = cf * (-0.5 * (v[2] * kv[0] - v[0] * kv[2]) + 0.5 * v[4]);
+  cext[2] = cf * (-0.5 * (v[0] * kv[1] - v[1] * kv[0]) + 0.5 * v[5]);
+  // radiated part over the quadrature directions
+  std::vector<ScatDir> dirs;
+  if (scat_integration_dirs(ak, e01, etasca, c.nat0, dirs)) return 350;
+  *navg = (int)dirs.size();
+  std::vector<float> kd(3 * dirs.size());
+  for (size_t i = 0; i < dirs.size(); ++i)
+    for (int a = 0; a < 3; ++a) kd[3 * i + a] = dirs[i].k[a];
+  std::vector<std::complex<double>> SR;
+  if (farfield_sums(c, c.sx, kd, dx, x0, SR, nullptr, true)) return 351;
+  using cd = std::complex<double>;
+  double acc[3] = {0, 0, 0};
+  for (size_t i = 0; i < dirs.size(); ++i) {
+    const cd* q = SR.data() + 12 * i;
+    double n[3];
+    double kk = 0.0;
+    for (int a = 0; a < 3; ++a) { n[a] = kd[3 * i + a]; kk += n[a] * n[a]; }
+    kk = std::sqrt(kk);
+    for (int a = 0; a < 3; ++a) n[a] /= kk;
+    const cd S[3] = {q[0], q[1], q[2]};
+    const cd ns = n[0] * S[0] + n[1] * S[1] + n[2] * S[2];
+    const cd Sp[3] = {S[0] - n[0] * ns, S[1] - n[1] * ns, S[2] - n[2] * ns};
+    double t[3];
+    for (int a = 0; a < 3; ++a) {
+      const int b = (a + 1) % 3, c2 = (a + 2) % 3;
+      // (1/2) Re(-i (S x Sp^*)_a)
+      const cd cr = S[b] * std::conj(Sp[c2]) - S[c2] * std::conj(Sp[b]);
+      t[a] = 0.5 * std::imag(cr);
+    }
+    // m_a = Re(sum_b conj(R_ab) Sp_b); orbital -(k/2) m x n
+    double m[3];
+    for (int a = 0; a < 3; ++a) {
+      cd z = 0.0;
+      for (int b = 0; b < 3; ++b) z += std::conj(q[3 + 3 * a + b]) * Sp[b];
+      m[a] = z.real();
+    }
+    t[0] -= 0.5 * k * (m[1] * n[2] - m[2] * n[1]);
+    t[1] -= 0.5 * k * (m[2] * n[0] - m[0] * n[2]);
+    t[2] -= 0.5 * k * (m[0] * n[1] - m[1] * n[0]);
+    for (int a = 0; a < 3; ++a) acc[a] += dirs[i].w[0] * t[a];
+  }
+  const double pre = k * k * k / (4.0 * M_PI) * cf;
+  for (int a = 0; a < 3; ++a) crad[a] = pre * acc[a];
+  return 0;
+}
+
 int scat_matrix(Context& c, int npair, const float2* p, int ndir, const float* ak, const float2* e0,
                 const float* aks, const float* em1, const float* em2, const float* dx, const float* x0, float e02,
                 float2* famp, double* mueller) {
@@ -361,7 +598,7 @@ int scat_matrix(Context& c, int npair, const float2* p, int ndir, const float* a
     std::vector<float> kd(aks + o, aks + o + 3 * (size_t)ndir);
     std::vector<std::complex<double>> S[2];
     for (int j = 0; j < 2; ++j)
-      if (farfield_sums(c, p + (2 * (size_t)q + j) * n, kd, dx, x0, S[j])) return 341;
+      if (farfield_sums(c, p + (2 * (size_t)q + j) * n, kd, dx, x0, S[j], nullptr)) return 341;
     for (int d = 0; d < ndir; ++d) {
       const float* a = em1 + o + 3 * d;
       const float* b = em2 + o + 3 * d;
@@ -412,5 +649,11 @@ extern "C" DDSCAT_HIP_API int ddscat_cuda_scat_f32(int jpbc, int ndir, const flo
                                                    double* csca, double* cscag, double* cscag2, int* navg, void* f1,
                                                    void* f2) {
   return scat(ctx(), jpbc, ndir, ak, aks, dx, em1, em2, e02, etasca, e01, x0, cbksca, csca, cscag, cscag2, navg, f1,
-              f2);
+              f2, nullptr);
+}
+
+extern "C" DDSCAT_HIP_API int ddscat_hip_torque(const float* ak, const void* e01, float e02, float etasca,
+                                                const float* dx, const float* x0, double* cext, double* crad,
+                                                int* navg) {
+  return torque(ctx(), ak, (const float2*)e01, e02, etasca, dx, x0, cext, crad, navg);
 }

@@ -79,7 +79,7 @@ int solver_end(Context& c, const SolverArgs& a, const float2* xs, int it, double
 // true residual ||rhs - A x|| and gap ||rhs - A x - rrec|| in FP64 (cu:543-556);
 // leaves A x in c.sy
 int reliable_check(Context& c, const float2* rhs, const float2* xcur, const float2* rrec, double rhsnorm,
-                   double rec_res, double tol, int& nc, Reliable& rr) {
+                   double rec_res, double tol, int& nc, Reliable& rr, bool replaceable) {
   const size_t n = (size_t)3 * c.nat0;
   if (int rc = run_mv(c, 'N', xcur, c.sy, nc)) return rc;
   DotSet d;
@@ -96,8 +96,12 @@ int reliable_check(Context& c, const float2* rhs, const float2* xcur, const floa
   }
   rr.converged = rr.true_res <= tol;
   rr.restart = rr.gap >= RELIABLE_GAP || (rec_res <= tol && !rr.converged);
+  if (rr.restart && replaceable && c.reliable_mode == RELIABLE_REPLACE) {
+    rr.restart = false;
+    rr.replace = true;
+  }
   if (rr.restart) ++c.restarts;
-  c.log.check(0, rr.true_res, rr.gap, rr.restart);
+  c.log.check(0, rr.true_res, rr.gap, rr.restart ? SolveLog::RESTART : rr.replace ? SolveLog::REPLACE : 0);
   return 0;
 }
 
@@ -327,9 +331,14 @@ int solve_qmrccg(Context& c, const SolverArgs& a) {
     c.log.iter(0, resid);
     if ((it % RELIABLE_PERIOD) == 0 || resid <= a.tol) {
       Reliable rr;
-      if (int rc = reliable_check(c, c.sb, xs, r, bnorm, resid, a.tol, nc, rr)) return rc > 1 ? rc : 59;
+      if (int rc = reliable_check(c, c.sb, xs, r, bnorm, resid, a.tol, nc, rr, true)) return rc > 1 ? rc : 59;
       if (rr.converged) { resid = rr.true_res; break; }
-      if (rr.restart) {
+      if (rr.replace) {
+        // r = b - A x (A x left in c.sy by the check); v~, w~ and the scalars go on
+        LinComb l; l.lit(c.sb, 1.0).lit(c.sy, -1.0);
+        if (lincomb(c, r, l, n, st)) return 59;
+        resid = rr.true_res;
+      } else if (rr.restart) {
         LinComb l; l.lit(c.sb, 1.0).lit(c.sy, -1.0);
         if (lincomb(c, c.sy, l, n, st)) return 59;
         if (int rc = restart_from(c.sy)) return rc;

@@ -36,6 +36,7 @@ struct SolverArgs {
 struct Reliable {
   double true_res = INFINITY, gap = INFINITY;
   bool converged = false, restart = false;
+  bool replace = false;  // QMRCCG under RELIABLE_REPLACE: take the true residual, keep the recurrence
 };
 
 // argument checks, boundary copies (full host -> compact device), clears the
@@ -43,8 +44,11 @@ struct Reliable {
 int solver_begin(Context& c, const SolverArgs& a, int code_arg, int code_io, const char* name, float2* xs);
 // publishes xs as the resident solution, copies it to the host full vector
 int solver_end(Context& c, const SolverArgs& a, const float2* xs, int it, double resid, int nc, int code);
+// replaceable: the solver's recursive residual only monitors convergence
+// (QMRCCG), so under c.reliable_mode == RELIABLE_REPLACE a failed check asks
+// for the true residual to replace it (rr.replace) instead of a restart
 int reliable_check(Context& c, const float2* rhs, const float2* xcur, const float2* rrec, double rhsnorm,
-                   double rec_res, double tol, int& nc, Reliable& rr);
+                   double rec_res, double tol, int& nc, Reliable& rr, bool replaceable = false);
 
 // Single-RHS QMRCCG vector steps (cu:437, cu:438, cu:443) on the fused
 // kernels of the batched solver with one RHS; coefficients from c.scal.

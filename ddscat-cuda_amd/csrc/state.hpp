@@ -29,9 +29,16 @@ struct MatvecPlan {
 
 // Convergence record of the last solve (batched: one entry per RHS;
 // single-RHS solvers: RHS 0), read by ddscat_hip_solve_log.
+// What a failed reliable-residual check (gap >= 1e-3 or false convergence,
+// cu:543-556) does to QMRCCG: REPLACE takes the true residual as the
+// recursive one and keeps the Lanczos recurrence (default); RESTART restarts
+// the recurrence from the true residual, as the reference does.
+enum ReliableMode { RELIABLE_REPLACE = 0, RELIABLE_RESTART = 1 };
+
 struct SolveLog {
+  enum { RESTART = 1, REPLACE = 2 };  // action column of a check row
   std::vector<std::vector<double>> resid;   // recursive residual after each iteration
-  std::vector<std::vector<double>> checks;  // (iteration, true residual, gap, restarted) per reliable check
+  std::vector<std::vector<double>> checks;  // (iteration, true residual, gap, action) per reliable check
   std::vector<int> restarts;
   void reset(int n) {
     resid.assign(n, {});
@@ -41,10 +48,10 @@ struct SolveLog {
   void iter(int r, double v) {
     if (r >= 0 && r < (int)resid.size()) resid[r].push_back(v);
   }
-  void check(int r, double true_res, double gap, bool restarted) {
+  void check(int r, double true_res, double gap, int action) {
     if (r < 0 || r >= (int)resid.size()) return;
-    checks[r].insert(checks[r].end(), {(double)resid[r].size(), true_res, gap, restarted ? 1.0 : 0.0});
-    if (restarted) ++restarts[r];
+    checks[r].insert(checks[r].end(), {(double)resid[r].size(), true_res, gap, (double)action});
+    if (action == RESTART) ++restarts[r];
   }
 };
 
@@ -84,6 +91,7 @@ struct Context {
   unsigned long long matvec_count = 0;
   int restarts = 0;               // reliable-residual restarts of the last single-RHS solve
   SolveLog log;                   // convergence record of the last solve
+  int reliable_mode = RELIABLE_REPLACE;  // QMRCCG's response to a failed reliable check
   double forced_gap = 0.0;        // added once to the next reliable check's gap (test hook)
   double green_build_ms = 0.0;
 };

@@ -33,6 +33,7 @@ EXTENSION_SYMBOLS = [
     "ddscat_hip_cabs_adjoint", "ddscat_hip_setup_rhs", "ddscat_hip_set_autotune",
     "ddscat_hip_nearfield", "ddscat_hip_set_pass_kernel", "ddscat_hip_scat_matrix", "ddscat_hip_force_restart",
     "ddscat_hip_solve_log", "ddscat_hip_set_polarizability",
+    "ddscat_hip_torque", "ddscat_hip_set_reliable_mode",
 ]
 # method ids of ddscat_hip_solve_device
 SOLVER_IDS = {"pbcgs2": 0, "qmrccg": 1, "petrkp": 2, "gpbicg": 3, "pbcgst": 4, "bicgstab4": 5,
@@ -100,6 +101,10 @@ class Library:
         L.ddscat_cuda_evalq_f32.restype = i
         L.ddscat_cuda_scat_f32.argtypes = [i, i, vp, vp, vp, vp, vp, f, f, vp, vp, vp, vp, vp, vp, vp, vp, vp]
         L.ddscat_cuda_scat_f32.restype = i
+        L.ddscat_hip_torque.argtypes = [vp, vp, f, f, vp, vp, vp, vp, vp]
+        L.ddscat_hip_torque.restype = i
+        L.ddscat_hip_set_reliable_mode.argtypes = [i]
+        L.ddscat_hip_set_reliable_mode.restype = i
         L.ddscat_cuda_release.argtypes = []
         L.ddscat_cuda_release.restype = i
         L.ddscat_cuda_last_error.argtypes = []
@@ -313,15 +318,32 @@ class Library:
         cb, cs, cg2 = C.c_double(), C.c_double(), C.c_double()
         cg = (C.c_double * 3)()
         navg = C.c_int(0)
-        rc = self.lib.ddscat_cuda_scat_f32(jpbc, ndir, _p(ak), _p(aks), _p(dx), _p(em1), _p(em2), float(e02),
-                                           float(etasca), _p(e01), _p(x0), C.byref(cb), C.byref(cs), cg,
-                                           C.byref(cg2), C.byref(navg), _p(f1), _p(f2))
+        fn = self.lib.ddscat_cuda_scat_f32
+        rc = fn(jpbc, ndir, _p(ak), _p(aks), _p(dx), _p(em1), _p(em2), float(e02), float(etasca), _p(e01), _p(x0),
+                C.byref(cb), C.byref(cs), cg, C.byref(cg2), C.byref(navg), _p(f1), _p(f2))
         if check:
             self._check("scat", rc)
         res = dict(cbksca=cb.value, csca=cs.value, cscag=np.array(cg[:]), cscag2=cg2.value, navg=navg.value, rc=rc)
         if ndir:
             res["f1"], res["f2"] = f1, f2
         return res
+
+    def torque(self, ak, e01, etasca, dx=(1.0, 1.0, 1.0), x0=(0.0, 0.0, 0.0), e02=None, check=True):
+        """Torque cross sections (ext, rad, total [3] each, navg) of the resident
+        solution; e02 defaults to |e01|^2."""
+        ak = np.ascontiguousarray(ak, dtype=np.float32)
+        dx = np.ascontiguousarray(dx, dtype=np.float32)
+        x0 = np.ascontiguousarray(x0, dtype=np.float32)
+        e01 = _c64(e01, 3)
+        e02 = float(np.sum(np.abs(e01.astype(np.complex128)) ** 2)) if e02 is None else float(e02)
+        ce = np.zeros(3)
+        cr = np.zeros(3)
+        navg = C.c_int(0)
+        rc = self.lib.ddscat_hip_torque(_p(ak), _p(e01), e02, float(etasca), _p(dx), _p(x0), _p(ce), _p(cr),
+                                        C.byref(navg))
+        if check:
+            self._check("torque", rc)
+        return dict(ext=ce, rad=cr, total=ce + cr, navg=navg.value, rc=rc)
 
     # ----------------------------------------------------------- extensions
     def set_backend(self, backend):
@@ -372,7 +394,8 @@ class Library:
     def solve_log(self, rhs=0) -> dict:
         """Convergence record of RHS `rhs` of the last solve: restarts, the
         recursive residual after each iteration, and the reliable checks as
-        rows (iteration, true residual, gap, restarted)."""
+        rows (iteration, true residual, gap, action: 0 none, 1 restart, 2
+        residual replaced)."""
         nrst = C.c_int(0)
         nck = C.c_int(0)
         n = self.lib.ddscat_hip_solve_log(int(rhs), C.byref(nrst), None, 0, None, 0, C.byref(nck))
@@ -387,6 +410,11 @@ class Library:
         """LDR prescription of setup_rhs: "LATTDR" (default) or "GKDLDR"."""
         code = {"LATTDR": 0, "GKDLDR": 1}.get(prescription, -1)
         return self._check("set_polarizability", self.lib.ddscat_hip_set_polarizability(code))
+
+    def set_reliable_mode(self, mode):
+        """QMRCCG's response to a failed reliable check: "replace" (default) or "restart" (the reference's)."""
+        code = {"replace": 0, "restart": 1}.get(mode, -1)
+        return self._check("set_reliable_mode", self.lib.ddscat_hip_set_reliable_mode(code))
 
     def set_autotune(self, on):
         return self._check("set_autotune", self.lib.ddscat_hip_set_autotune(1 if on else 0))

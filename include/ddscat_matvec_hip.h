@@ -116,6 +116,19 @@ DDSCAT_HIP_API int ddscat_hip_set_backend(int backend);
  * FP32 = plain single precision throughout. */
 DDSCAT_HIP_API int ddscat_hip_set_precision(int precision);
 
+/* What QMRCCG (single-RHS and batched) does when a reliable-residual check
+ * (every 20 iterations, cu:543-556) finds a gap >= 1e-3 between the
+ * recursive and the true residual, or a false convergence:
+ *   REPLACE (default): the true residual replaces the recursive one and the
+ *     Lanczos recurrence goes on (QMR's residual vector only monitors
+ *     convergence).  With FP32 vectors the two residuals drift apart by
+ *     about 1e-6 |b|, so below a residual of about 1e-3 every check fails;
+ *   RESTART: the reference's policy, a restart of the recurrence from the
+ *     true residual at every failed check.
+ * The other solvers keep the reference's restart.  Code 1: unknown mode. */
+enum { DDSCAT_HIP_RELIABLE_REPLACE = 0, DDSCAT_HIP_RELIABLE_RESTART = 1 };
+DDSCAT_HIP_API int ddscat_hip_set_reliable_mode(int mode);
+
 /* FFT kernel family: 0 = register-resident two-level FFTs where the padded
  * axis length allows (64, 128, 256), LDS-staged Stockham otherwise
  * (default); 1 = LDS-staged Stockham for every axis. */
@@ -195,6 +208,25 @@ DDSCAT_HIP_API int ddscat_hip_cabs_adjoint(int method, int nrhs, const void *p, 
                                            int maxit, float tol, void *g_adia, void *g_aoff, int *itno,
                                            float *tole);
 
+/* Radiative torque of the resident solution (DDSCAT's DOTORQ sums, which the
+ * reference's GPU SCAT leaves to the CPU SCAT:
+ * ddscat_cuda_unified/GPU_FINAL_SECTIONS.md:75-80, dispatch
+ * fortran/getfml_cuda.f90:1135-1150), as torque cross sections
+ * C = Gamma k / u_rad (u_rad = e02 / 8 pi; Q_Gamma = C / (pi a_eff^2)) about
+ * the origin of the site coordinates r_j = ((i+1) + x0) dx:
+ *   cext[3]  the incident wave's torque on the dipoles, with the resident
+ *            right-hand side as the incident field (FP64 site sums);
+ *   crad[3]  the recoil of the radiated field: the far-field sums S(n) and the
+ *            position-weighted sums R(n) = sum_j r_j P_j exp(-i k n.r_j)
+ *            integrated over SCAT's ETASCA quadrature (*navg directions).
+ * cext + crad is the total torque (DDSCAT's QTRQAB + QTRQSC split it
+ * differently into absorption and scattering parts).  ak, e01 (complex64
+ * [3]), e02, etasca, dx, x0 as for ddscat_cuda_scat_f32.  Codes: 350
+ * invalid state (no resident solution) or argument, 351 device failure. */
+DDSCAT_HIP_API int ddscat_hip_torque(const float *ak, const void *e01, float e02, float etasca,
+                                     const float *dx, const float *x0, double *cext, double *crad,
+                                     int *navg);
+
 /* LDR polarizability prescription used by ddscat_hip_setup_rhs (DDSCAT's
  * CALPHA, ddscat_cuda_unified/fortran/reapar_cuda.f90:497-513): LATTDR
  * (Draine & Goodman 1993, default) or GKDLDR (Gutkowicz-Krusin & Draine
@@ -260,7 +292,8 @@ DDSCAT_HIP_API int ddscat_hip_force_restart(void);
  * single-RHS solve.  *restarts = reliable-residual restarts (cu:543-556
  * policy); resid[0..n-1] = the recursive residual after each iteration (as
  * many as fit; PBCGS2 and QMRCCG); checks[4*i..4*i+3] = (iteration, true
- * residual, gap, restarted) of reliable check i (up to nchecks of them);
+ * residual, gap, action: 0 none, 1 restart, 2 residual replaced) of
+ * reliable check i (up to nchecks of them);
  * *nchecks_out = the number of checks.  Any output pointer may be NULL.
  * Returns the number of iterations recorded, -1 if there is no such RHS. */
 DDSCAT_HIP_API int ddscat_hip_solve_log(int rhs, int *restarts, double *resid, int n, double *checks,

@@ -146,3 +146,98 @@ def mie_s12(x, m, theta):
         pi0, pi1 = pin, ((2 * n + 1) * mu * pin - (n + 1) * pi0) / n
         psi0, psi1, chi0, chi1, xi1 = psi1, psi, chi1, chi, xi
     return S1, S2
+
+
+# ----------------------------------------------------------------------------
+# Radiative torque (DDSCAT's DOTORQ sums; the reference's GPU SCAT hands them
+# back to the CPU, ddscat_cuda_unified/GPU_FINAL_SECTIONS.md:75-80, dispatch
+# fortran/getfml_cuda.f90:1135-1150)
+# ----------------------------------------------------------------------------
+def torque_sums(p_c, e_c, occ, nx, ny, ak, dx, x0, etasca, e01, nat0=None):
+    """Torque on the dipole array, as torque cross sections C = Gamma k / u_rad
+    (u_rad = |E0|^2 / 8 pi, Gaussian units; Q = C / (pi a_eff^2)), in two parts:
+
+      ext: the incident plane wave's torque on the dipoles,
+           Gamma = sum_j [ r_j x F_j + (1/2) Re(P_j x E_j^*) ],
+           F_j = -(1/2) Im(P_j^* . E_j) k      (E_j the incident field at r_j);
+      rad: the recoil of the radiated field, from the radiative part of the
+           interaction, G_I(r) = (k^3 / 4 pi) int dOmega (I - n n) e^{i k n.r}
+           (which includes the radiation reaction (2/3) k^3 at r = 0):
+           Gamma = (k^3 / 4 pi) int dOmega [ (1/2) Re(-i S x S_perp^*)
+                                            - (k/2) Re(conj(R) S_perp) x n ],
+           S(n) = sum_j P_j e^{-i k n.r_j},  R_ab(n) = sum_j r_ja P_jb e^{-i k n.r_j},
+           S_perp = S - n (n.S), integrated with SCAT's ETASCA quadrature.
+
+    The total ext + rad is the torque; for an absorbing sphere in circularly
+    polarised light its component along k is +-C_abs.  Returns dict(ext, rad,
+    total [3] each, navg).  FP64, compact [3][NAT0] P and E."""
+    from .ddscat_oracle import scat_integration_dirs, site_positions
+    ak = np.asarray(ak, np.float64)
+    k = float(np.linalg.norm(ak))
+    e02 = float(np.sum(np.abs(np.asarray(e01, complex)) ** 2))
+    n0 = np.asarray(p_c).size // 3
+    P = np.asarray(p_c, np.complex128).reshape(3, n0).T
+    E = np.asarray(e_c, np.complex128).reshape(3, n0).T
+    r = site_positions(occ, nx, ny, dx, x0)
+    u = e02 / (8.0 * np.pi)
+    # incident-beam torque
+    imPE = np.imag(np.sum(np.conj(P) * E, axis=1))                    # Im(P^* . E) per site
+    orb = -0.5 * np.cross(np.sum(imPE[:, None] * r, axis=0), ak)     # sum_j r_j x F_j
+    spin = 0.5 * np.real(np.sum(np.cross(P, np.conj(E)), axis=0))
+    g_ext = orb + spin
+    # radiated recoil
+    dirs, w = scat_integration_dirs(ak, e01, etasca, n0 if nat0 is None else nat0)
+    g_rad = np.zeros(3)
+    for i0 in range(0, len(dirs), 256):
+        d = dirs[i0:i0 + 256]
+        ph = np.exp(-1j * (d @ r.T))                                   # [nd, nat0]
+        S = ph @ P                                                     # [nd, 3]
+        R = np.einsum("dj,ja,jb->dab", ph, r, P)                       # [nd, 3, 3]
+        n = d / k
+        Sp = S - n * np.sum(n * S, axis=1, keepdims=True)
+        t_spin = 0.5 * np.real(-1j * np.cross(S, np.conj(Sp)))
+        t_orb = -0.5 * k * np.cross(np.real(np.einsum("dab,db->da", np.conj(R), Sp)), n)
+        g_rad += (k ** 3 / (4 * np.pi)) * np.sum(w[i0:i0 + 256, 0:1] * (t_spin + t_orb), axis=0)
+    c = k / u
+    return dict(ext=g_ext * c, rad=g_rad * c, total=(g_ext + g_rad) * c, navg=len(dirs))
+
+
+def torque_dense_mechanics(p_c, e_c, occ, nx, ny, nz, ak, x0, e02=1.0, h=1e-5):
+    """Torque cross section from the mechanics of every dipole in the total
+    field (test pin of torque_sums, small targets): Gamma = sum_j [ r_j x F_j
+    + (1/2) Re(P_j x E_j^*) ], with E_j = E_inc + sum_{l != j} G(r_j - r_l) P_l
+    + (2/3) i k^3 P_j (radiation reaction) and F_j = (1/2) Re sum_i P_ji^*
+    grad E_i(r_j), the pair-field gradients by central differences of the
+    closed-form tensor (step h).  Reactive pair forces cancel in the sum;
+    what remains equals torque_sums' ext + rad."""
+    from .ddscat_oracle import green_elements, COMP_IJ
+    ak = np.asarray(ak, np.float64)
+    k = float(np.linalg.norm(ak))
+    n0 = np.asarray(p_c).size // 3
+    P = np.asarray(p_c, np.complex128).reshape(3, n0).T
+    E = np.asarray(e_c, np.complex128).reshape(3, n0).T
+    occ = np.asarray(occ)
+    ix, iy, iz = occ % nx, (occ // nx) % ny, occ // (nx * ny)
+    r = np.stack([ix + 1 + x0[0], iy + 1 + x0[1], iz + 1 + x0[2]], 1).astype(np.float64)
+
+    def tensor(dv):
+        g = green_elements(dv[..., 0], dv[..., 1], dv[..., 2], k)
+        T = np.zeros(dv.shape[:-1] + (3, 3), np.complex128)
+        for c, (a, b) in enumerate(COMP_IJ):
+            T[..., a, b] = g[c]
+            T[..., b, a] = g[c]
+        return T
+
+    dvec = r[:, None, :] - r[None, :, :]                               # r_j - r_l
+    field = np.einsum("jlab,lb->ja", tensor(dvec), P)                  # zero for l == j
+    Ej = E + field + (2.0 / 3.0) * 1j * k ** 3 * P
+    grad = np.zeros((n0, 3, 3), np.complex128)                         # [j, m, i] d_m E_i
+    for m in range(3):
+        sh = np.zeros(3); sh[m] = h
+        off = ~np.eye(n0, dtype=bool)
+        gp = np.einsum("jlab,lb->ja", tensor(dvec + sh) * off[..., None, None], P)
+        gm = np.einsum("jlab,lb->ja", tensor(dvec - sh) * off[..., None, None], P)
+        grad[:, m, :] = (gp - gm) / (2 * h) + 1j * ak[m] * E
+    F = 0.5 * np.real(np.einsum("ji,jmi->jm", np.conj(P), grad))
+    gam = np.sum(np.cross(r, F), axis=0) + 0.5 * np.real(np.sum(np.cross(P, np.conj(Ej)), axis=0))
+    return gam * k / (e02 / (8.0 * np.pi))
