@@ -797,6 +797,9 @@ int bsolve_qmrccg(Context& c, BatchState& S, const float2* b, float2* x, const f
   Graph gr;
   // true residual / gap for the RHS in `sel`; returns per-RHS outcomes
   std::vector<double> slots;
+  std::vector<int> conv(B), rst(B), rplc(B, 0);
+  std::vector<double> tres(B, 0.0);
+
   auto reliable = [&](std::vector<int>& conv, std::vector<int>& rst, std::vector<double>& tres) -> int {
     if (o.upload(S.sel, sel)) return 59;
     if (int rc = o.mv(xs, sy, Sm)) return rc;
@@ -815,14 +818,14 @@ int bsolve_qmrccg(Context& c, BatchState& S, const float2* b, float2* x, const f
       const double gap = tr > 0.0 ? gp / tr : (gp == 0.0 ? 0.0 : INFINITY);
       conv[k] = tres[k] <= tol;
       rst[k] = !conv[k] && (gap >= RELIABLE_GAP || resid[k] <= tol);
-      const int action = !rst[k] || it[k] >= maxit ? 0
-                         : c.reliable_mode == RELIABLE_REPLACE ? SolveLog::REPLACE : SolveLog::RESTART;
+      int action = 0;
+      if (rst[k] && it[k] < maxit)
+        action = c.reliable_mode == RELIABLE_REPLACE ? SolveLog::REPLACE : SolveLog::RESTART;
+      rplc[k] = action == SolveLog::REPLACE;
       c.log.check(k, tres[k], gap, action);
     }
     return 0;
   };
-  std::vector<int> conv(B), rst(B);
-  std::vector<double> tres(B, 0.0);
   auto body = [&]() -> int {
     if (o.f64) hipLaunchKernelGGL(kb_qmr_pq<double>, o.vgrid(), dim3(BT), 0, o.st, p, q, vt, atw, S.scal, S.n, A);
     else hipLaunchKernelGGL(kb_qmr_pq<float>, o.vgrid(), dim3(BT), 0, o.st, p, q, vt, atw, S.scal, S.n, A);
@@ -871,7 +874,7 @@ int bsolve_qmrccg(Context& c, BatchState& S, const float2* b, float2* x, const f
           resid[k] = tres[k];
           act[k] = 0;
         } else if (rst[k]) {
-          (c.reliable_mode == RELIABLE_REPLACE ? replace : restart)[k] = 1;
+          (rplc[k] ? replace : restart)[k] = 1;
           resid[k] = tres[k];
         }
       }

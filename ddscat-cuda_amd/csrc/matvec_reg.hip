@@ -327,25 +327,58 @@ __device__ __forceinline__ int p3_row(int j, int ny, int gy, int k0) {
   return j == 0 ? 0 : j == 1 ? ny : ((j & 1) ? gy - ((j - 2) / 2 + 1) : (j - 2) / 2 + 1);
 }
 
+// Pass-3 grid: one dimension of ceil(U/8)*8*B workgroups over the U = NT x GY
+// column units (NT column tiles, GY rows) and B right-hand sides.  Workgroups
+// go round-robin over the 8 XCDs by linear id, so unit l = 8 s' + x of RHS r
+// is workgroup (s' B + r) 8 + x: it runs on XCD x, and the B right-hand sides
+// of a unit run back to back there, so the unit's Green octant entries come
+// from HBM once and from that XCD's L2 for the other B - 1.  Units keep the
+// single-RHS order l = bx + NT by (the mirror locality below).  False for the
+// padding workgroups of the last group of eight.
+struct P3Grid {
+  int nt, units, nrhs;
+};
+struct P3Unit {
+  int bx, by, rb;
+};
+__device__ __forceinline__ bool p3_unit(const P3Grid& g3, P3Unit& u) {
+  const int L = (int)blockIdx.x, x = L & 7, s = L >> 3;
+  const int sp = s / g3.nrhs, l = sp * 8 + x;
+  if (l >= g3.units) return false;
+  u.rb = s - sp * g3.nrhs;
+  u.by = l / g3.nt;
+  u.bx = l - u.by * g3.nt;
+  return true;
+}
+static P3Grid p3_grid(int gx, int gy, int C, int nrhs) {
+  P3Grid g3;
+  g3.nt = (gx + C - 1) / C;
+  g3.units = g3.nt * gy;
+  g3.nrhs = nrhs;
+  return g3;
+}
+static unsigned p3_blocks(const P3Grid& g3) { return (unsigned)(((g3.units + 7) / 8) * 8 * g3.nrhs); }
+
 template <int R1, int R2, int C>
 __global__ __launch_bounds__(R1 * C) void kr_pass3(float2* __restrict__ a2, const float2* __restrict__ green,
                                                const float2* __restrict__ twg, int nx, int ny, int nz, int gx,
-                                               int gy, size_t plane, BatchArgs ba, GreenGeo gg) {
+                                               int gy, size_t plane, BatchArgs ba, GreenGeo gg, P3Grid g3) {
   constexpr int L = R1 * R2, RT = R1 * C, P = C + 1, G = Reg2<R1, R2>::G, VN = Reg2<R1, R2>::VN;
   __shared__ float2 twL[L];
   __shared__ float2 buf[L * P];
-  const int rb = blockIdx.z;
+  P3Unit u;
+  if (!p3_unit(g3, u)) return;
+  const int rb = u.rb;
   if (ba.active && !ba.active[rb]) return;
   a2 += rb * ba.a2s;
   const int s = threadIdx.x % C, t = threadIdx.x / C;
-  // Grid order puts Green-sharing blocks on one XCD (workgroups go round-robin
-  // over the 8 XCDs by linear id): ky runs 0, ny, 1, gy-1, 2, gy-2, ... so
-  // mirror rows are one grid row apart, and column tile b and its mirror
-  // NT-1-b are h = ceil(NT/2) grid columns apart (NT = column tiles).
-  const int j = blockIdx.y;
+  // Unit order puts Green-sharing blocks on one XCD: ky runs 0, ny, 1, gy-1,
+  // 2, gy-2, ... so mirror rows are one unit row apart, and column tile b and
+  // its mirror NT-1-b are h = ceil(NT/2) unit columns apart.
+  const int j = u.by;
   const int ky = j == 0 ? 0 : j == 1 ? ny : ((j & 1) ? gy - ((j - 2) / 2 + 1) : (j - 2) / 2 + 1);
-  const int NT = gridDim.x, h = (NT + 1) / 2;
-  const int tile = (int)blockIdx.x < h ? (int)blockIdx.x : NT - 1 - ((int)blockIdx.x - h);
+  const int NT = g3.nt, h = (NT + 1) / 2;
+  const int tile = u.bx < h ? u.bx : NT - 1 - (u.bx - h);
   const int kx = tile * C + s;
   for (int i = threadIdx.x; i < L; i += RT) twL[i] = twg[i];
   const bool live = kx < gx;
@@ -591,7 +624,7 @@ __global__ __launch_bounds__(3 * R1 * C) void kr_pass3c(float2* __restrict__ a2,
 template <int M, int C, bool SLY = false>
 __global__ __launch_bounds__(M * 16 * C) void kr_pass3m(float2* __restrict__ a2, const float2* __restrict__ green,
                                                     const float2* __restrict__ twg, int nx, int ny, int nz, int gx,
-                                                    int gy, size_t plane, BatchArgs ba, GreenGeo gg) {
+                                                    int gy, size_t plane, BatchArgs ba, GreenGeo gg, P3Grid g3) {
   constexpr int R = 16, S = R * R, L = M * S, GT = R * C, RT = M * GT;
   constexpr int PS = M * C + 1;  // 256-point exchange: M C sequences per workgroup
   constexpr int PC = C + 1;      // M-point exchange layout (u S + k1) PC + s, u < M
@@ -599,18 +632,25 @@ __global__ __launch_bounds__(M * 16 * C) void kr_pass3m(float2* __restrict__ a2,
   __shared__ float2 twL[L];
   __shared__ float2 twS[S];
   __shared__ float2 xb[XB];
-  const int rb = blockIdx.z;
+  // FFT3D: the unit grid of kr_pass3; SLICES: the batch's 3-D grid
+  P3Unit u{(int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z};
+  int NT = (int)gridDim.x;
+  if (!SLY) {
+    if (!p3_unit(g3, u)) return;
+    NT = g3.nt;
+  }
+  const int rb = u.rb;
   if (ba.active && !ba.active[rb]) return;
   a2 += rb * ba.a2s;
   const int m = threadIdx.x / GT, lt = threadIdx.x % GT;
   const int s = lt % C, t = lt / C, sv = m * C + s;
-  // XCD-aware grid order as in kr_pass3
-  const int tile = xcd_tile<C>((int)blockIdx.x, (int)gridDim.x, true);
+  // XCD-aware unit order as in kr_pass3
+  const int tile = xcd_tile<C>(u.bx, NT, true);
   const int kx = tile * C + s;
   for (int i = threadIdx.x; i < L; i += RT) twL[i] = twg[i];
   for (int i = threadIdx.x; i < S; i += RT) twS[i] = twg[M * i];
   const bool live = kx < gx;
-  const P3Col q = p3_col<SLY>(a2, kx, p3_row<SLY>((int)blockIdx.y, ny, gy, (int)plane), nx, ny, nz, gx, plane, gg);
+  const P3Col q = p3_col<SLY>(a2, kx, p3_row<SLY>(u.by, ny, gy, (int)plane), nx, ny, nz, gx, plane, gg);
   float2 v[3][R];
 #pragma unroll
   for (int n2 = 0; n2 < R; ++n2) {
@@ -1131,10 +1171,12 @@ static void launch_reg(Context& c, int cj, int compact, const float2* x, float2*
       hipLaunchKernelGGL((kr_passy<R1, R2, C, +1>), dim3((m.gx + C - 1) / C, c.nz, 3 * nb), dim3(RT), 0, st,
                          bt.a1, bt.a2, m.twy, m.gx, c.ny, m.gy, c.nz, m.p1, m.p2, bt.k);
       break;
-    case 3:
-      hipLaunchKernelGGL((kr_pass3<R1, R2, C>), dim3((m.gx + C - 1) / C, m.gy, nb), dim3(RT), 0, st, bt.a2,
-                         c.green, m.twz, c.nx, c.ny, c.nz, m.gx, m.gy, m.p2, bt.k, green_geo(c));
+    case 3: {
+      const P3Grid g3 = p3_grid(m.gx, m.gy, C, (int)nb);
+      hipLaunchKernelGGL((kr_pass3<R1, R2, C>), dim3(p3_blocks(g3)), dim3(RT), 0, st, bt.a2, c.green, m.twz, c.nx,
+                         c.ny, c.nz, m.gx, m.gy, m.p2, bt.k, green_geo(c), g3);
       break;
+    }
     case 4:
       hipLaunchKernelGGL((kr_passy<R1, R2, C, -1>), dim3((m.gx + C - 1) / C, c.nz, 3 * nb), dim3(RT), 0, st,
                          bt.a2, bt.a1, m.twy, m.gx, m.gy, c.ny, c.nz, m.p2, m.p1, bt.k);
@@ -1228,8 +1270,9 @@ static void launch_p3m(Context& c, int cj, int compact, const float2* x, float2*
   (void)cj; (void)compact; (void)x; (void)y; (void)adia; (void)aoff;
   if (which != 3) return;
   const MatvecPlan& m = c.mv;
-  hipLaunchKernelGGL((kr_pass3m<M, C>), dim3((m.gx + C - 1) / C, m.gy, (unsigned)bt.nrhs), dim3(M * 16 * C), 0, st,
-                     bt.a2, c.green, m.twz, c.nx, c.ny, c.nz, m.gx, m.gy, m.p2, bt.k, green_geo(c));
+  const P3Grid g3 = p3_grid(m.gx, m.gy, C, bt.nrhs);
+  hipLaunchKernelGGL((kr_pass3m<M, C>), dim3(p3_blocks(g3)), dim3(M * 16 * C), 0, st, bt.a2, c.green, m.twz, c.nx,
+                     c.ny, c.nz, m.gx, m.gy, m.p2, bt.k, green_geo(c), g3);
 }
 
 // pass 3 by component-split thread groups (kr_pass3c); serves pass 3 only
@@ -1485,7 +1528,7 @@ template <int M, int C>
 static void launch_sly_m(Context& c, float2* sl, int k0, int nb, hipStream_t st) {
   const MatvecPlan& m = c.mv;
   hipLaunchKernelGGL((kr_pass3m<M, C, true>), dim3((m.gx + C - 1) / C, nb, 1), dim3(M * 16 * C), 0, st, sl, c.green,
-                     m.twy, c.nx, c.ny, c.nz, m.gx, m.gy, (size_t)k0, BatchArgs{}, green_geo(c));
+                     m.twy, c.nx, c.ny, c.nz, m.gx, m.gy, (size_t)k0, BatchArgs{}, green_geo(c), P3Grid{});
 }
 using SlyLaunch = void (*)(Context&, float2*, int, int, hipStream_t);
 struct SlyShape {

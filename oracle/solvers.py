@@ -130,9 +130,10 @@ def pbcgs2(mv, b, x0=None, maxit=300, tol=1e-5):
 def qmrccg(mv, b, x0=None, maxit=300, tol=1e-5, policy="replace", log=None):
     """Complex-symmetric QMR (Freund) with unconjugated bilinear forms.
     policy: what a failed reliable check does, "replace" (the true residual
-    replaces the recursive one, the recurrence goes on; the library's
+    replaces the recursive one and the recurrence goes on; the library's
     default) or "restart" (the reference's restart, cu:543-556).  log: an
-    optional list that receives (iteration, true residual, gap, action)."""
+    optional list that receives (iteration, true residual, gap, action 0 /
+    1 restart / 2 replace)."""
     b = np.asarray(b, np.complex128)
     n = b.size
     x = np.zeros(n, complex) if x0 is None else np.asarray(x0, np.complex128).copy()
@@ -181,15 +182,16 @@ def qmrccg(mv, b, x0=None, maxit=300, tol=1e-5, policy="replace", log=None):
         if it % RELIABLE_PERIOD == 0 or resid <= tol:
             rprev = r
             ax, tres, conv, restart, nc = _reliable(mv, b, x, r, bnorm, resid, tol, nc)
+            replace = restart and policy == "replace"
             if log is not None:
                 gap = np.linalg.norm(b - ax - rprev) / max(np.linalg.norm(b - ax), 1e-300)
-                log.append((it, tres, gap, 0 if (conv or not restart) else (2 if policy == "replace" else 1)))
+                log.append((it, tres, gap, 0 if (conv or not restart) else (2 if replace else 1)))
             if conv:
                 resid = tres
                 break
             if restart:
                 r = b - ax
-                if policy != "replace":
+                if not replace:
                     vt, wt, atw, S, p, ap, q, d, sv = init(r)
                 resid = tres
     if resid > tol:

@@ -1,6 +1,6 @@
 """Writes the FP64-oracle fixtures of the full-size configuration tests.
 
-    python tests/golden/make_config_golden.py [block128] [ellipsoids] [design64]
+    python tests/golden/make_config_golden.py [block128] [ellipsoids] [design64] [sphere64]
 
 Each workload (tests/workloads.py) is solved here, in complex128, by the
 oracle's pruned FFT MATVEC (oracle.ddscat_oracle.FftMatvec64, equal to the
@@ -11,8 +11,8 @@ sections, iteration counts, the gradient map of the 64^3 design) and a
 checksum of the regenerated inputs.  The reference repository ships no
 expected outputs for these workloads; these numbers are the oracle's.
 
-Run time on 8 host cores: about 30 min (block128), 3 min (ellipsoids),
-10 min (design64).
+Run time on 8 host cores: about 20 min (block128), 6 min (ellipsoids),
+6 min (design64), 10 min (sphere64).
 """
 from __future__ import annotations
 
@@ -50,26 +50,32 @@ def solve(op, b, adia, aoff, maxit=3000):
 
 
 def block128():
-    w = W.block128()
+    single_target(W.block128(), "config_block128.npz")
+
+
+def sphere64():
+    single_target(W.sphere64(), "config_sphere64.npz")
+
+
+def single_target(w, fname):
     tg, kd, khat = w["tg"], w["kd"], w["khat"]
     ak = (khat * kd).astype(np.float32)
     akd32 = float(np.linalg.norm(ak.astype(np.float64)))
-    log("block128: green octant", tg.nx)
+    log(fname, "green octant", tg.nx)
     go = O.green_octant_separable(tg.nx, tg.ny, tg.nz, akd32)
     op = O.FftMatvec64(go, tg.nx, tg.ny, tg.nz, tg.iocc, workers=WORKERS)
     geo = W.geometric_cross_section(tg)
     out = dict(qext=[], qabs=[], qpha=[], iterations=[], true_residual=[])
     for k, (b, adia, aoff) in enumerate(w["rhs"]):
-        log("block128: solve polarization", k)
+        log(fname, "solve polarization", k)
         x, it, tr = solve(op, b.astype(np.complex128), adia.astype(np.complex128), aoff.astype(np.complex128))
         cabs, cext, cpha = O.evalq(b, x, adia, aoff, ak, 1.0)
         out["qext"].append(cext / geo); out["qabs"].append(cabs / geo); out["qpha"].append(cpha / geo)
         out["iterations"].append(it); out["true_residual"].append(tr)
-        log(f"block128: pol {k}: {it} iterations, residual {tr:.2e}, Qext {cext / geo:.9f} Qabs {cabs / geo:.9f}")
+        log(f"{fname}: pol {k}: {it} iterations, residual {tr:.2e}, Qext {cext / geo:.9f} Qabs {cabs / geo:.9f}")
     sums = W.checksum(*[a for r in w["rhs"] for a in r])
-    np.savez(HERE / "config_block128.npz", checksum=np.array(sums), tol=np.array(TOL),
-             **{k: np.array(v) for k, v in out.items()})
-    log("wrote config_block128.npz")
+    np.savez(HERE / fname, checksum=np.array(sums), tol=np.array(TOL), **{k: np.array(v) for k, v in out.items()})
+    log("wrote", fname)
 
 
 def ellipsoids():

@@ -1,10 +1,12 @@
 """Host-side checks of the drop-in boundary: both libraries load, export
 exactly what include/ddscat_matvec_hip.h declares, and answer the host-only
-error paths; the reference loader contract (1001 / 1002 / -777) holds.
+error paths; the reference's own loader (built from its source by
+oracle/Makefile) binds them and keeps its contract (1001 / 1002 / -777).
 No compute call reaches the GPU here."""
 import os
 import re
 import subprocess
+import sys
 from pathlib import Path
 
 import pytest
@@ -60,12 +62,36 @@ def test_host_side_error_paths_without_device():
     assert L.info()["nat"] == 0
 
 
-def run_loader(dll):
-    exe = LOADER / "_bin" / "loader_compat"
-    if not exe.exists():
-        subprocess.run(["make", "-C", str(LOADER)], check=True, capture_output=True)
+REF_LOADER = ROOT / "oracle" / "_ref" / "libddscat_cuda_loader.so"
+
+# One loader per process (it keeps the first library it loads): each probe
+# runs in a child python that binds the reference's own loader, compiled from
+# ddscat_cuda_unified/cuda/ddscat_cuda_loader.c by oracle/Makefile, and makes
+# only calls the plugin answers on the host.
+PROBE = r"""
+import ctypes as C, sys
+L = C.CDLL(sys.argv[1])
+L.ddscat_cuda_loader_last_error.restype = C.c_char_p
+f, i = C.c_float, C.c_int
+L.ddscat_cuda_loader_prepare_green_f32.argtypes = [i] * 7 + [f] * 9 + [C.c_void_p]
+print("prepare_green_invalid", L.ddscat_cuda_loader_prepare_green_f32(0, 2, 2, 0, 8, 8, 0, 0, 0, 0, .1, 0, 0, 1, 1, 1, None),
+      flush=True)
+if len(sys.argv) > 2:
+    sys.exit(0)
+print("apply_unprepared", L.ddscat_cuda_loader_apply_f32(78, None, None, None, None))
+L.ddscat_cuda_loader_solve_pbcgs2_f32.argtypes = [C.c_void_p] * 4 + [i, i, f] + [C.c_void_p] * 3
+print("pbcgs2_unprepared", L.ddscat_cuda_loader_solve_pbcgs2_f32(None, None, None, None, 24, 10, 1e-5, None, None, None))
+print("last_error", L.ddscat_cuda_loader_last_error().decode())
+print("release", L.ddscat_cuda_loader_release())
+"""
+
+
+def run_loader(dll, first_call_only=False):
+    if not REF_LOADER.exists():
+        pytest.skip("oracle/_ref/libddscat_cuda_loader.so not built (needs the reference source; oracle/Makefile)")
     env = dict(os.environ, DDSCAT_CUDA_DLL=str(dll))
-    out = subprocess.run([str(exe)], capture_output=True, text=True, env=env, timeout=60).stdout
+    cmd = [sys.executable, "-c", PROBE, str(REF_LOADER)] + (["first"] if first_call_only else [])
+    out = subprocess.run(cmd, capture_output=True, text=True, env=env, timeout=120).stdout
     return dict(l.split(" ", 1) for l in out.strip().splitlines() if " " in l)
 
 
@@ -76,10 +102,17 @@ def test_reference_loader_binds_both_libraries():
         assert r["apply_unprepared"] == "10"
         assert r["pbcgs2_unprepared"] == "90"
         assert "invalid argument" in r["last_error"]
+        assert r["release"] == "0"
 
 
 def test_reference_loader_failure_codes():
-    assert run_loader("/nonexistent/libddscat_matvec_cuda.so")["prepare_green_invalid"] == "1001"
+    """The reference loader's own contract: 1001 when dlopen fails, 1002 when
+    a required symbol is missing, -777 when only prepare_green is absent (the
+    Fortran bridge then builds the Green tensor on the CPU).  After a
+    missing-symbol failure the reference loader keeps the library handle and
+    a second call goes through a NULL function pointer, so each probe here
+    makes one call (the Fortran driver stops at the first nonzero code)."""
+    assert run_loader("/nonexistent/libddscat_matvec_cuda.so", True)["prepare_green_invalid"] == "1001"
     subprocess.run(["make", "-C", str(LOADER)], check=True, capture_output=True)
-    assert run_loader(LOADER / "_bin" / "libstub_missing.so")["prepare_green_invalid"] == "1002"
-    assert run_loader(LOADER / "_bin" / "libstub_nogreen.so")["prepare_green_invalid"] == "-777"
+    assert run_loader(LOADER / "_bin" / "libstub_missing.so", True)["prepare_green_invalid"] == "1002"
+    assert run_loader(LOADER / "_bin" / "libstub_nogreen.so", True)["prepare_green_invalid"] == "-777"

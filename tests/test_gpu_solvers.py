@@ -197,24 +197,35 @@ def test_solve_device_matches_host_entry(lib, small, name):
     assert cext > 0 and cabs > 0
 
 
-@pytest.mark.parametrize("name", ["pbcgs2", "qmrccg", "gpbicg", "bicgstab4"])
-def test_forced_reliable_restart_still_converges(lib, small, name):
+@pytest.mark.parametrize("name,mode", [("pbcgs2", "restart"), ("qmrccg", "restart"), ("qmrccg", "replace"),
+                                       ("gpbicg", "restart"), ("bicgstab4", "restart")])
+def test_forced_reliable_restart_still_converges(lib, small, name, mode):
     """The reliable-residual policy (cu:543-556): a gap >= 1e-3 at a check
-    restarts the recursion from the true residual.  The hook forces one such
-    gap; the solve must restart and still reach the dense solution."""
+    restarts the recursion from the true residual (QMRCCG in the reference's
+    restart mode; in its default mode it replaces the recursive residual and
+    keeps the recurrence).  The hook forces one such gap; the solve must act
+    on it and still reach the dense solution."""
     P = small
     tg = P["tg"]
     lib.prepare_green(tg.nx, tg.ny, tg.nz, tg.iocc, P["khat"] * P["kd"])
     tol = 1e-6
-    x0, it0, *_ = lib.solve(name, P["b"], P["adia"], P["aoff"], maxit=300, tol=tol)
-    lib.force_restart()
-    x, it, tole, nc, rc = lib.solve(name, P["b"], P["adia"], P["aoff"], maxit=300, tol=tol)
+    lib.set_reliable_mode(mode)
+    try:
+        x0, it0, *_ = lib.solve(name, P["b"], P["adia"], P["aoff"], maxit=300, tol=tol)
+        lib.force_restart()
+        x, it, tole, nc, rc = lib.solve(name, P["b"], P["adia"], P["aoff"], maxit=300, tol=tol)
+    finally:
+        lib.set_reliable_mode("replace")
     restarts = lib.info()["restarts"]
+    actions = lib.solve_log(0)["checks"][:, 3].astype(int)
     xo = x.astype(complex)[P["occ3"]]
     err = np.linalg.norm(xo - P["xref"]) / np.linalg.norm(P["xref"])
-    print(f"{name}: restarts={restarts} it={it} (unforced {it0}) err={err:.2e}")
+    print(f"{name} ({mode}): restarts={restarts} actions={actions.tolist()} it={it} (unforced {it0}) err={err:.2e}")
     assert rc == 0
-    assert restarts >= 1
+    if mode == "restart":
+        assert restarts >= 1 and np.any(actions == 1)
+    else:
+        assert restarts == 0 and np.any(actions == 2)
     assert err <= 30 * tol
 
 

@@ -12,6 +12,8 @@ generated inputs is stored with each fixture and compared before use.
              random SO(3) rotations, kd = 0.1
   design64   64^3 block with a smooth random permittivity in [1,4] + 0.05i,
              +z incidence, x = 6, J = mean Q_abs of both polarizations
+  sphere64   sphere D=64 with the large-target material (m = 1.7+0.05i,
+             kd = 0.25, |m| kd = 0.43), +x incidence, E || y and z
 """
 from __future__ import annotations
 
@@ -56,6 +58,20 @@ def block128(edge=128, m=1.5 + 0.1j, xsize=10.0):
     compact arrays built on the host (LDR of Draine & Goodman 1993)."""
     tg = T.block(edge)
     kd = T.kd_for_size_parameter(tg, xsize)
+    khat = np.array([1.0, 0.0, 0.0])
+    pols = [np.array([0, 1, 0], complex), np.array([0, 0, 1], complex)]
+    rhs = []
+    for e0 in pols:
+        adia, aoff = T.operators(tg, m, kd, khat, e0)
+        rhs.append((incident_compact(tg, khat * kd, e0), compact(tg, adia), compact(tg, aoff)))
+    return dict(tg=tg, kd=kd, khat=khat, pols=pols, m=m, rhs=rhs)
+
+
+# ---------------------------------------------------------------- sphere64 --
+def sphere64(diameter=64, m=1.7 + 0.05j, kd=0.25):
+    """A mid-size sphere with the D=384 workload's material and lattice
+    spacing (the same slow QMRCCG convergence, at 1/216 of the dipoles)."""
+    tg = T.sphere(diameter)
     khat = np.array([1.0, 0.0, 0.0])
     pols = [np.array([0, 1, 0], complex), np.array([0, 0, 1], complex)]
     rhs = []
