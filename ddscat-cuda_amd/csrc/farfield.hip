@@ -657,3 +657,50 @@ extern "C" DDSCAT_HIP_API int ddscat_hip_torque(const float* ak, const void* e01
                                                 int* navg) {
   return torque(ctx(), ak, (const float2*)e01, e02, etasca, dx, x0, cext, crad, navg);
 }
+
+// ---- allocation-free SCAT ---------------------------------------------------
+// The far-field workspace of ddscat_hip_scat_f32_noalloc lives in this
+// file-scope device array, one copy per GPU, held until the process ends.
+// Host code never uses the array's name as an address: its device address
+// is looked up once per device with hipGetSymbolAddress and kept in
+// g_scratch_addr.  Every call on a GPU shares that GPU's copy (calls that
+// overlap on one GPU race); nothing in it needs to start at zero (every
+// kernel writes what the next one reads).
+namespace {
+constexpr size_t SCAT_SCRATCH_BYTES = (size_t)64 << 20;
+__device__ __align__(256) unsigned char scratch[SCAT_SCRATCH_BYTES];
+constexpr int SCRATCH_MAX_DEVICES = 64;
+void* g_scratch_addr[SCRATCH_MAX_DEVICES] = {};
+
+int scratch_region(FarfieldRegion& r) {
+  int dev = 0;
+  DDH_TRY(hipGetDevice(&dev), "SCAT scratch", 1);
+  if (dev < 0 || dev >= SCRATCH_MAX_DEVICES) { set_error("SCAT scratch", "device index out of range"); return 1; }
+  if (!g_scratch_addr[dev]) {
+    void* ptr = nullptr;
+    DDH_TRY(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "SCAT scratch address", 1);
+    if (!ptr) { set_error("SCAT scratch", "no device address for the scratch array"); return 1; }
+    g_scratch_addr[dev] = ptr;
+  }
+  r.base = (unsigned char*)g_scratch_addr[dev];
+  r.bytes = SCAT_SCRATCH_BYTES;
+  return 0;
+}
+}  // namespace
+
+extern "C" DDSCAT_HIP_API int ddscat_hip_scat_f32_noalloc(int jpbc, int ndir, const float* ak, const float* aks,
+                                                          const float* dx, const float* em1, const float* em2,
+                                                          float e02, float etasca, const void* e01, const float* x0,
+                                                          double* cbksca, double* csca, double* cscag,
+                                                          double* cscag2, int* navg, void* f1, void* f2) {
+  Context& c = ctx();
+  if (!c.ready) {
+    clear_error();
+    set_error("GPU SCAT", "invalid state/argument");
+    return 320;
+  }
+  FarfieldRegion r;
+  if (scratch_region(r)) return 335;
+  return scat(c, jpbc, ndir, ak, aks, dx, em1, em2, e02, etasca, e01, x0, cbksca, csca, cscag, cscag2, navg, f1, f2,
+              &r);
+}

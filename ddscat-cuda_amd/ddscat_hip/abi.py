@@ -32,7 +32,7 @@ EXTENSION_SYMBOLS = [
     "ddscat_hip_solve_batch", "ddscat_hip_apply_batch", "ddscat_hip_evalq_batch",
     "ddscat_hip_cabs_adjoint", "ddscat_hip_setup_rhs", "ddscat_hip_set_autotune",
     "ddscat_hip_nearfield", "ddscat_hip_set_pass_kernel", "ddscat_hip_scat_matrix", "ddscat_hip_force_restart",
-    "ddscat_hip_solve_log", "ddscat_hip_set_polarizability",
+    "ddscat_hip_solve_log", "ddscat_hip_set_polarizability", "ddscat_hip_scat_f32_noalloc",
     "ddscat_hip_torque", "ddscat_hip_set_reliable_mode",
 ]
 # method ids of ddscat_hip_solve_device
@@ -101,6 +101,8 @@ class Library:
         L.ddscat_cuda_evalq_f32.restype = i
         L.ddscat_cuda_scat_f32.argtypes = [i, i, vp, vp, vp, vp, vp, f, f, vp, vp, vp, vp, vp, vp, vp, vp, vp]
         L.ddscat_cuda_scat_f32.restype = i
+        L.ddscat_hip_scat_f32_noalloc.argtypes = list(L.ddscat_cuda_scat_f32.argtypes)
+        L.ddscat_hip_scat_f32_noalloc.restype = i
         L.ddscat_hip_torque.argtypes = [vp, vp, f, f, vp, vp, vp, vp, vp]
         L.ddscat_hip_torque.restype = i
         L.ddscat_hip_set_reliable_mode.argtypes = [i]
@@ -301,7 +303,9 @@ class Library:
         return out[0], out[1], out[2], rc
 
     # ---------------------------------------------------------------- scat
-    def scat(self, ak, dx, x0, e01, etasca, e02=1.0, aks=None, em1=None, em2=None, jpbc=0, check=True):
+    def scat(self, ak, dx, x0, e01, etasca, e02=1.0, aks=None, em1=None, em2=None, jpbc=0, check=True,
+             noalloc=False):
+        """ddscat_cuda_scat_f32, or its allocation-free variant (noalloc=True)."""
         ak = np.ascontiguousarray(ak, dtype=np.float32)
         dx = np.ascontiguousarray(dx, dtype=np.float32)
         x0 = np.ascontiguousarray(x0, dtype=np.float32)
@@ -318,7 +322,7 @@ class Library:
         cb, cs, cg2 = C.c_double(), C.c_double(), C.c_double()
         cg = (C.c_double * 3)()
         navg = C.c_int(0)
-        fn = self.lib.ddscat_cuda_scat_f32
+        fn = self.lib.ddscat_hip_scat_f32_noalloc if noalloc else self.lib.ddscat_cuda_scat_f32
         rc = fn(jpbc, ndir, _p(ak), _p(aks), _p(dx), _p(em1), _p(em2), float(e02), float(etasca), _p(e01), _p(x0),
                 C.byref(cb), C.byref(cs), cg, C.byref(cg2), C.byref(navg), _p(f1), _p(f2))
         if check:

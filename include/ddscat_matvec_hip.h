@@ -227,6 +227,23 @@ DDSCAT_HIP_API int ddscat_hip_torque(const float *ak, const void *e01, float e02
                                      const float *dx, const float *x0, double *cext, double *crad,
                                      int *navg);
 
+/* ddscat_cuda_scat_f32 without device allocation: same arguments, results
+ * and codes.  No counterpart in the reference; for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  The far-field workspace
+ * (directions, phase tables, row partials, sums) lives in a 64 MiB
+ * file-scope device array per GPU, held until the process ends, instead of
+ * per-call allocations; directions are processed in slabs that fit it.
+ * Calls that overlap on one GPU share that array and race.  Code 334: the
+ * target has too many lattice rows for one direction's workspace to fit
+ * (NY*NZ above about 1.4 M); 335: the array's device address could not be
+ * obtained. */
+DDSCAT_HIP_API int ddscat_hip_scat_f32_noalloc(int jpbc, int ndir, const float *ak, const float *aks,
+                                               const float *dx, const float *em1, const float *em2,
+                                               float e02, float etasca, const void *e01, const float *x0,
+                                               double *cbksca, double *csca, double *cscag,
+                                               double *cscag2, int *navg, void *f1, void *f2);
+
 /* LDR polarizability prescription used by ddscat_hip_setup_rhs (DDSCAT's
  * CALPHA, ddscat_cuda_unified/fortran/reapar_cuda.f90:497-513): LATTDR
  * (Draine & Goodman 1993, default) or GKDLDR (Gutkowicz-Krusin & Draine
