@@ -58,38 +58,49 @@ def setup_workload(lib, edge=128, m=1.5 + 0.1j, xsize=10.0, seed=20261202):
     return tg, kd, bufs, (x, adia, aoff)
 
 
-def time_to_qext(lib, tg, kd, m=1.5 + 0.1j, method="qmrccg", tol=1e-5, maxit=1000, graph=True):
-    """Time to Q_ext for incidence +x with both polarizations (E || y, E || z;
-    per-polarization LDR operators), everything resident in HBM:
-      plain_fp32   - one single-RHS solve per polarization, FP32 reductions
-                     and updates (PREC_FP32), then EVALQ;
+def time_to_qext(lib, tg, kd, m=1.5 + 0.1j, method="qmrccg", tol=1e-5, maxit=1000, graph=True, n_orient=1,
+                 seed=20261202, oracle=None):
+    """Time to Q_ext of B = 2 n_orient right-hand sides, everything resident
+    in HBM.  n_orient = 1: incidence +x with E || y and E || z; otherwise
+    n_orient seeded SO(3) orientations x their two polarizations (per-RHS
+    LDR operators):
+      plain_fp32   - one single-RHS solve per RHS, FP32 reductions and
+                     updates (PREC_FP32), then EVALQ;
       mixed        - the same with FP64 reductions / scalars (PREC_MIXED);
-      batched_mixed- both polarizations in one batched solve (one HIP graph
-                     replay per iteration), then the batched EVALQ.
-    Wall time from the first solve call to all Q values on the host."""
+      batched_mixed- all B in one batched solve (one HIP graph replay per
+                     iteration), then the batched EVALQ.
+    Wall time from the first solve call to all Q values on the host.  A
+    speed-up is reported only when every RHS of both modes converged;
+    ``oracle`` (Q_ext, Q_abs per polarization of the FP64 oracle, n_orient
+    = 1) adds the batched result's relative error against it."""
     import numpy as np
     import torch
     from ddscat_hip import PREC_MIXED, PREC_FP32
     from ddscat_hip import targets as T
-    khat = np.array([1.0, 0.0, 0.0])
-    pols = [np.array([0, 1, 0], complex), np.array([0, 0, 1], complex)]
+    if n_orient == 1:
+        khat = np.array([1.0, 0.0, 0.0])
+        inc = [(khat, np.array([0, 1, 0], complex)), (khat, np.array([0, 0, 1], complex))]
+    else:
+        rots = T.random_rotations(np.random.default_rng(seed), n_orient)
+        inc = [(R[:, 0], R[:, c].astype(complex)) for R in rots for c in (1, 2)]
     occ = tg.occ
     comp = lambda v: np.ascontiguousarray(np.asarray(v).reshape(3, -1)[:, occ].reshape(-1)).astype(np.complex64)
-    bs, ads, aos = [], [], []
-    for e0 in pols:
+    bs, ads, aos, aks = [], [], [], []
+    for khat, e0 in inc:
         ad, ao = T.operators(tg, m, kd, khat, e0)
         bs.append(T_incident_compact(tg, khat * kd, e0))
-        ads.append(comp(ad)); aos.append(comp(ao))
+        ads.append(comp(ad)); aos.append(comp(ao)); aks.append((khat * kd).astype(np.float32))
     # flat float32 views of [B][3 NAT0] complex64 arrays; RHS r is row(t, r)
     dev = lambda a: torch.from_numpy(np.ascontiguousarray(a).reshape(-1).view(np.float32)).cuda()
-    B = len(pols)
+    B = len(inc)
     bd, add, aod = dev(np.stack(bs)), dev(np.stack(ads)), dev(np.stack(aos))
+    del bs, ads, aos
+    aks = np.asarray(aks, np.float32)
     xd = torch.zeros_like(bd)
     n2 = bd.numel() // B
     assert bd.dim() == 1 and n2 == 2 * 3 * tg.nat0
     row = lambda t, r: t[r * n2:(r + 1) * n2]
     geo = math.pi * (3.0 * tg.nat0 / (4.0 * math.pi)) ** (2.0 / 3.0)
-    ak = (khat * kd).astype(np.float32)
     # untimed one-iteration solves: allocate the Krylov workspaces
     lib.solve_device(method, row(bd, 0).data_ptr(), row(xd, 0).data_ptr(), row(add, 0).data_ptr(),
                      row(aod, 0).data_ptr(), maxit=1, tol=tol, check=False)
@@ -106,28 +117,79 @@ def time_to_qext(lib, tg, kd, m=1.5 + 0.1j, method="qmrccg", tol=1e-5, maxit=100
             it, tole, nc, rc = lib.solve_device(method, row(bd, r).data_ptr(), row(xd, r).data_ptr(),
                                                 row(add, r).data_ptr(), row(aod, r).data_ptr(), maxit=maxit,
                                                 tol=tol, check=False)
-            cabs, cext, _, rq = lib.evalq(ak, check=False)
+            cabs, cext, _, rq = lib.evalq(aks[r], check=False)
             per.append(dict(iterations=it, matvecs=nc, resid=float(tole), rc=rc or rq, qext=cext / geo,
                             qabs=cabs / geo))
         dt = time.perf_counter() - t0
-        out[mode] = dict(seconds=round(dt, 4), rhs=per)
+        out[mode] = dict(seconds=round(dt, 4), rhs=per if B <= 2 else None,
+                         mean_iterations=float(np.mean([p["iterations"] for p in per])),
+                         converged=all(p["resid"] <= 1.01 * tol and p["rc"] == 0 for p in per))
+        out[mode]["_per"] = per
     lib.set_precision(PREC_MIXED)
     xd.zero_()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     it, te, nc, rc = lib.solve_batch(method, B, bd.data_ptr(), xd.data_ptr(), add.data_ptr(), aod.data_ptr(),
                                      maxit=maxit, tol=tol, graph=graph, check=False)
-    q = lib.evalq_batch(B, bd.data_ptr(), xd.data_ptr(), add.data_ptr(), aod.data_ptr(), ak, check=False)
+    q = lib.evalq_batch(B, bd.data_ptr(), xd.data_ptr(), add.data_ptr(), aod.data_ptr(), aks, check=False)
     dt = time.perf_counter() - t0
-    out["batched_mixed"] = dict(seconds=round(dt, 4), graph=graph, rc=rc,
-                                rhs=[dict(iterations=int(it[r]), matvecs=int(nc[r]), resid=float(te[r]),
-                                          qext=q[r, 1] / geo, qabs=q[r, 0] / geo) for r in range(B)])
-    f, bm = out["plain_fp32"], out["batched_mixed"]
-    qdiff = max(abs(a["qext"] - b["qext"]) / abs(b["qext"]) for a, b in zip(f["rhs"], bm["rhs"]))
-    return dict(method=method, tol=tol, incidence="+x, E||y and E||z", **out,
-                speedup_batched_mixed_vs_plain_fp32=round(f["seconds"] / bm["seconds"], 3),
-                speedup_mixed_vs_plain_fp32=round(f["seconds"] / out["mixed"]["seconds"], 3),
-                qext_max_rel_diff_fp32_vs_mixed=qdiff)
+    per_b = [dict(iterations=int(it[r]), matvecs=int(nc[r]), resid=float(te[r]), qext=q[r, 1] / geo,
+                  qabs=q[r, 0] / geo) for r in range(B)]
+    out["batched_mixed"] = dict(seconds=round(dt, 4), graph=graph, rc=rc, rhs=per_b if B <= 2 else None,
+                                mean_iterations=float(np.mean(it)),
+                                converged=bool(rc == 0 and np.all(te <= 1.01 * tol)))
+    f, mx, bm = out["plain_fp32"], out["mixed"], out["batched_mixed"]
+    qdiff = max(abs(a["qext"] - b["qext"]) / abs(b["qext"]) for a, b in zip(f.pop("_per"), per_b))
+    mx.pop("_per")
+    ok = f["converged"] and bm["converged"]
+    res = dict(method=method, tol=tol, nrhs=B,
+               incidence="+x, E||y and E||z" if n_orient == 1 else f"{n_orient} seeded SO(3) orientations x 2",
+               **out,
+               speedup_batched_mixed_vs_plain_fp32=round(f["seconds"] / bm["seconds"], 3) if ok else None,
+               speedup_mixed_vs_plain_fp32=round(f["seconds"] / mx["seconds"], 3) if ok and mx["converged"] else None,
+               qext_max_rel_diff_fp32_vs_batched_mixed=qdiff)
+    if oracle is not None and n_orient == 1:
+        res["batched_vs_fp64_oracle"] = dict(
+            qext_max_rel_err=max(abs(p["qext"] - oracle["qext"][k]) / abs(oracle["qext"][k]) for k, p in enumerate(per_b)),
+            qabs_max_rel_err=max(abs(p["qabs"] - oracle["qabs"][k]) / abs(oracle["qabs"][k]) for k, p in enumerate(per_b)),
+            oracle="complex128 FFT MATVEC + QMRCCG to 1e-8 (tests/golden/config_block128.npz)")
+    del bd, add, aod, xd
+    torch.cuda.empty_cache()
+    return res
+
+
+def batched_matvec_scaling(lib, tg, bufs, sizes=(1, 2, 4, 8, 16), reps=10):
+    """Milliseconds per right-hand side of one batched MATVEC launch sequence
+    (ddscat_hip_apply_batch, compact vectors, per-RHS operators) at B = 1, 2,
+    4, ... right-hand sides of the prepared target.  The B copies of each
+    pass-3 column unit run back to back on one XCD, so the Green octant is read
+    from HBM about once per launch, not once per RHS."""
+    import torch
+    out = []
+    st = torch.cuda.ExternalStream(lib.stream())
+    n = bufs["x"].numel() // 2
+    for B in sizes:
+        x = bufs["x"].view(torch.complex64).repeat(B)
+        ad = bufs["adia"].view(torch.complex64).repeat(B)
+        ao = bufs["aoff"].view(torch.complex64).repeat(B)
+        y = torch.empty_like(x)
+        torch.cuda.synchronize()
+        for _ in range(2):
+            lib.apply_batch("N", B, x.data_ptr(), y.data_ptr(), ad.data_ptr(), ao.data_ptr())
+        lib.sync()
+        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        with torch.cuda.stream(st):
+            ev0.record(st)
+            for _ in range(reps):
+                lib.apply_batch("N", B, x.data_ptr(), y.data_ptr(), ad.data_ptr(), ao.data_ptr())
+            ev1.record(st)
+        ev1.synchronize()
+        ms = ev0.elapsed_time(ev1) / reps
+        out.append(dict(nrhs=B, ms_per_launch=round(ms, 3), ms_per_rhs=round(ms / B, 4)))
+        del x, ad, ao, y
+        torch.cuda.empty_cache()
+    assert n == 3 * tg.nat0
+    return out
 
 
 def batched_small_target(lib, n_orient=16, method="pbcgs2", tol=1e-5, maxit=300, seed=20261202):
@@ -357,6 +419,71 @@ def large_target(lib, edge=384, kd=0.25, m=1.7 + 0.05j, steps=10, warmup=2, seed
     return out
 
 
+def large_sphere_solve(lib, edge=384, kd=0.25, m=1.7 + 0.05j, tol=1e-5, maxit=1500, qmr_budget=300):
+    """The large single-target solve: sphere D=edge (384: 29.6 M dipoles,
+    padded 768^3 x 3), m = 1.7+0.05i, kd = 0.25, incidence +x with E || y and
+    E || z, inputs built on the device (ddscat_hip_setup_rhs), FFT3D.
+      pbcgs2: batched mixed PBCGS2 to tol, then the batched EVALQ: time to
+              Q_ext; both polarizations' Q_abs must agree (the lattice's
+              cubic symmetry);
+      far_field: the S-matrix on 181 theta x 4 phi directions (amplitudes and
+              Mueller matrices of the device solutions);
+      qmrccg: the configured solver, batched mixed QMRCCG, for a budget of
+              qmr_budget iterations: the residual it reaches."""
+    import numpy as np
+    import torch
+    from ddscat_hip import targets as T, scattering as DS, PREC_MIXED
+    tg = T.sphere(edge)
+    khat = np.array([1.0, 0.0, 0.0])
+    e0s = np.array([[0, 1, 0], [0, 0, 1]], np.complex64)
+    aks = np.tile((khat * kd).astype(np.float32), (2, 1))
+    x0 = tg.centroid_x0()
+    lib.set_precision(PREC_MIXED)
+    t0 = time.perf_counter()
+    lib.prepare_green(tg.nx, tg.ny, tg.nz, tg.iocc, khat * kd)
+    prep = time.perf_counter() - t0
+    n = 3 * tg.nat0
+    eps = torch.full((tg.nat0,), complex(m) ** 2, dtype=torch.complex64, device="cuda")
+    b = torch.empty((2, n), dtype=torch.complex64, device="cuda")
+    ad, ao = torch.empty_like(b), torch.empty_like(b)
+    lib.setup_rhs(aks, e0s, x0, eps.data_ptr(), b.data_ptr(), ad.data_ptr(), ao.data_ptr())
+    geo = math.pi * tg.a_eff() ** 2
+    out = dict(workload=f"sphere D={edge} ({tg.nat0} dipoles, padded {2 * edge}^3 x 3), m=1.7+0.05i, kd={kd} "
+                        f"(x={kd * tg.a_eff():.1f}), +x incidence, E||y and E||z, FFT3D", prepare_seconds=round(prep, 3))
+    x = torch.zeros_like(b)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    it, te, nc, rc = lib.solve_batch("pbcgs2", 2, b.data_ptr(), x.data_ptr(), ad.data_ptr(), ao.data_ptr(),
+                                     maxit=maxit, tol=tol, check=False)
+    q = lib.evalq_batch(2, b.data_ptr(), x.data_ptr(), ad.data_ptr(), ao.data_ptr(), aks, check=False)
+    dt = time.perf_counter() - t0
+    qabs, qext = q[:, 0] / geo, q[:, 1] / geo
+    out["pbcgs2"] = dict(seconds_to_qext=round(dt, 3), iterations=it.tolist(), resid=te.tolist(), rc=int(rc),
+                         converged=bool(rc == 0 and np.all(te <= 1.01 * tol)), qext=qext.tolist(), qabs=qabs.tolist(),
+                         qabs_rel_diff_between_polarizations=float(abs(qabs[0] - qabs[1]) / abs(qabs[0])))
+    theta = np.radians(np.arange(181.0))
+    phis = np.radians([0.0, 90.0, 180.0, 270.0])
+    th, ph = np.repeat(theta, 4), np.tile(phis, 181)
+    ks, epar, eperp = DS.directions(khat, e0s[0], th, ph)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    f, M = lib.scat_matrix(x.data_ptr(), 1, aks[:1], e0s[None], kd * ks, epar, eperp, x0)
+    out["far_field"] = dict(directions=int(len(th)), seconds=round(time.perf_counter() - t0, 3),
+                            s11_forward=float(M[0, 0, 0, 0]), s11_backward=float(M[0, 4 * 180, 0, 0]))
+    x.zero_()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    it, te, nc, rc = lib.solve_batch("qmrccg", 2, b.data_ptr(), x.data_ptr(), ad.data_ptr(), ao.data_ptr(),
+                                     maxit=qmr_budget, tol=tol, check=False)
+    out["qmrccg"] = dict(iteration_budget=qmr_budget, seconds=round(time.perf_counter() - t0, 3),
+                         iterations=it.tolist(), resid=te.tolist(),
+                         replacements=[int(np.sum(lib.solve_log(r)["checks"][:, 3] == 2)) for r in range(2)])
+    del b, ad, ao, x, eps
+    torch.cuda.empty_cache()
+    lib.release()
+    return out
+
+
 def adjoint_design(lib, edge=64, xsize=6.0, tol=1e-5, h=1e-2, fd_tol=1e-6, seed=20261202):
     """The inverse-design workload: a 64^3 block (262,144 dipoles) with a
     smooth random permittivity (eps_j = U[1,4] smoothed by a Gaussian of
@@ -521,6 +648,8 @@ def main():
     ap.add_argument("--backend", choices=["fft3d", "slices"], default="fft3d")
     ap.add_argument("--pmc-calib", action="store_true", help="run one torch copy of known size (rocprof scale)")
     ap.add_argument("--no-large", action="store_true", help="skip the 384^3 sphere MATVEC leg")
+    ap.add_argument("--batch-study", action="store_true",
+                    help="time to Q_ext at 2 and 16 right-hand sides on the 128^3 and 256^3 blocks, then exit")
     ap.add_argument("--large-only", action="store_true",
                     help="run only the 384^3 sphere MATVEC leg (FFT3D and SLICES) and print its JSON")
     args = ap.parse_args()
@@ -535,6 +664,19 @@ def main():
     lib = Library()
     if args.large_only:
         print(json.dumps(dict(large_target=large_target(lib, steps=args.steps, warmup=args.warmup))), flush=True)
+        return
+    if args.batch_study:
+        from ddscat_hip import targets as T
+        res = {}
+        for edge, xs in ((128, 10.0), (256, 20.0)):
+            tgb = T.block(edge)
+            kdb = T.kd_for_size_parameter(tgb, xs)
+            lib.prepare_green(tgb.nx, tgb.ny, tgb.nz, tgb.iocc, (kdb, 0.0, 0.0))
+            for no in (1, 8):
+                res[f"block{edge}_rhs{2 * no}"] = time_to_qext(lib, tgb, kdb, method=args.solver, n_orient=no)
+                print(json.dumps({f"block{edge}_rhs{2 * no}": res[f"block{edge}_rhs{2 * no}"]}), flush=True)
+            lib.release()
+        print(json.dumps(dict(batch_study=res)), flush=True)
         return
     lib.set_backend(BACKEND_SLICES if args.backend == "slices" else BACKEND_FFT3D)
     tg, kd, bufs, host = setup_workload(lib, edge=args.edge)
@@ -604,7 +746,14 @@ def main():
 
     solve = small = None
     if not args.no_solve:
-        solve = time_to_qext(lib, tg, kd, method=args.solver)
+        oracle = None
+        fx = ROOT / "tests" / "golden" / "config_block128.npz"
+        if args.edge == 128 and fx.exists():
+            f = np.load(fx, allow_pickle=False)
+            oracle = dict(qext=f["qext"], qabs=f["qabs"])
+        solve = time_to_qext(lib, tg, kd, method=args.solver, oracle=oracle)
+        solve16 = time_to_qext(lib, tg, kd, method=args.solver, n_orient=8)
+        scaling = batched_matvec_scaling(lib, tg, bufs)
 
     cpu = None
     if not args.no_cpu:
@@ -637,6 +786,9 @@ def main():
         "cpu_baseline": cpu,
     }
     if not args.no_solve:
+        out["time_to_qext_16rhs"] = solve16
+        out["batched_matvec_per_rhs"] = scaling
+    if not args.no_solve:
         # small-target batched leg last: it re-prepares the library for its own target
         out["batched_sphere32"] = batched_small_target(lib)
         out["batched_targets"] = batched_targets(lib, n_targets=args.targets)
@@ -646,6 +798,8 @@ def main():
         out["time_to_qext_256"] = large_solve(lib)
     if not args.no_large:
         out["large_target"] = large_target(lib)
+    if not args.no_large and not args.no_solve:
+        out["large_sphere_solve"] = large_sphere_solve(lib)
     print(json.dumps(out))
     lib.release()
 
