@@ -266,14 +266,18 @@ def batched_small_target(lib, n_orient=16, method="pbcgs2", tol=1e-5, maxit=300,
 
 
 def batched_targets(lib, n_targets=48, n_orient=8, kd=0.1, method="pbcgs2", tol=1e-5, maxit=500,
-                    single_sample=3, seed=20261202):
+                    single_sample=3, seed=20261202, group=8):
     """Uneven batch: n_targets seeded unions of 1-4 ellipsoids (semi-axes
-    U[6,40] dipoles), m = U[1.2,2.0] + i U[0,0.3] per target, n_orient
-    orientations uniform on SO(3) x 2 polarizations per target, PBCGS2 at
-    tol 1e-5, mixed precision.  Per target: prepare (device Green build),
-    device RHS setup (incident fields + LDR operators for all its RHS), one
-    batched solve (graph replay), batched EVALQ.  The plain-FP32 single-RHS
-    rate is measured on the first `single_sample` targets."""
+    U[6,40] dipoles, grids 24-96 per axis), m = U[1.2,2.0] + i U[0,0.3] per
+    target, n_orient orientations uniform on SO(3) x 2 polarizations per
+    target, PBCGS2 at tol 1e-5, mixed precision.  Targets go through in
+    groups of `group`: each target of a group is prepared in its own target
+    slot (device Green build) with its inputs built on the device
+    (incident fields + LDR operators of all its RHS), then one
+    ddscat_hip_solve_targets call solves the whole group (one stream and host
+    thread per target, graph replay), then the batched EVALQ per slot.  The
+    same targets one after another (solve_batch per target) and plain-FP32
+    single-RHS solves are timed on the first `single_sample` targets."""
     import numpy as np
     import torch
     from ddscat_hip import PREC_MIXED, PREC_FP32
@@ -284,66 +288,103 @@ def batched_targets(lib, n_targets=48, n_orient=8, kd=0.1, method="pbcgs2", tol=
         tg = T.ellipsoid_union(rng)
         m = complex(rng.uniform(1.2, 2.0), rng.uniform(0.0, 0.3))
         tgts.append((tg, m, T.random_rotations(rng, n_orient)))
-    rhs_total, iters, sizes = 0, [], []
-    stage = dict(prepare=0.0, setup=0.0, solve=0.0, evalq=0.0)
-    t_batch_sample, t_single, rhs_single = 0.0, 0.0, 0
-    for ti, (tg, m, rots) in enumerate(tgts):
+
+    def inputs(tg, m, rots):
         aks, e0s = [], []
         for R in rots:
-            khat, ey, ez = R[:, 0], R[:, 1], R[:, 2]
-            for e0 in (ey, ez):
-                aks.append(khat * kd); e0s.append(e0.astype(np.complex64))
+            for e0 in (R[:, 1], R[:, 2]):
+                aks.append(R[:, 0] * kd); e0s.append(e0.astype(np.complex64))
         aks = np.asarray(aks, np.float32)
-        B = len(aks)
-        n = 3 * tg.nat0
+        B, n = len(aks), 3 * tg.nat0
         eps = torch.full((tg.nat0,), m * m, dtype=torch.complex64, device="cuda")
         b = torch.empty((B, n), dtype=torch.complex64, device="cuda")
-        ad, ao, x = torch.empty_like(b), torch.empty_like(b), torch.zeros_like(b)
-        x0 = tg.centroid_x0()
+        ad, ao = torch.empty_like(b), torch.empty_like(b)
+        lib.setup_rhs(aks, np.asarray(e0s), tg.centroid_x0(), eps.data_ptr(), b.data_ptr(), ad.data_ptr(),
+                      ao.data_ptr())
+        return dict(aks=aks, B=B, b=b, ad=ad, ao=ao, x=torch.zeros_like(b))
+
+    lib.set_precision(PREC_MIXED)
+    rhs_total, iters, sizes = 0, [], []
+    stage = dict(prepare=0.0, setup=0.0, solve=0.0, evalq=0.0)
+    for g0 in range(0, n_targets, group):
+        grp = tgts[g0:g0 + group]
+        slots = list(range(1, len(grp) + 1))
+        bufs = []
+        torch.cuda.synchronize()
+        for s_, (tg, m, rots) in zip(slots, grp):
+            lib.select_target(s_)
+            t0 = time.perf_counter()
+            lib.prepare_green(tg.nx, tg.ny, tg.nz, tg.iocc, rots[0][:, 0] * kd)
+            t1 = time.perf_counter()
+            bufs.append(inputs(tg, m, rots))
+            torch.cuda.synchronize()
+            t2 = time.perf_counter()
+            stage["prepare"] += t1 - t0
+            stage["setup"] += t2 - t1
+            sizes.append((tg.nx, tg.ny, tg.nz, tg.nat0))
+        t0 = time.perf_counter()
+        per, rc = lib.solve_targets(method, slots, [d["B"] for d in bufs], [d["b"].data_ptr() for d in bufs],
+                                    [d["x"].data_ptr() for d in bufs], [d["ad"].data_ptr() for d in bufs],
+                                    [d["ao"].data_ptr() for d in bufs], maxit=maxit, tol=tol, check=False)
+        t1 = time.perf_counter()
+        for s_, d in zip(slots, bufs):
+            lib.select_target(s_)
+            lib.evalq_batch(d["B"], d["b"].data_ptr(), d["x"].data_ptr(), d["ad"].data_ptr(), d["ao"].data_ptr(),
+                            d["aks"], check=False)
+        t2 = time.perf_counter()
+        stage["solve"] += t1 - t0
+        stage["evalq"] += t2 - t1
+        for d, (it, te, nc) in zip(bufs, per):
+            rhs_total += d["B"]
+            iters.extend(int(v) for v in it)
+        for s_ in slots:
+            lib.select_target(s_)
+            lib.release()
+        lib.select_target(0)
+        del bufs
+        torch.cuda.empty_cache()
+    t_batch = sum(stage.values())
+    # the first targets, one after another: batched solve per target, and
+    # plain-FP32 single-RHS solves
+    t_seq = t_single = 0.0
+    rhs_single = 0
+    for tg, m, rots in tgts[:single_sample]:
+        lib.prepare_green(tg.nx, tg.ny, tg.nz, tg.iocc, rots[0][:, 0] * kd)
+        d = inputs(tg, m, rots)
         lib.set_precision(PREC_MIXED)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        lib.prepare_green(tg.nx, tg.ny, tg.nz, tg.iocc, aks[0])
-        t1 = time.perf_counter()
-        lib.setup_rhs(aks, np.asarray(e0s), x0, eps.data_ptr(), b.data_ptr(), ad.data_ptr(), ao.data_ptr())
-        t2 = time.perf_counter()
-        it, te, nc, rc = lib.solve_batch(method, B, b.data_ptr(), x.data_ptr(), ad.data_ptr(), ao.data_ptr(),
-                                         maxit=maxit, tol=tol, check=False)
-        t3 = time.perf_counter()
-        lib.evalq_batch(B, b.data_ptr(), x.data_ptr(), ad.data_ptr(), ao.data_ptr(), aks, check=False)
-        t4 = time.perf_counter()
-        for k, v in zip(stage, (t1 - t0, t2 - t1, t3 - t2, t4 - t3)):
-            stage[k] += v
-        rhs_total += B
-        iters.extend(int(v) for v in it)
-        sizes.append((tg.nx, tg.ny, tg.nz, tg.nat0))
-        if ti < single_sample:
-            # same target, same prepared state: B single-RHS plain-FP32 solves + EVALQ
-            t_batch_sample += (t3 - t2) + (t4 - t3)
-            lib.set_precision(PREC_FP32)
-            x.zero_()
-            torch.cuda.synchronize()
-            t0 = time.perf_counter()
-            for r in range(B):
-                lib.solve_device(method, b[r].data_ptr(), x[r].data_ptr(), ad[r].data_ptr(), ao[r].data_ptr(),
-                                 maxit=maxit, tol=tol, check=False)
-                lib.evalq(aks[r], check=False)
-            t_single += time.perf_counter() - t0
-            rhs_single += B
-        del b, ad, ao, x, eps
-    t_batch = sum(stage.values())
+        lib.solve_batch(method, d["B"], d["b"].data_ptr(), d["x"].data_ptr(), d["ad"].data_ptr(), d["ao"].data_ptr(),
+                        maxit=maxit, tol=tol, check=False)
+        lib.evalq_batch(d["B"], d["b"].data_ptr(), d["x"].data_ptr(), d["ad"].data_ptr(), d["ao"].data_ptr(),
+                        d["aks"], check=False)
+        t_seq += time.perf_counter() - t0
+        lib.set_precision(PREC_FP32)
+        d["x"].zero_()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for r in range(d["B"]):
+            lib.solve_device(method, d["b"][r].data_ptr(), d["x"][r].data_ptr(), d["ad"][r].data_ptr(),
+                             d["ao"][r].data_ptr(), maxit=maxit, tol=tol, check=False)
+            lib.evalq(d["aks"][r], check=False)
+        t_single += time.perf_counter() - t0
+        rhs_single += d["B"]
+        lib.set_precision(PREC_MIXED)
+        del d
+    lib.release()
     return dict(workload=f"{n_targets} seeded ellipsoid unions (grids {min(min(s[:3]) for s in sizes)}-"
                          f"{max(max(s[:3]) for s in sizes)} per axis, {sum(s[3] for s in sizes)} dipoles), "
                          f"{n_orient} SO(3) orientations x 2 polarizations = {rhs_total} RHS, kd={kd}, "
-                         f"{method.upper()} tol {tol}",
+                         f"{method.upper()} tol {tol}, groups of {group} targets per ddscat_hip_solve_targets call",
                 batched_mixed=dict(seconds=round(t_batch, 3), rhs_per_s=round(rhs_total / t_batch, 1),
+                                   solve_rhs_per_s=round(rhs_total / stage["solve"], 1),
                                    stage_seconds={k: round(v, 3) for k, v in stage.items()},
                                    mean_iterations=float(np.mean(iters)), max_iterations=int(np.max(iters))),
                 solve_comparison=dict(sample_targets=single_sample, rhs=rhs_single,
                                       plain_fp32_single_seconds=round(t_single, 3),
-                                      batched_mixed_seconds=round(t_batch_sample, 3),
-                                      speedup_batched_mixed_vs_plain_fp32=round(t_single / t_batch_sample, 3)
-                                      if t_batch_sample > 0 else None))
+                                      batched_mixed_one_target_at_a_time_seconds=round(t_seq, 3),
+                                      speedup_batched_mixed_vs_plain_fp32=round(t_single / t_seq, 3)
+                                      if t_seq > 0 else None))
 
 
 def large_solve(lib, edge=256, xsize=20.0):

@@ -22,6 +22,7 @@
 #include <cmath>
 #include <cstdio>
 #include <vector>
+#include <thread>
 
 namespace ddhip {
 namespace {
@@ -49,7 +50,8 @@ struct BatchState {
   int* hflags = nullptr;    // pinned [B]
   int capB = 0;
 };
-BatchState g_bs;
+BatchState g_bs[MAX_TARGETS + 1];  // one per target slot
+BatchState& bstate(const Context& c) { return g_bs[c.slot]; }
 
 void bs_free(BatchState& S) {
   for (void** p : {(void**)&S.W, (void**)&S.a1, (void**)&S.a2, (void**)&S.scal, (void**)&S.part,
@@ -1089,7 +1091,7 @@ int bsolve_pbcgs2(Context& c, BatchState& S, const float2* b, float2* x, const f
 
 }  // namespace
 
-void batch_release() { bs_free(g_bs); }
+void batch_release(Context& c) { bs_free(bstate(c)); }
 
 }  // namespace ddhip
 
@@ -1108,7 +1110,7 @@ extern "C" DDSCAT_HIP_API int ddscat_hip_apply_batch(int cwhat, int nrhs, const 
     set_error("apply_batch", "batched MATVEC runs on the FFT3D backend");
     return 11;
   }
-  BatchState& S = g_bs;
+  BatchState& S = bstate(c);
   if (int rc = bs_ensure(c, S, nrhs, 0)) return rc;
   MvBatch bt;
   bt.nrhs = nrhs;
@@ -1133,7 +1135,7 @@ extern "C" DDSCAT_HIP_API int ddscat_hip_evalq_batch(int nrhs, const void* e, co
     set_error("evalq_batch", "invalid argument");
     return 300;
   }
-  BatchState& S = g_bs;
+  BatchState& S = bstate(c);
   if (int rc = bs_ensure(c, S, nrhs, 1)) return rc;
   std::vector<float> rabs(nrhs);
   std::vector<double> akk(nrhs);
@@ -1188,7 +1190,7 @@ extern "C" DDSCAT_HIP_API int ddscat_hip_cabs_adjoint(int method, int nrhs, cons
     set_error("cabs_adjoint", "adjoint solves run batched QMRCCG or PBCGS2 on the FFT3D backend");
     return 2;
   }
-  BatchState& S = g_bs;
+  BatchState& S = bstate(c);
   // solver vectors 0..10, adjoint right-hand side 11, adjoint solution 12
   if (int rc = bs_ensure(c, S, nrhs, 13)) return rc;
   float2* rhs = S.W + (size_t)11 * nrhs * S.n;
@@ -1243,11 +1245,81 @@ extern "C" DDSCAT_HIP_API int ddscat_hip_solve_batch(int method, int nrhs, const
     set_error("solve_batch", "method not available in batched form");
     return 2;
   }
-  BatchState& S = g_bs;
+  BatchState& S = bstate(c);
   if (int rc = bs_ensure(c, S, nrhs, 11)) return rc;
   c.solution_valid = false;
   const size_t os = op_per_rhs ? S.n : 0;
   auto fn = method == DDSCAT_HIP_QMRCCG ? bsolve_qmrccg : bsolve_pbcgs2;
   return fn(c, S, (const float2*)b, (float2*)x, (const float2*)adia, (const float2*)aoff, os, maxit, (double)tol,
             itno, tole, ncompte, use_graph != 0);
+}
+
+extern "C" DDSCAT_HIP_API int ddscat_hip_select_target(int slot) {
+  if (slot < 0 || slot > MAX_TARGETS) {
+    set_error("select_target", "slot out of range");
+    return 1;
+  }
+  select_target(slot);
+  return 0;
+}
+
+extern "C" DDSCAT_HIP_API int ddscat_hip_solve_targets(int method, int ntargets, const int* slots, const int* nrhs,
+                                                       void* const* b, void* const* x, void* const* adia,
+                                                       void* const* aoff, int op_per_rhs, int maxit, float tol,
+                                                       int* itno, float* tole, int* ncompte, int use_graph) {
+  clear_error();
+  if (ntargets <= 0 || !slots || !nrhs || !b || !x || !adia || !aoff || !itno || !tole || !ncompte || maxit <= 0 ||
+      !(tol > 0.f) || (op_per_rhs != 0 && op_per_rhs != 1)) {
+    set_error("solve_targets", "invalid argument");
+    return 1;
+  }
+  if (method != DDSCAT_HIP_QMRCCG && method != DDSCAT_HIP_PBCGS2) {
+    set_error("solve_targets", "method not available in batched form");
+    return 2;
+  }
+  std::vector<size_t> first(ntargets + 1, 0);
+  for (int t = 0; t < ntargets; ++t) {
+    const int s = slots[t];
+    if (s < 0 || s > MAX_TARGETS || nrhs[t] <= 0 || !b[t] || !x[t] || !adia[t] || !aoff[t]) {
+      set_error("solve_targets", "invalid target entry");
+      return 1;
+    }
+    for (int u = 0; u < t; ++u)
+      if (slots[u] == s) { set_error("solve_targets", "a target slot appears twice"); return 1; }
+    Context& c = ctx_at(s);
+    if (!c.ready || c.backend != BACKEND_FFT3D) {
+      set_error("solve_targets", "target not prepared on the FFT3D backend");
+      return 2;
+    }
+    first[t + 1] = first[t] + (size_t)nrhs[t];
+  }
+  // workspaces first, from this thread: no allocation runs while another
+  // target's iteration is being captured
+  for (int t = 0; t < ntargets; ++t) {
+    Context& c = ctx_at(slots[t]);
+    if (int rc = bs_ensure(c, bstate(c), nrhs[t], 11)) return rc;
+    c.solution_valid = false;
+  }
+  int dev = 0;
+  DDH_TRY(hipGetDevice(&dev), "solve_targets", 1);
+  auto fn = method == DDSCAT_HIP_QMRCCG ? bsolve_qmrccg : bsolve_pbcgs2;
+  std::vector<int> rcs(ntargets, 0);
+  auto run = [&](int t) {
+    if (hipSetDevice(dev) != hipSuccess) { rcs[t] = 1; return; }
+    Context& c = ctx_at(slots[t]);
+    const size_t os = op_per_rhs ? (size_t)3 * c.nat0 : 0;
+    const size_t f = first[t];
+    rcs[t] = fn(c, bstate(c), (const float2*)b[t], (float2*)x[t], (const float2*)adia[t], (const float2*)aoff[t], os,
+                maxit, (double)tol, itno + f, tole + f, ncompte + f, use_graph != 0);
+  };
+  // every target on its own stream, driven from its own host thread, so
+  // their kernels share the GPU
+  std::vector<std::thread> pool;
+  pool.reserve(ntargets);
+  for (int t = 1; t < ntargets; ++t) pool.emplace_back(run, t);
+  run(0);
+  for (std::thread& th : pool) th.join();
+  for (int t = 0; t < ntargets; ++t)
+    if (rcs[t]) return rcs[t];
+  return 0;
 }

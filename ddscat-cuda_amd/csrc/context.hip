@@ -7,22 +7,38 @@
 #include "solver_common.hpp"
 #include <vector>
 #include <cstring>
+#include <mutex>
 
 namespace ddhip {
 
+// error text: one per process, as the reference's (several targets may
+// solve at once from worker threads, so writes are serialised)
 static std::string g_last_error;
+static std::mutex g_error_mutex;
 
 void set_error(const char* where, const char* msg) {
   char b[1024];
   std::snprintf(b, sizeof(b), "%s: %s", where, msg);
+  std::lock_guard<std::mutex> lock(g_error_mutex);
   g_last_error = b;
 }
 void set_hip_error(const char* where, hipError_t e) { set_error(where, hipGetErrorString(e)); }
 const char* error_text() { return g_last_error.c_str(); }
-void clear_error() { g_last_error.clear(); }
+void clear_error() {
+  std::lock_guard<std::mutex> lock(g_error_mutex);
+  g_last_error.clear();
+}
 
-static Context g_ctx;
-Context& ctx() { return g_ctx; }
+static Context g_ctx[MAX_TARGETS + 1];
+static int g_selected = 0;
+Context& ctx_at(int slot) {
+  Context& c = g_ctx[slot];
+  c.slot = slot;
+  return c;
+}
+Context& ctx() { return ctx_at(g_selected); }
+int selected_target() { return g_selected; }
+void select_target(int slot) { g_selected = slot; }
 
 // hipFree only what was allocated: a context that never reached the device
 // releases without touching the HIP runtime
@@ -50,7 +66,7 @@ void release_context(Context& c) {
   if (c.stream) hipStreamSynchronize(c.stream);
   matvec_release(c);
   slices_release(c);
-  batch_release();
+  batch_release(c);
   dfree(c.green);
   dfree(c.iocc); dfree(c.occ); dfree(c.rowptr); dfree(c.rowlo); dfree(c.qmap);
   dfree(c.adia); dfree(c.aoff); dfree(c.adia_c); dfree(c.aoff_c);
@@ -61,7 +77,9 @@ void release_context(Context& c) {
   if (c.hscal) hipHostFree(c.hscal);
   hipStream_t keep = c.stream;
   int backend = c.backend, prec = c.precision, gen = c.fft_generic, tune = c.autotune, rmode = c.reliable_mode;
+  const int slot = c.slot;
   c = Context{};
+  c.slot = slot;
   c.stream = keep;
   c.reliable_mode = rmode;
   c.backend = backend;

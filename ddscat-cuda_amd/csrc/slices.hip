@@ -30,7 +30,7 @@ struct SliceState {
   int batch = 4;
   int cz = 1, cy = 1, rx_f = 1, rx_i = 1;
 };
-static SliceState g_sl;
+static SliceState g_sl[MAX_TARGETS + 1];  // one per target slot
 
 // --------------------------------------------------- z-FFT forward (x -> zwork)
 __global__ __launch_bounds__(SL_THREADS) void k_sl_zfwd(const float2* __restrict__ x, const int* __restrict__ qmap,
@@ -358,7 +358,7 @@ int slices_setup(Context& c) {
     set_error("SLICES plan", "padded grid axis exceeds the LDS FFT limit (2N <= 2048)");
     return 206;
   }
-  SliceState& s = g_sl;
+  SliceState& s = g_sl[c.slot];
   s = SliceState{};
   // z passes: column tiles of consecutive ix; the forward pass is single component
   s.cz = std::min(16, p2floor(std::max(1, narrow / m.gz)));
@@ -384,7 +384,7 @@ int slices_setup(Context& c) {
 }
 
 void slices_release(Context& c) {
-  SliceState& s = g_sl;
+  SliceState& s = g_sl[c.slot];
   if (s.zwork) {
     for (int i = 0; i < 2; ++i) {
       if (s.side[i]) hipStreamSynchronize(s.side[i]);
@@ -409,7 +409,7 @@ void slices_release(Context& c) {
 
 int slices_matvec(Context& c, int cwhat, const float2* x, float2* y, const float2* adia, const float2* aoff,
                   int layout, hipStream_t st) {
-  SliceState& s = g_sl;
+  SliceState& s = g_sl[c.slot];
   const MatvecPlan& m = c.mv;
   if (!s.zwork) { set_error("SLICES MATVEC", "backend not prepared"); return 20; }
   const int cj = (cwhat == 'C') ? 1 : 0;

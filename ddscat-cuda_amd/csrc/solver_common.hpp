@@ -81,7 +81,7 @@ int slices_matvec(Context& c, int cwhat, const float2* x, float2* y, const float
 
 int finish_prepare(Context& c, const int16_t* iocc, int code_alloc, int code_occ);
 // frees the batched-solver buffers (bsolve.hip)
-void batch_release();
+void batch_release(Context& c);
 int ensure_stream(Context& c);
 
 // MATVEC on the compact solver vectors with the context's operators

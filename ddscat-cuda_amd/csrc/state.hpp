@@ -55,7 +55,13 @@ struct SolveLog {
   }
 };
 
+// Prepared targets the library holds at once: slot 0 is the reference ABI's
+// single target; slots 1..MAX_TARGETS serve multi-target batched solves
+// (ddscat_hip_select_target, ddscat_hip_solve_targets).
+constexpr int MAX_TARGETS = 64;
+
 struct Context {
+  int slot = 0;  // index of this context among the held targets
   bool ready = false;
   int nx = 0, ny = 0, nz = 0, ipbc = 0, nat = 0, nat0 = 0;
   int backend = BACKEND_FFT3D;
@@ -96,7 +102,10 @@ struct Context {
   double green_build_ms = 0.0;
 };
 
-Context& ctx();
+Context& ctx();                 // the selected target's context
+Context& ctx_at(int slot);      // target slot 0..MAX_TARGETS
+int selected_target();
+void select_target(int slot);
 void release_context(Context& c);
 int  ensure_solver_workspace(Context& c, int nvec);
 void release_solver_workspace(Context& c);

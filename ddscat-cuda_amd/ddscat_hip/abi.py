@@ -33,7 +33,7 @@ EXTENSION_SYMBOLS = [
     "ddscat_hip_cabs_adjoint", "ddscat_hip_setup_rhs", "ddscat_hip_set_autotune",
     "ddscat_hip_nearfield", "ddscat_hip_set_pass_kernel", "ddscat_hip_scat_matrix", "ddscat_hip_force_restart",
     "ddscat_hip_solve_log", "ddscat_hip_set_polarizability", "ddscat_hip_scat_f32_noalloc",
-    "ddscat_hip_torque", "ddscat_hip_set_reliable_mode",
+    "ddscat_hip_torque", "ddscat_hip_set_reliable_mode", "ddscat_hip_select_target", "ddscat_hip_solve_targets",
 ]
 # method ids of ddscat_hip_solve_device
 SOLVER_IDS = {"pbcgs2": 0, "qmrccg": 1, "petrkp": 2, "gpbicg": 3, "pbcgst": 4, "bicgstab4": 5,
@@ -107,6 +107,10 @@ class Library:
         L.ddscat_hip_torque.restype = i
         L.ddscat_hip_set_reliable_mode.argtypes = [i]
         L.ddscat_hip_set_reliable_mode.restype = i
+        L.ddscat_hip_select_target.argtypes = [i]
+        L.ddscat_hip_select_target.restype = i
+        L.ddscat_hip_solve_targets.argtypes = [i, i, vp, vp, vp, vp, vp, vp, i, i, f, vp, vp, vp, i]
+        L.ddscat_hip_solve_targets.restype = i
         L.ddscat_cuda_release.argtypes = []
         L.ddscat_cuda_release.restype = i
         L.ddscat_cuda_last_error.argtypes = []
@@ -419,6 +423,31 @@ class Library:
         """QMRCCG's response to a failed reliable check: "replace" (default) or "restart" (the reference's)."""
         code = {"replace": 0, "restart": 1}.get(mode, -1)
         return self._check("set_reliable_mode", self.lib.ddscat_hip_set_reliable_mode(code))
+
+    def select_target(self, slot):
+        """Make target slot `slot` (0 = the reference ABI's) the one every call acts on."""
+        return self._check("select_target", self.lib.ddscat_hip_select_target(int(slot)))
+
+    def solve_targets(self, name, slots, nrhs, b_ptrs, x_ptrs, adia_ptrs, aoff_ptrs, op_per_rhs=True, maxit=300,
+                      tol=1e-5, graph=True, check=True):
+        """Batched solves of several prepared targets at once (one stream and
+        host thread each).  Returns (itno, tole, ncompte) per target and rc."""
+        n = len(slots)
+        sl = np.ascontiguousarray(slots, np.int32)
+        nr = np.ascontiguousarray(nrhs, np.int32)
+        arr = lambda ptrs: (C.c_void_p * n)(*[C.c_void_p(int(v)) for v in ptrs])
+        tot = int(nr.sum())
+        it = np.zeros(tot, np.int32)
+        nc = np.zeros(tot, np.int32)
+        te = np.zeros(tot, np.float32)
+        rc = self.lib.ddscat_hip_solve_targets(SOLVER_IDS[name], n, _p(sl), _p(nr), arr(b_ptrs), arr(x_ptrs),
+                                               arr(adia_ptrs), arr(aoff_ptrs), 1 if op_per_rhs else 0, int(maxit),
+                                               float(tol), _p(it), _p(te), _p(nc), 1 if graph else 0)
+        if check:
+            self._check(f"solve_targets_{name}", rc)
+        cut = np.cumsum(np.concatenate([[0], nr]))
+        per = [(it[cut[t]:cut[t + 1]], te[cut[t]:cut[t + 1]], nc[cut[t]:cut[t + 1]]) for t in range(n)]
+        return per, rc
 
     def set_autotune(self, on):
         return self._check("set_autotune", self.lib.ddscat_hip_set_autotune(1 if on else 0))

@@ -186,6 +186,32 @@ DDSCAT_HIP_API int ddscat_hip_solve_batch(int method, int nrhs, const void *b, v
                                           const void *aoff, int op_per_rhs, int maxit, float tol, int *itno,
                                           float *tole, int *ncompte, int use_graph);
 
+/* Target slots.  The library holds up to 64 prepared targets besides the
+ * reference ABI's one (slot 0, selected by default).  Every entry point,
+ * ddscat_cuda_* and ddscat_hip_*, acts on the selected slot: select a slot,
+ * then prepare it (ddscat_cuda_prepare_green_f32 / _prepare_f32), build its
+ * inputs (ddscat_hip_setup_rhs) and read its results (ddscat_hip_evalq_batch)
+ * as for the single target; ddscat_cuda_release frees the selected slot.
+ * The selection is process-wide (not per thread).  Code 1: slot outside
+ * 0..64. */
+DDSCAT_HIP_API int ddscat_hip_select_target(int slot);
+
+/* Batched solves of several prepared targets in one call (the uneven
+ * multi-target batch): target t = slots[t] with nrhs[t] right-hand sides and
+ * its own compact DEVICE arrays b[t], x[t] (initial guess in, solution out),
+ * adia[t], aoff[t] laid out as for ddscat_hip_solve_batch (per-RHS operators
+ * when op_per_rhs = 1).  Each target runs the batched QMRCCG or PBCGS2 of
+ * ddscat_hip_solve_batch on its own stream, driven from its own host thread,
+ * so the targets' kernels share the GPU; the call returns when all are done.
+ * itno, tole, ncompte: host arrays of sum(nrhs) entries, target after
+ * target.  Every slot must be prepared on the FFT3D backend and appear once.
+ * Returns 0 or the first target's nonzero code (1 invalid argument, 2
+ * method/backend, 29 memory, else the solver's). */
+DDSCAT_HIP_API int ddscat_hip_solve_targets(int method, int ntargets, const int *slots, const int *nrhs,
+                                            void *const *b, void *const *x, void *const *adia,
+                                            void *const *aoff, int op_per_rhs, int maxit, float tol, int *itno,
+                                            float *tole, int *ncompte, int use_graph);
+
 /* EVALQ for nrhs solutions on DEVICE: e (incident field), p (polarization)
  * [nrhs][3][NAT0], adia / aoff per RHS (op_per_rhs = 1) or shared (0);
  * ak [nrhs][3] host floats; out [nrhs][3] host doubles = Cabs, Cext, Cpha

@@ -59,6 +59,10 @@ def test_host_side_error_paths_without_device():
     assert L.lib.ddscat_hip_set_backend(7) == 1 and L.lib.ddscat_hip_set_precision(3) == 1
     assert L.lib.ddscat_hip_set_polarizability(2) == 1 and L.lib.ddscat_hip_set_polarizability(0) == 0
     assert L.lib.ddscat_hip_solve_log(5, None, None, 0, None, 0, None) == -1
+    assert L.lib.ddscat_hip_select_target(-1) == 1 and L.lib.ddscat_hip_select_target(65) == 1
+    assert L.lib.ddscat_hip_select_target(0) == 0
+    assert L.lib.ddscat_hip_solve_targets(1, 0, None, None, None, None, None, None, 1, 10, 1e-5, None, None, None,
+                                          1) == 1
     assert L.info()["nat"] == 0
 
 
