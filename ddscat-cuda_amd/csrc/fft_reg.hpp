@@ -251,7 +251,7 @@ __device__ __forceinline__ void fft2_fwd(float2* v, float2* __restrict__ lds, co
   for (int k2 = 0; k2 < R2; ++k2) {
     float2 w = twL[t * k2];
     if (DIR < 0) w.y = -w.y;
-    lds[lidx<LAY, R1>(k2, t, s, P)] = cmul(v[k2], w);
+    lds[lidx<LAY, R1>(k2, t, s, P)] = k2 == 0 ? v[0] : cmul(v[k2], w);
   }
   __syncthreads();
 #pragma unroll
@@ -284,7 +284,7 @@ __device__ __forceinline__ void fft2_inv(float2* v, float2* __restrict__ lds, co
     for (int n1 = 0; n1 < R1; ++n1) {
       float2 w = twL[n1 * k2];
       if (DIR < 0) w.y = -w.y;
-      lds[lidx<LAY, R1>(k2, n1, s, P)] = cmul(u[n1], w);
+      lds[lidx<LAY, R1>(k2, n1, s, P)] = n1 == 0 ? u[0] : cmul(u[n1], w);
     }
   }
   __syncthreads();
@@ -477,6 +477,180 @@ __device__ __forceinline__ void fft2p_inv(float2* v, float2* __restrict__ lds, c
 #pragma unroll
   for (int j = 0; j < H; ++j) v[j] = lds[pidx<Q>(H * h + j, t, s, C, P2)];
   dif_pair<Q, DIR>(v, h);
+}
+
+// ---- register Stockham stages
+// A length L = R0 R1 R2 R3 transform (R3 = 1: three stages) held by T = L / V
+// threads per sequence, V values per thread.  Forward stage s of radix R and
+// span P = R0 ... R_{s-1}: butterfly i = j + T b (b < V / R) of thread j
+// gathers u_r = d[i + r L / R] (r < R), multiplies u_r by w_{PR}^{r (i mod P)},
+// does the R-point DFT and scatters y_r to (i - i mod P) R + i mod P + r P.
+// After the last stage d is the transform in natural order (self-sorting
+// Stockham).  Stage 0 gathers from registers the caller loaded, the stages
+// between exchange through LDS, and the last stage's outputs stay in
+// registers: thread j ends with X[i + r L / R_last] at v[b R_last + r].  The
+// inverse runs the transposed stages in reverse order (gather from the
+// scatter positions, conjugate DFT, conjugate twiddle, scatter to the gather
+// positions) and leaves y where stage 0 gathered from.  Half the registers
+// of the two-level form for the same length, one more LDS exchange.
+constexpr int sk_rad(int s, int r0, int r1, int r2, int r3) { return s == 0 ? r0 : s == 1 ? r1 : s == 2 ? r2 : r3; }
+template <int V_, int R0, int R1, int R2, int R3 = 1>
+struct SkPlan {
+  static constexpr int V = V_, L = R0 * R1 * R2 * R3, T = L / V, S = R3 > 1 ? 4 : 3;
+  static constexpr int rad(int s) { return sk_rad(s, R0, R1, R2, R3); }
+  static constexpr int span(int s) { return s == 0 ? 1 : span(s - 1) * rad(s - 1); }
+  // LDS swizzle shift: stage 0 scatters with stride R0
+  static constexpr int SH = R0 == 8 ? 3 : 2;
+  static_assert(V % R0 == 0 && V % R1 == 0 && V % R2 == 0 && V % R3 == 0, "every radix divides V");
+  static_assert(R0 % 2 == 0, "even first radix: the zero half of a padded input is a compile-time set");
+  static_assert(L % 32 == 0, "the LDS swizzle permutes within 32-point blocks");
+};
+// LDS slot of point idx of sequence col: C sequences interleaved, the point
+// index XOR-swizzled within 32-point blocks so that the stride-R0 scatter of
+// stage 0 and the unit-stride gathers of a wave both hit distinct banks.
+template <int C, int SH>
+__device__ __forceinline__ int sk_lds(int idx, int col) {
+  constexpr int M = C >= 32 ? 0 : 32 / C - 1;
+  return C * (idx ^ ((idx >> SH) & M)) + col;
+}
+template <class PL, int s>
+__device__ __forceinline__ int sk_scatter(int i, int r) {
+  constexpr int R = PL::rad(s), P = PL::span(s);
+  const int k = i % P;
+  return (i - k) * R + k + r * P;
+}
+template <class PL, int s>
+__device__ __forceinline__ int sk_gather(int i, int r) {
+  return i + r * (PL::L / PL::rad(s));
+}
+// stage-s twiddles and DFTs of thread j's butterflies (forward order: twiddle, DFT)
+template <class PL, int s, int DIR>
+__device__ __forceinline__ void sk_fwd_bfly(float2* v, const float2* __restrict__ twL, int j) {
+  constexpr int R = PL::rad(s), P = PL::span(s), NB = PL::V / R, TS = PL::L / (P * R);
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    if constexpr (P > 1) {
+      const int k = (j + PL::T * b) % P;
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        float2 w = twL[r * k * TS];
+        if (DIR < 0) w.y = -w.y;
+        v[b * R + r] = cmul(v[b * R + r], w);
+      }
+    }
+    dft_reg<R, DIR>(v + b * R);
+  }
+}
+// transposed stage s (DFT, then twiddle; DIR is the inverse's sign)
+template <class PL, int s, int DIR>
+__device__ __forceinline__ void sk_inv_bfly(float2* v, const float2* __restrict__ twL, int j) {
+  constexpr int R = PL::rad(s), P = PL::span(s), NB = PL::V / R, TS = PL::L / (P * R);
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    dft_reg<R, DIR>(v + b * R);
+    if constexpr (P > 1) {
+      const int k = (j + PL::T * b) % P;
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        float2 w = twL[r * k * TS];
+        if (DIR < 0) w.y = -w.y;
+        v[b * R + r] = cmul(v[b * R + r], w);
+      }
+    }
+  }
+}
+// LDS exchange buffer of an exchange (K sequences of L points per column,
+// K L C slots): with NBUF = 2 consecutive exchanges alternate between two
+// buffers, so a write needs no barrier of its own (every thread passed the
+// previous exchange's barrier after its reads of the buffer it now
+// overwrites).
+template <class PL, int C, int NBUF, int K>
+struct SkLds {
+  float2* xb;
+  int e = 0;
+  __device__ __forceinline__ float2* next() {
+    if (NBUF == 1) {
+      __syncthreads();
+      return xb;
+    }
+    float2* b = xb + (e & 1) * (K * PL::L * C);
+    ++e;
+    return b;
+  }
+};
+// K sequences advance stage by stage together (v[q V .. q V + V) is
+// sequence q), so one barrier serves all K and a stage's twiddles are
+// read once for the K.
+// forward stages s .. S-1 (stage s-1's butterflies done, its outputs in v)
+template <class PL, int s, int C, int NBUF, int K, int DIR>
+__device__ __forceinline__ void sk_fwd_from(float2* v, SkLds<PL, C, NBUF, K>& lx, const float2* __restrict__ twL,
+                                            int j, int col) {
+  if constexpr (s < PL::S) {
+    constexpr int V = PL::V, LC = PL::L * C;
+    constexpr int Rp = PL::rad(s - 1), NBp = V / Rp, R = PL::rad(s), NB = V / R;
+    float2* xb = lx.next();
+#pragma unroll
+    for (int q = 0; q < K; ++q)
+#pragma unroll
+      for (int b = 0; b < NBp; ++b)
+#pragma unroll
+        for (int r = 0; r < Rp; ++r)
+          xb[q * LC + sk_lds<C, PL::SH>(sk_scatter<PL, s - 1>(j + PL::T * b, r), col)] = v[q * V + b * Rp + r];
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < K; ++q)
+#pragma unroll
+      for (int b = 0; b < NB; ++b)
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+          v[q * V + b * R + r] = xb[q * LC + sk_lds<C, PL::SH>(sk_gather<PL, s>(j + PL::T * b, r), col)];
+#pragma unroll
+    for (int q = 0; q < K; ++q) sk_fwd_bfly<PL, s, DIR>(v + q * V, twL, j);
+    sk_fwd_from<PL, s + 1, C, NBUF, K, DIR>(v, lx, twL, j, col);
+  }
+}
+// whole forward transform of K sequences: stage-0 gathers in, last stage's outputs out
+template <class PL, int C, int NBUF, int K, int DIR>
+__device__ __forceinline__ void sk_forward(float2* v, SkLds<PL, C, NBUF, K>& lx, const float2* __restrict__ twL, int j,
+                                           int col) {
+#pragma unroll
+  for (int q = 0; q < K; ++q) sk_fwd_bfly<PL, 0, DIR>(v + q * PL::V, twL, j);
+  sk_fwd_from<PL, 1, C, NBUF, K, DIR>(v, lx, twL, j, col);
+}
+// transposed stages s .. 0 (v holds stage s's scatter positions)
+template <class PL, int s, int C, int NBUF, int K, int DIR>
+__device__ __forceinline__ void sk_inv_from(float2* v, SkLds<PL, C, NBUF, K>& lx, const float2* __restrict__ twL,
+                                            int j, int col) {
+  constexpr int V = PL::V, LC = PL::L * C;
+#pragma unroll
+  for (int q = 0; q < K; ++q) sk_inv_bfly<PL, s, DIR>(v + q * V, twL, j);
+  if constexpr (s > 0) {
+    constexpr int R = PL::rad(s), NB = V / R, Rn = PL::rad(s - 1), NBn = V / Rn;
+    float2* xb = lx.next();
+#pragma unroll
+    for (int q = 0; q < K; ++q)
+#pragma unroll
+      for (int b = 0; b < NB; ++b)
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+          xb[q * LC + sk_lds<C, PL::SH>(sk_gather<PL, s>(j + PL::T * b, r), col)] = v[q * V + b * R + r];
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < K; ++q)
+#pragma unroll
+      for (int b = 0; b < NBn; ++b)
+#pragma unroll
+        for (int r = 0; r < Rn; ++r)
+          v[q * V + b * Rn + r] = xb[q * LC + sk_lds<C, PL::SH>(sk_scatter<PL, s - 1>(j + PL::T * b, r), col)];
+    sk_inv_from<PL, s - 1, C, NBUF, K, DIR>(v, lx, twL, j, col);
+  }
+}
+// whole inverse transform of K sequences: the forward's output distribution
+// in, stage-0 gather positions out
+template <class PL, int C, int NBUF, int K, int DIR>
+__device__ __forceinline__ void sk_inverse(float2* v, SkLds<PL, C, NBUF, K>& lx, const float2* __restrict__ twL, int j,
+                                           int col) {
+  sk_inv_from<PL, PL::S - 1, C, NBUF, K, DIR>(v, lx, twL, j, col);
 }
 
 }  // namespace ddhip

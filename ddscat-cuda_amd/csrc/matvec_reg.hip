@@ -319,6 +319,46 @@ __device__ __forceinline__ void p3_green(const P3Col& q, const float2* __restric
   p1 = cadd(cadd(cmul(g12, a), cmul(G.yy, b)), cmul(g23, c));
   p2 = cadd(cadd(cmul(g13, a), cmul(g23, b)), cmul(G.zz, c));
 }
+// g0 p0 + g1 p1 + g2 p2 as two FMA chains (6 instructions per part)
+__device__ __forceinline__ float2 cdot3(float2 g0, float2 p0, float2 g1, float2 p1, float2 g2, float2 p2) {
+  float re = g0.x * p0.x, im = g0.x * p0.y;
+  re = fmaf(-g0.y, p0.y, re);
+  im = fmaf(g0.y, p0.x, im);
+  re = fmaf(g1.x, p1.x, re);
+  im = fmaf(g1.x, p1.y, im);
+  re = fmaf(-g1.y, p1.y, re);
+  im = fmaf(g1.y, p1.x, im);
+  re = fmaf(g2.x, p2.x, re);
+  im = fmaf(g2.x, p2.y, im);
+  re = fmaf(-g2.y, p2.y, re);
+  im = fmaf(g2.y, p2.x, im);
+  return make_float2(re, im);
+}
+#ifndef P3S_GREEN_GROUP
+#define P3S_GREEN_GROUP 4
+#endif
+__device__ __forceinline__ Green6 green_at_b(const char* __restrict__ g, unsigned off) {
+  const float4 a = *reinterpret_cast<const float4*>(g + off);
+  const float4 b = *reinterpret_cast<const float4*>(g + off + 16u);
+  const float4 c = *reinterpret_cast<const float4*>(g + off + 32u);
+  return Green6{make_float2(a.x, a.y), make_float2(a.z, a.w), make_float2(b.x, b.y),
+                make_float2(b.z, b.w), make_float2(c.x, c.y), make_float2(c.z, c.w)};
+}
+// p3_green with the rows as FMA chains
+template <bool SLY>
+__device__ __forceinline__ void p3_green_fma(const P3Col& q, const float2* __restrict__ green, int k, float2& p0,
+                                             float2& p1, float2& p2) {
+  const int kr = min(k, q.mf - k);
+  const float sf = k <= q.tf ? 1.f : -1.f;
+  const Green6 G = green_at(green, q.pb + q.step * kr);
+  const float2 g12 = cscale(G.xy, SLY ? q.c12 * sf : q.c12);
+  const float2 g13 = cscale(G.xz, SLY ? q.c13 : q.c13 * sf);
+  const float2 g23 = cscale(G.yz, q.c23 * sf);
+  const float2 a = p0, b = p1, c = p2;
+  p0 = cdot3(G.xx, a, g12, b, g13, c);
+  p1 = cdot3(g12, a, G.yy, b, g23, c);
+  p2 = cdot3(g13, a, g23, b, G.zz, c);
+}
 // row of block row index j: FFT3D orders ky as 0, NY, 1, 2NY-1, 2, ... so
 // mirror rows (same Green octant entries) run back to back on one XCD
 template <bool SLY>
@@ -382,17 +422,23 @@ __global__ __launch_bounds__(R1 * C) void kr_pass3(float2* __restrict__ a2, cons
   const int kx = tile * C + s;
   for (int i = threadIdx.x; i < L; i += RT) twL[i] = twg[i];
   const bool live = kx < gx;
-  const size_t cs = (size_t)nz * plane;
-  float2* col = a2 + (size_t)ky * gx + kx;
+  // workgroup-uniform base of the tile's first column per component, 32-bit
+  // byte offsets per lane (the launcher checks a component spans < 4 GiB)
+  const char* const c0 = reinterpret_cast<const char*>(a2 + (size_t)ky * gx + (size_t)tile * C);
+  const size_t csb = (size_t)nz * plane * sizeof(float2);
+  const unsigned esb = (unsigned)(plane * sizeof(float2)), lane0 = (unsigned)(s * sizeof(float2));
+  auto at = [&](int c, unsigned o) -> float2* { return reinterpret_cast<float2*>(const_cast<char*>(c0) + c * csb + o); };
+  // element n = t + R1 n2 at ob + n2 (R1 esb)
+  const unsigned ob = lane0 + (unsigned)t * esb, sb = (unsigned)R1 * esb;
   float2 v0[VN], v1[VN], v2[VN];
 #pragma unroll
   for (int n2 = 0; n2 < R2; ++n2) {
     const int n = t + R1 * n2;
     const bool ok = live && lower_half<R2>(n2, n, nz);
-    const size_t o = (size_t)n * plane;
-    v0[n2] = ok ? col[o] : make_float2(0.f, 0.f);
-    v1[n2] = ok ? col[o + cs] : make_float2(0.f, 0.f);
-    v2[n2] = ok ? col[o + 2 * cs] : make_float2(0.f, 0.f);
+    const unsigned o = ob + (unsigned)n2 * sb;
+    v0[n2] = ok ? *at(0, o) : make_float2(0.f, 0.f);
+    v1[n2] = ok ? *at(1, o) : make_float2(0.f, 0.f);
+    v2[n2] = ok ? *at(2, o) : make_float2(0.f, 0.f);
   }
   __syncthreads();
   fft2_fwd<R1, R2, +1>(v0, buf, twL, t, s, P);
@@ -409,7 +455,8 @@ __global__ __launch_bounds__(R1 * C) void kr_pass3(float2* __restrict__ a2, cons
   const int jr = min(ky, gg.my - ky);
   const float sxy = ((kx <= gg.tx) == (ky <= gg.ty)) ? 1.f : -1.f;
   const float sx = kx <= gg.tx ? 1.f : -1.f, sy = ky <= gg.ty ? 1.f : -1.f;
-  const size_t pb = (size_t)ir + (size_t)ox * jr;
+  const char* const gb = reinterpret_cast<const char*>(green);
+  const unsigned pb48 = (unsigned)(ir + ox * jr) * 48u, st48 = (unsigned)(ox * oy) * 48u;
 #pragma unroll
   for (int i = 0; i < G; ++i) {
     if (!reg2_owns<R1, R2>(t, i)) continue;
@@ -418,18 +465,15 @@ __global__ __launch_bounds__(R1 * C) void kr_pass3(float2* __restrict__ a2, cons
       const int kz = (t + R1 * i) + R2 * k1;
       const int kr = min(kz, gg.mz - kz);
       const float sz = kz <= gg.tz ? 1.f : -1.f;
-      const Green6 G = green_at(green, pb + (size_t)ox * oy * kr);
-      const float2 g11 = G.xx;
+      const Green6 G = green_at_b(gb, pb48 + (unsigned)kr * st48);
       const float2 g12 = cscale(G.xy, sxy);
       const float2 g13 = cscale(G.xz, sx * sz);
-      const float2 g22 = G.yy;
       const float2 g23 = cscale(G.yz, sy * sz);
-      const float2 g33 = G.zz;
       const int e = i * R1 + k1;
       const float2 p0 = v0[e], p1 = v1[e], p2 = v2[e];
-      v0[e] = cadd(cadd(cmul(g11, p0), cmul(g12, p1)), cmul(g13, p2));
-      v1[e] = cadd(cadd(cmul(g12, p0), cmul(g22, p1)), cmul(g23, p2));
-      v2[e] = cadd(cadd(cmul(g13, p0), cmul(g23, p1)), cmul(g33, p2));
+      v0[e] = cdot3(G.xx, p0, g12, p1, g13, p2);
+      v1[e] = cdot3(g12, p0, G.yy, p1, g23, p2);
+      v2[e] = cdot3(g13, p0, g23, p1, G.zz, p2);
     }
   }
   __syncthreads();
@@ -443,10 +487,10 @@ __global__ __launch_bounds__(R1 * C) void kr_pass3(float2* __restrict__ a2, cons
     for (int n2 = 0; n2 < R2; ++n2) {
       const int n = t + R1 * n2;
       if (lower_half<R2>(n2, n, nz)) {
-        const size_t o = (size_t)n * plane;
-        col[o] = v0[n2];
-        col[o + cs] = v1[n2];
-        col[o + 2 * cs] = v2[n2];
+        const unsigned o = ob + (unsigned)n2 * sb;
+        *at(0, o) = v0[n2];
+        *at(1, o) = v1[n2];
+        *at(2, o) = v2[n2];
       }
     }
   }
@@ -709,6 +753,99 @@ __global__ __launch_bounds__(M * 16 * C) void kr_pass3m(float2* __restrict__ a2,
   }
 }
 
+// Pass 3 with register Stockham transforms (fft_reg.hpp, SkPlan): a column
+// of L = 2 NZ points is held by T = L / V threads with V values per
+// component, so the three components take 6 V registers (48 at V = 8, where
+// kr_pass3m keeps 96) and a CU holds two workgroups: one computes while the
+// other waits on its loads.  C columns per workgroup, thread (col, j) at
+// threadIdx.x = j C + col so that a wave's loads run along kx.  Unit grid
+// and Green reconstruction as in kr_pass3m.  Addresses are a workgroup-
+// uniform base per component plus a 32-bit byte offset per lane (one
+// register per address instead of two); the launcher checks that a
+// component of a2 and the Green array span less than 4 GiB.
+template <class PL, int C, int NBUF, int K, int WPE, bool SLY = false>
+__global__ __launch_bounds__(PL::T * C) __attribute__((amdgpu_waves_per_eu(WPE))) void kr_pass3s(
+    float2* __restrict__ a2, const float2* __restrict__ green, const float2* __restrict__ twg, int nx, int ny, int nz,
+    int gx, int gy, size_t plane, BatchArgs ba, GreenGeo gg, P3Grid g3) {
+  constexpr int L = PL::L, T = PL::T, V = PL::V, NTH = T * C, R0 = PL::rad(0), RL = PL::rad(PL::S - 1);
+  __shared__ float2 twL[L];
+  __shared__ float2 xb[NBUF * K * L * C];
+  P3Unit u{(int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z};
+  int NT = (int)gridDim.x;
+  if (!SLY) {
+    if (!p3_unit(g3, u)) return;
+    NT = g3.nt;
+  }
+  const int rb = u.rb;
+  if (ba.active && !ba.active[rb]) return;
+  a2 += rb * ba.a2s;
+  const int col = threadIdx.x % C, j = threadIdx.x / C;
+  const int tile = xcd_tile<C>(u.bx, NT, true);
+  const int kx = tile * C + col;
+  for (int i = threadIdx.x; i < L; i += NTH) twL[i] = twg[i];
+  const bool live = kx < gx;
+  const int row = p3_row<SLY>(u.by, ny, gy, (int)plane);
+  const P3Col q = p3_col<SLY>(a2, kx, row, nx, ny, nz, gx, plane, gg);
+  // uniform base of the tile's first column; lane offset (col + n es) * 8 bytes
+  char* const cb = reinterpret_cast<char*>(
+      a2 + (SLY ? (size_t)(row - (int)plane) * 3 * ny * gx : (size_t)row * gx) + (size_t)tile * C);
+  const size_t csb = q.cs * sizeof(float2);
+  const unsigned esb = (unsigned)(q.es * sizeof(float2));
+  const unsigned lane0 = (unsigned)(col * sizeof(float2));
+  // stage-0 gathers: v[c][b R0 + r] = x_c[j + T b + r L / R0]; the upper
+  // half (r >= R0 / 2) is the zero padding
+  float2 v[3][V];
+#pragma unroll
+  for (int b = 0; b < V / R0; ++b)
+#pragma unroll
+    for (int r = 0; r < R0; ++r) {
+      const unsigned o = lane0 + (unsigned)sk_gather<PL, 0>(j + T * b, r) * esb;
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+        v[c][b * R0 + r] = (live && 2 * r < R0) ? *reinterpret_cast<const float2*>(cb + c * csb + o)
+                                                : make_float2(0.f, 0.f);
+    }
+  __syncthreads();
+  // K = 3: the three components advance stage by stage together; K = 1: one after another
+  SkLds<PL, C, NBUF, K> lx{xb};
+#pragma unroll
+  for (int c = 0; c < 3; c += K) sk_forward<PL, C, NBUF, K, +1>(v[c], lx, twL, j, col);
+  // 3x3 symmetric Green multiply at k = i + r L / R_last (octant point
+  // pb + step min(k, mf - k), 48 bytes each)
+  const char* const gb = reinterpret_cast<const char*>(green);
+  const unsigned pb48 = (unsigned)q.pb * 48u, st48 = (unsigned)q.step * 48u;
+#pragma unroll
+  for (int b = 0; b < V / RL; ++b)
+#pragma unroll
+    for (int r = 0; r < RL; ++r) {
+      const int e = b * RL + r, k = j + T * b + r * (L / RL);
+      const int kr = min(k, q.mf - k);
+      const float sf = k <= q.tf ? 1.f : -1.f;
+      const Green6 G = green_at_b(gb, pb48 + (unsigned)kr * st48);
+      const float2 g12 = cscale(G.xy, SLY ? q.c12 * sf : q.c12);
+      const float2 g13 = cscale(G.xz, SLY ? q.c13 : q.c13 * sf);
+      const float2 g23 = cscale(G.yz, q.c23 * sf);
+      const float2 p0 = v[0][e], p1 = v[1][e], p2 = v[2][e];
+      v[0][e] = cdot3(G.xx, p0, g12, p1, g13, p2);
+      v[1][e] = cdot3(g12, p0, G.yy, p1, g23, p2);
+      v[2][e] = cdot3(g13, p0, g23, p1, G.zz, p2);
+      // at most P3S_GREEN_GROUP points' Green loads (12 registers each) in flight
+      if ((e + 1) % P3S_GREEN_GROUP == 0) __builtin_amdgcn_sched_barrier(0);
+    }
+#pragma unroll
+  for (int c = 0; c < 3; c += K) sk_inverse<PL, C, NBUF, K, -1>(v[c], lx, twL, j, col);
+  if (live) {
+#pragma unroll
+    for (int b = 0; b < V / R0; ++b)
+#pragma unroll
+      for (int r = 0; 2 * r < R0; ++r) {
+        const unsigned o = lane0 + (unsigned)sk_gather<PL, 0>(j + T * b, r) * esb;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) *reinterpret_cast<float2*>(cb + c * csb + o) = v[c][b * R0 + r];
+      }
+  }
+}
+
 // Pass 3 with the mirror rows ky and gy - ky in the two halves of one
 // workgroup: both halves read the same Green octant entries at the same
 // time, so one of each pair of loads is served by the CU's L1.
@@ -737,17 +874,23 @@ __global__ __launch_bounds__(2 * R1 * C) void kr_pass3y(float2* __restrict__ a2,
   const int kx = tile * C + s;
   for (int i = threadIdx.x; i < L; i += RT) twL[i] = twg[i];
   const bool live = kx < gx;
-  const size_t cs = (size_t)nz * plane;
-  float2* col = a2 + (size_t)ky * gx + kx;
+  // workgroup-uniform base of the tile's first column per component, 32-bit
+  // byte offsets per lane (the launcher checks a component spans < 4 GiB)
+  const char* const c0 = reinterpret_cast<const char*>(a2 + (size_t)ky * gx + (size_t)tile * C);
+  const size_t csb = (size_t)nz * plane * sizeof(float2);
+  const unsigned esb = (unsigned)(plane * sizeof(float2)), lane0 = (unsigned)(s * sizeof(float2));
+  auto at = [&](int c, unsigned o) -> float2* { return reinterpret_cast<float2*>(const_cast<char*>(c0) + c * csb + o); };
+  // element n = t + R1 n2 at ob + n2 (R1 esb)
+  const unsigned ob = lane0 + (unsigned)t * esb, sb = (unsigned)R1 * esb;
   float2 v0[VN], v1[VN], v2[VN];
 #pragma unroll
   for (int n2 = 0; n2 < R2; ++n2) {
     const int n = t + R1 * n2;
     const bool ok = live && lower_half<R2>(n2, n, nz);
-    const size_t o = (size_t)n * plane;
-    v0[n2] = ok ? col[o] : make_float2(0.f, 0.f);
-    v1[n2] = ok ? col[o + cs] : make_float2(0.f, 0.f);
-    v2[n2] = ok ? col[o + 2 * cs] : make_float2(0.f, 0.f);
+    const unsigned o = ob + (unsigned)n2 * sb;
+    v0[n2] = ok ? *at(0, o) : make_float2(0.f, 0.f);
+    v1[n2] = ok ? *at(1, o) : make_float2(0.f, 0.f);
+    v2[n2] = ok ? *at(2, o) : make_float2(0.f, 0.f);
   }
   __syncthreads();
   fft2_fwd<R1, R2, +1>(v0, buf, twL, t, s, P);
@@ -764,7 +907,8 @@ __global__ __launch_bounds__(2 * R1 * C) void kr_pass3y(float2* __restrict__ a2,
   const int jr = min(ky, gg.my - ky);
   const float sxy = ((kx <= gg.tx) == (ky <= gg.ty)) ? 1.f : -1.f;
   const float sx = kx <= gg.tx ? 1.f : -1.f, sy = ky <= gg.ty ? 1.f : -1.f;
-  const size_t pb = (size_t)ir + (size_t)ox * jr;
+  const char* const gb = reinterpret_cast<const char*>(green);
+  const unsigned pb48 = (unsigned)(ir + ox * jr) * 48u, st48 = (unsigned)(ox * oy) * 48u;
 #pragma unroll
   for (int i = 0; i < G; ++i) {
     if (!reg2_owns<R1, R2>(t, i)) continue;
@@ -773,18 +917,15 @@ __global__ __launch_bounds__(2 * R1 * C) void kr_pass3y(float2* __restrict__ a2,
       const int kz = (t + R1 * i) + R2 * k1;
       const int kr = min(kz, gg.mz - kz);
       const float sz = kz <= gg.tz ? 1.f : -1.f;
-      const Green6 G = green_at(green, pb + (size_t)ox * oy * kr);
-      const float2 g11 = G.xx;
+      const Green6 G = green_at_b(gb, pb48 + (unsigned)kr * st48);
       const float2 g12 = cscale(G.xy, sxy);
       const float2 g13 = cscale(G.xz, sx * sz);
-      const float2 g22 = G.yy;
       const float2 g23 = cscale(G.yz, sy * sz);
-      const float2 g33 = G.zz;
       const int e = i * R1 + k1;
       const float2 p0 = v0[e], p1 = v1[e], p2 = v2[e];
-      v0[e] = cadd(cadd(cmul(g11, p0), cmul(g12, p1)), cmul(g13, p2));
-      v1[e] = cadd(cadd(cmul(g12, p0), cmul(g22, p1)), cmul(g23, p2));
-      v2[e] = cadd(cadd(cmul(g13, p0), cmul(g23, p1)), cmul(g33, p2));
+      v0[e] = cdot3(G.xx, p0, g12, p1, g13, p2);
+      v1[e] = cdot3(g12, p0, G.yy, p1, g23, p2);
+      v2[e] = cdot3(g13, p0, g23, p1, G.zz, p2);
     }
   }
   __syncthreads();
@@ -798,10 +939,10 @@ __global__ __launch_bounds__(2 * R1 * C) void kr_pass3y(float2* __restrict__ a2,
     for (int n2 = 0; n2 < R2; ++n2) {
       const int n = t + R1 * n2;
       if (lower_half<R2>(n2, n, nz)) {
-        const size_t o = (size_t)n * plane;
-        col[o] = v0[n2];
-        col[o + cs] = v1[n2];
-        col[o + 2 * cs] = v2[n2];
+        const unsigned o = ob + (unsigned)n2 * sb;
+        *at(0, o) = v0[n2];
+        *at(1, o) = v1[n2];
+        *at(2, o) = v2[n2];
       }
     }
   }
@@ -1275,6 +1416,25 @@ static void launch_p3m(Context& c, int cj, int compact, const float2* x, float2*
                      c.ny, c.nz, m.gx, m.gy, m.p2, bt.k, green_geo(c), g3);
 }
 
+// kr_pass3s addresses a component of a2 and the Green array with 32-bit
+// byte offsets
+static bool p3s_fits(const Context& c) {
+  const size_t comp = (size_t)c.nz * c.mv.p2 * sizeof(float2);
+  const size_t gbytes = green_points(c.nx, c.ny, c.nz, c.ipbc) * 6 * sizeof(float2);
+  return comp < ((size_t)1 << 32) && gbytes < ((size_t)1 << 32);
+}
+// pass 3 by kr_pass3s (register Stockham); serves pass 3 only
+template <class PL, int C, int NBUF, int K, int WPE>
+static void launch_p3s(Context& c, int cj, int compact, const float2* x, float2* y, const float2* adia,
+                       const float2* aoff, hipStream_t st, const MvBatch& bt, int which) {
+  (void)cj; (void)compact; (void)x; (void)y; (void)adia; (void)aoff;
+  if (which != 3) return;
+  const MatvecPlan& m = c.mv;
+  const P3Grid g3 = p3_grid(m.gx, m.gy, C, bt.nrhs);
+  hipLaunchKernelGGL((kr_pass3s<PL, C, NBUF, K, WPE>), dim3(p3_blocks(g3)), dim3(PL::T * C), 0, st, bt.a2, c.green, m.twz,
+                     c.nx, c.ny, c.nz, m.gx, m.gy, m.p2, bt.k, green_geo(c), g3);
+}
+
 // pass 3 by component-split thread groups (kr_pass3c); serves pass 3 only
 template <int R1, int R2, int C>
 static void launch_split3(Context& c, int cj, int compact, const float2* x, float2* y, const float2* adia,
@@ -1354,6 +1514,18 @@ static const RegShape kPairShapes[] = {
     RegShape{512, 16, 16, 4, &launch_p5m<2, 4, false>, 1 << 5}, RegShape{512, 16, 16, 8, &launch_p5m<2, 8, false>, 1 << 5},
     RegShape{768, 16, 16, 4, &launch_p5m<3, 4, true>, 1 << 5}, RegShape{768, 16, 16, 4, &launch_p5m<3, 4, false>, 1 << 5},
     RegShape{768, 16, 16, 8, &launch_p5m<3, 8, false>, 1 << 5},
+    // register Stockham Green passes: SkPlan<V, radices>, C columns, NBUF
+    // exchange buffers, K components per exchange, waves per SIMD the
+    // registers must allow (r1 = values per thread, r2 = first radix)
+    RegShape{512, 8, 8, 16, &launch_p3s<SkPlan<8, 8, 8, 8>, 16, 1, 1, 4>, 1 << 3},
+    RegShape{512, 8, 8, 8, &launch_p3s<SkPlan<8, 8, 8, 8>, 8, 2, 1, 3>, 1 << 3},
+    RegShape{512, 8, 8, 8, &launch_p3s<SkPlan<8, 8, 8, 8>, 8, 1, 3, 2>, 1 << 3},
+    RegShape{512, 8, 8, 4, &launch_p3s<SkPlan<8, 8, 8, 8>, 4, 1, 3, 3>, 1 << 3},
+    RegShape{512, 8, 8, 4, &launch_p3s<SkPlan<8, 8, 8, 8>, 4, 1, 3, 2>, 1 << 3},
+    RegShape{256, 8, 8, 8, &launch_p3s<SkPlan<8, 8, 8, 4>, 8, 2, 1, 4>, 1 << 3},
+    RegShape{256, 8, 8, 16, &launch_p3s<SkPlan<8, 8, 8, 4>, 16, 2, 1, 4>, 1 << 3},
+    RegShape{256, 8, 8, 8, &launch_p3s<SkPlan<8, 8, 8, 4>, 8, 1, 3, 3>, 1 << 3},
+    RegShape{256, 8, 8, 16, &launch_p3s<SkPlan<8, 8, 8, 4>, 16, 1, 3, 2>, 1 << 3},
 };
 #undef DDH_PAIR
 #undef DDH_SHAPE
@@ -1580,6 +1752,9 @@ int reg_pass(Context& c, int which, int cand, int cj, int compact, const float2*
   const int L = (which == 1 || which == 5) ? m.gx : (which == 3 ? m.gz : m.gy);
   const RegShape* r = cand >= 0 ? find_shape(L, which, cand) : nullptr;
   if (!r) return 0;
+  // register Green passes address a2 and the Green array with 32-bit byte
+  // offsets; larger grids take the LDS-staged kernel
+  if (which == 3 && !p3s_fits(c)) return 0;
   r->fn(c, cj, compact, x, y, adia, aoff, st, bt, which);
   return 1;
 }
