@@ -564,7 +564,10 @@ __device__ __forceinline__ void sk_inv_bfly(float2* v, const float2* __restrict_
 // buffers, so a write needs no barrier of its own (every thread passed the
 // previous exchange's barrier after its reads of the buffer it now
 // overwrites).
-template <class PL, int C, int NBUF, int K>
+// RC: the stage functions recompute the exchange addresses from fresh
+// copies of the thread indices at every exchange instead of letting the
+// compiler keep them live across transforms (fewer registers, more VALU).
+template <class PL, int C, int NBUF, int K, bool RC = false>
 struct SkLds {
   float2* xb;
   int e = 0;
@@ -582,13 +585,16 @@ struct SkLds {
 // sequence q), so one barrier serves all K and a stage's twiddles are
 // read once for the K.
 // forward stages s .. S-1 (stage s-1's butterflies done, its outputs in v)
-template <class PL, int s, int C, int NBUF, int K, int DIR>
-__device__ __forceinline__ void sk_fwd_from(float2* v, SkLds<PL, C, NBUF, K>& lx, const float2* __restrict__ twL,
+template <class PL, int s, int C, int NBUF, int K, int DIR, bool RC>
+__device__ __forceinline__ void sk_fwd_from(float2* v, SkLds<PL, C, NBUF, K, RC>& lx, const float2* __restrict__ twL,
                                             int j, int col) {
   if constexpr (s < PL::S) {
     constexpr int V = PL::V, LC = PL::L * C;
     constexpr int Rp = PL::rad(s - 1), NBp = V / Rp, R = PL::rad(s), NB = V / R;
     float2* xb = lx.next();
+    // fresh copies of the thread indices: the exchange addresses are then
+    // recomputed here instead of kept live from an earlier transform
+    if constexpr (RC) asm volatile("" : "+v"(j), "+v"(col));
 #pragma unroll
     for (int q = 0; q < K; ++q)
 #pragma unroll
@@ -610,16 +616,16 @@ __device__ __forceinline__ void sk_fwd_from(float2* v, SkLds<PL, C, NBUF, K>& lx
   }
 }
 // whole forward transform of K sequences: stage-0 gathers in, last stage's outputs out
-template <class PL, int C, int NBUF, int K, int DIR>
-__device__ __forceinline__ void sk_forward(float2* v, SkLds<PL, C, NBUF, K>& lx, const float2* __restrict__ twL, int j,
+template <class PL, int C, int NBUF, int K, int DIR, bool RC>
+__device__ __forceinline__ void sk_forward(float2* v, SkLds<PL, C, NBUF, K, RC>& lx, const float2* __restrict__ twL, int j,
                                            int col) {
 #pragma unroll
   for (int q = 0; q < K; ++q) sk_fwd_bfly<PL, 0, DIR>(v + q * PL::V, twL, j);
   sk_fwd_from<PL, 1, C, NBUF, K, DIR>(v, lx, twL, j, col);
 }
 // transposed stages s .. 0 (v holds stage s's scatter positions)
-template <class PL, int s, int C, int NBUF, int K, int DIR>
-__device__ __forceinline__ void sk_inv_from(float2* v, SkLds<PL, C, NBUF, K>& lx, const float2* __restrict__ twL,
+template <class PL, int s, int C, int NBUF, int K, int DIR, bool RC>
+__device__ __forceinline__ void sk_inv_from(float2* v, SkLds<PL, C, NBUF, K, RC>& lx, const float2* __restrict__ twL,
                                             int j, int col) {
   constexpr int V = PL::V, LC = PL::L * C;
 #pragma unroll
@@ -627,6 +633,7 @@ __device__ __forceinline__ void sk_inv_from(float2* v, SkLds<PL, C, NBUF, K>& lx
   if constexpr (s > 0) {
     constexpr int R = PL::rad(s), NB = V / R, Rn = PL::rad(s - 1), NBn = V / Rn;
     float2* xb = lx.next();
+    if constexpr (RC) asm volatile("" : "+v"(j), "+v"(col));
 #pragma unroll
     for (int q = 0; q < K; ++q)
 #pragma unroll
@@ -647,8 +654,8 @@ __device__ __forceinline__ void sk_inv_from(float2* v, SkLds<PL, C, NBUF, K>& lx
 }
 // whole inverse transform of K sequences: the forward's output distribution
 // in, stage-0 gather positions out
-template <class PL, int C, int NBUF, int K, int DIR>
-__device__ __forceinline__ void sk_inverse(float2* v, SkLds<PL, C, NBUF, K>& lx, const float2* __restrict__ twL, int j,
+template <class PL, int C, int NBUF, int K, int DIR, bool RC>
+__device__ __forceinline__ void sk_inverse(float2* v, SkLds<PL, C, NBUF, K, RC>& lx, const float2* __restrict__ twL, int j,
                                            int col) {
   sk_inv_from<PL, PL::S - 1, C, NBUF, K, DIR>(v, lx, twL, j, col);
 }

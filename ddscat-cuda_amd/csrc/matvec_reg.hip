@@ -457,24 +457,29 @@ __global__ __launch_bounds__(R1 * C) void kr_pass3(float2* __restrict__ a2, cons
   const float sx = kx <= gg.tx ? 1.f : -1.f, sy = ky <= gg.ty ? 1.f : -1.f;
   const char* const gb = reinterpret_cast<const char*>(green);
   const unsigned pb48 = (unsigned)(ir + ox * jr) * 48u, st48 = (unsigned)(ox * oy) * 48u;
+  // multiply of point e = i R1 + k1 (kz = t + R1 i + R2 k1) by its entry
+  auto gmul = [&](const Green6& Gm, int i, int k1) {
+    const int kz = (t + R1 * i) + R2 * k1;
+    const float sz = kz <= gg.tz ? 1.f : -1.f;
+    const float2 g12 = cscale(Gm.xy, sxy);
+    const float2 g13 = cscale(Gm.xz, sx * sz);
+    const float2 g23 = cscale(Gm.yz, sy * sz);
+    const int e = i * R1 + k1;
+    const float2 p0 = v0[e], p1 = v1[e], p2 = v2[e];
+    v0[e] = cdot3(Gm.xx, p0, g12, p1, g13, p2);
+    v1[e] = cdot3(g12, p0, Gm.yy, p1, g23, p2);
+    v2[e] = cdot3(g13, p0, g23, p1, Gm.zz, p2);
+  };
+  auto gload = [&](int i, int k1) {
+    const int kz = (t + R1 * i) + R2 * k1;
+    const int kr = min(kz, gg.mz - kz);
+    return green_at_b(gb, pb48 + (unsigned)kr * st48);
+  };
 #pragma unroll
   for (int i = 0; i < G; ++i) {
     if (!reg2_owns<R1, R2>(t, i)) continue;
 #pragma unroll
-    for (int k1 = 0; k1 < R1; ++k1) {
-      const int kz = (t + R1 * i) + R2 * k1;
-      const int kr = min(kz, gg.mz - kz);
-      const float sz = kz <= gg.tz ? 1.f : -1.f;
-      const Green6 G = green_at_b(gb, pb48 + (unsigned)kr * st48);
-      const float2 g12 = cscale(G.xy, sxy);
-      const float2 g13 = cscale(G.xz, sx * sz);
-      const float2 g23 = cscale(G.yz, sy * sz);
-      const int e = i * R1 + k1;
-      const float2 p0 = v0[e], p1 = v1[e], p2 = v2[e];
-      v0[e] = cdot3(G.xx, p0, g12, p1, g13, p2);
-      v1[e] = cdot3(g12, p0, G.yy, p1, g23, p2);
-      v2[e] = cdot3(g13, p0, g23, p1, G.zz, p2);
-    }
+    for (int k1 = 0; k1 < R1; ++k1) gmul(gload(i, k1), i, k1);
   }
   __syncthreads();
   fft2_inv<R1, R2, -1>(v0, buf, twL, t, s, P);
@@ -763,7 +768,7 @@ __global__ __launch_bounds__(M * 16 * C) void kr_pass3m(float2* __restrict__ a2,
 // uniform base per component plus a 32-bit byte offset per lane (one
 // register per address instead of two); the launcher checks that a
 // component of a2 and the Green array span less than 4 GiB.
-template <class PL, int C, int NBUF, int K, int WPE, bool SLY = false>
+template <class PL, int C, int NBUF, int K, int WPE, bool SLY = false, int GS = P3S_GREEN_GROUP, bool RC = false>
 __global__ __launch_bounds__(PL::T * C) __attribute__((amdgpu_waves_per_eu(WPE))) void kr_pass3s(
     float2* __restrict__ a2, const float2* __restrict__ green, const float2* __restrict__ twg, int nx, int ny, int nz,
     int gx, int gy, size_t plane, BatchArgs ba, GreenGeo gg, P3Grid g3) {
@@ -807,7 +812,7 @@ __global__ __launch_bounds__(PL::T * C) __attribute__((amdgpu_waves_per_eu(WPE))
     }
   __syncthreads();
   // K = 3: the three components advance stage by stage together; K = 1: one after another
-  SkLds<PL, C, NBUF, K> lx{xb};
+  SkLds<PL, C, NBUF, K, RC> lx{xb};
 #pragma unroll
   for (int c = 0; c < 3; c += K) sk_forward<PL, C, NBUF, K, +1>(v[c], lx, twL, j, col);
   // 3x3 symmetric Green multiply at k = i + r L / R_last (octant point
@@ -829,8 +834,8 @@ __global__ __launch_bounds__(PL::T * C) __attribute__((amdgpu_waves_per_eu(WPE))
       v[0][e] = cdot3(G.xx, p0, g12, p1, g13, p2);
       v[1][e] = cdot3(g12, p0, G.yy, p1, g23, p2);
       v[2][e] = cdot3(g13, p0, g23, p1, G.zz, p2);
-      // at most P3S_GREEN_GROUP points' Green loads (12 registers each) in flight
-      if ((e + 1) % P3S_GREEN_GROUP == 0) __builtin_amdgcn_sched_barrier(0);
+      // at most GS points' Green loads (12 registers each) in flight
+      if ((e + 1) % GS == 0) __builtin_amdgcn_sched_barrier(0);
     }
 #pragma unroll
   for (int c = 0; c < 3; c += K) sk_inverse<PL, C, NBUF, K, -1>(v[c], lx, twL, j, col);
@@ -1424,14 +1429,14 @@ static bool p3s_fits(const Context& c) {
   return comp < ((size_t)1 << 32) && gbytes < ((size_t)1 << 32);
 }
 // pass 3 by kr_pass3s (register Stockham); serves pass 3 only
-template <class PL, int C, int NBUF, int K, int WPE>
+template <class PL, int C, int NBUF, int K, int WPE, int GS = P3S_GREEN_GROUP, bool RC = false>
 static void launch_p3s(Context& c, int cj, int compact, const float2* x, float2* y, const float2* adia,
                        const float2* aoff, hipStream_t st, const MvBatch& bt, int which) {
   (void)cj; (void)compact; (void)x; (void)y; (void)adia; (void)aoff;
   if (which != 3) return;
   const MatvecPlan& m = c.mv;
   const P3Grid g3 = p3_grid(m.gx, m.gy, C, bt.nrhs);
-  hipLaunchKernelGGL((kr_pass3s<PL, C, NBUF, K, WPE>), dim3(p3_blocks(g3)), dim3(PL::T * C), 0, st, bt.a2, c.green, m.twz,
+  hipLaunchKernelGGL((kr_pass3s<PL, C, NBUF, K, WPE, false, GS, RC>), dim3(p3_blocks(g3)), dim3(PL::T * C), 0, st, bt.a2, c.green, m.twz,
                      c.nx, c.ny, c.nz, m.gx, m.gy, m.p2, bt.k, green_geo(c), g3);
 }
 
@@ -1516,16 +1521,18 @@ static const RegShape kPairShapes[] = {
     RegShape{768, 16, 16, 8, &launch_p5m<3, 8, false>, 1 << 5},
     // register Stockham Green passes: SkPlan<V, radices>, C columns, NBUF
     // exchange buffers, K components per exchange, waves per SIMD the
-    // registers must allow (r1 = values per thread, r2 = first radix)
+    // registers must allow, Green points per load group, exchange addresses
+    // recomputed per exchange (r1 = values per thread, r2 = first radix)
     RegShape{512, 8, 8, 16, &launch_p3s<SkPlan<8, 8, 8, 8>, 16, 1, 1, 4>, 1 << 3},
+    RegShape{512, 8, 8, 16, &launch_p3s<SkPlan<8, 8, 8, 8>, 16, 1, 1, 4, 4, true>, 1 << 3},
     RegShape{512, 8, 8, 8, &launch_p3s<SkPlan<8, 8, 8, 8>, 8, 2, 1, 3>, 1 << 3},
-    RegShape{512, 8, 8, 8, &launch_p3s<SkPlan<8, 8, 8, 8>, 8, 1, 3, 2>, 1 << 3},
-    RegShape{512, 8, 8, 4, &launch_p3s<SkPlan<8, 8, 8, 8>, 4, 1, 3, 3>, 1 << 3},
-    RegShape{512, 8, 8, 4, &launch_p3s<SkPlan<8, 8, 8, 8>, 4, 1, 3, 2>, 1 << 3},
-    RegShape{256, 8, 8, 8, &launch_p3s<SkPlan<8, 8, 8, 4>, 8, 2, 1, 4>, 1 << 3},
-    RegShape{256, 8, 8, 16, &launch_p3s<SkPlan<8, 8, 8, 4>, 16, 2, 1, 4>, 1 << 3},
-    RegShape{256, 8, 8, 8, &launch_p3s<SkPlan<8, 8, 8, 4>, 8, 1, 3, 3>, 1 << 3},
-    RegShape{256, 8, 8, 16, &launch_p3s<SkPlan<8, 8, 8, 4>, 16, 1, 3, 2>, 1 << 3},
+    RegShape{512, 8, 8, 8, &launch_p3s<SkPlan<8, 8, 8, 8>, 8, 2, 1, 4, 4, true>, 1 << 3},
+    RegShape{512, 16, 8, 8, &launch_p3s<SkPlan<16, 8, 8, 8>, 8, 1, 1, 2>, 1 << 3},
+    RegShape{768, 12, 12, 8, &launch_p3s<SkPlan<12, 12, 4, 4, 4>, 8, 2, 1, 2>, 1 << 3},
+    RegShape{768, 12, 12, 8, &launch_p3s<SkPlan<12, 12, 4, 4, 4>, 8, 1, 1, 2>, 1 << 3},
+    RegShape{768, 12, 12, 4, &launch_p3s<SkPlan<12, 12, 4, 4, 4>, 4, 1, 1, 3, 4, true>, 1 << 3},
+    RegShape{256, 8, 8, 8, &launch_p3s<SkPlan<8, 8, 8, 4>, 8, 1, 1, 4, 4, true>, 1 << 3},
+    RegShape{256, 8, 8, 16, &launch_p3s<SkPlan<8, 8, 8, 4>, 16, 1, 1, 4, 4, true>, 1 << 3},
 };
 #undef DDH_PAIR
 #undef DDH_SHAPE
