@@ -126,3 +126,93 @@ def test_dda_sphere_against_mie_and_optical_theorem():
     assert abs(cext / pia2 - qe_mie) < 0.05 * qe_mie
     assert abs(sc["csca"] / pia2 - qs_mie) < 0.06 * qs_mie
     assert abs(cext - cabs - sc["csca"]) < 0.01 * cext
+
+
+def _rings(dirs, ak):
+    """theta of every direction and the distinct ring angles (FP32 directions)."""
+    kh = np.asarray(ak, float) / np.linalg.norm(ak)
+    par = dirs @ kh
+    perp = np.linalg.norm(dirs - par[:, None] * kh[None, :], axis=1)
+    th = np.arctan2(perp, par)  # accurate next to the poles, unlike arccos
+    rings = np.unique(np.round(th, 6))
+    return th, rings
+
+
+@pytest.mark.parametrize("x_target, etasca", [(2.0, 0.5), (10.0, 0.25), (10.0, 0.1), (48.0, 0.5)])
+def test_quadrature_ring_spacing_rule(x_target, etasca):
+    """The ring count and theta map (cu:849-875) put the last ring spacing,
+    next to theta = pi, at (pi/2) eta / (3 + x) and the first one, next to
+    the forward direction, at theta0 / (1 + theta0) (1 + theta0 / (pi +
+    theta0)^2) of that (theta0 = 2 pi / (1 + x), finer where the diffraction
+    peak is).  Both follow from
+    the map's derivative at its ends, theta'(s=0) = theta0 / (1 + theta0)
+    and theta'(s_max) = 1 / (1 + theta0 / (pi + theta0)^2), with the ring
+    count 1 + round(2 (3 + x) / eta * (1 + 1/(pi + theta0)) /
+    (1 + theta0/(pi + theta0)^2)).  A wrong end point, a mis-scaled map or
+    a ring count off by a few fails; the exact count within one is the
+    reference's definitional rounding (cu:849-875), which no rule of the
+    geometry fixes."""
+    nat0 = 20000
+    kd = x_target / (3 * nat0 / (4 * math.pi)) ** (1 / 3)
+    ak = [kd, 0.0, 0.0]
+    dirs, w = O.scat_integration_dirs(ak, [0, 1, 0], etasca, nat0)
+    th, rings = _rings(dirs, ak)
+    nth = len(rings)
+    x = kd * (3 * nat0 / (4 * math.pi)) ** (1 / 3)
+    theta0 = 2 * math.pi / (1 + x)
+    back = (math.pi / 2) * etasca / (3 + x)
+    assert rings[0] == 0.0 and abs(rings[-1] - math.pi) < 1e-5
+    # each end spacing is the map's slope there times the s step ds, plus
+    # the curvature over one step (relative size below ds at both ends) and
+    # the rounding of the ring count (relative 1 / n_theta)
+    ds = (math.pi + math.pi / (theta0 + math.pi)) / (nth - 1)
+    slack = 2.0 / nth + ds
+    assert abs((rings[-1] - rings[-2]) / back - 1) < slack, (rings[-1] - rings[-2], back)
+    fwd = back * theta0 / (1 + theta0) * (1 + theta0 / (math.pi + theta0) ** 2)
+    assert abs((rings[1] - rings[0]) / fwd - 1) < slack, (rings[1] - rings[0], fwd)
+    # each interior ring carries n_phi = round(2 pi sin(theta) / dtheta) >= 3
+    # equally weighted azimuths (cells about as wide as the ring spacing
+    # dtheta, half the distance between its neighbours): the ring's weights
+    # sum to its solid-angle band
+    for i in (1, nth // 2, nth - 2):
+        sel = np.abs(th - rings[i]) < 2e-6
+        nphi = int(sel.sum())
+        assert nphi >= 3
+        band = 2 * math.pi * (math.cos(0.5 * (rings[i - 1] + rings[i])) - math.cos(0.5 * (rings[i] + rings[i + 1])))
+        assert abs(w[sel, 0].sum() - band) < 1e-3 * band
+        dth = 0.5 * (rings[i + 1] - rings[i - 1])
+        assert abs(nphi - 2 * math.pi * math.sin(rings[i]) / dth) <= 0.5 + 1e-3 or nphi == 3
+
+
+@pytest.mark.parametrize("etasca", [1.0, 0.5, 0.25])
+def test_quadrature_integrates_low_harmonics(etasca):
+    """Integrals over the sphere of the degree <= 4 functions the far-field
+    sums meet (the dipole pattern 1 - (n.a1)^2, cos^2, cos^4, and odd or
+    azimuthal terms that must vanish) converge to their exact values at the
+    rate of a midpoint rule in theta, O(dtheta^2): halving ETASCA cuts the
+    error about four times."""
+    nat0 = 5000
+    ak = np.array([0.4, 0.0, 0.0])
+    dirs, w = O.scat_integration_dirs(ak, [0, 1, 0], etasca, nat0)
+    n = dirs / np.linalg.norm(dirs, axis=1)[:, None]
+    ct, a1c, a2c = n[:, 0], n[:, 1], n[:, 2]
+    wt = w[:, 0]
+    exact = {
+        "dipole": (lambda: 1 - a1c ** 2, 8 * math.pi / 3),
+        "cos2": (lambda: ct ** 2, 4 * math.pi / 3),
+        "cos4": (lambda: ct ** 4, 4 * math.pi / 5),
+        "cos": (lambda: ct, 0.0),
+        "a1a2": (lambda: a1c * a2c, 0.0),
+        "a1": (lambda: a1c, 0.0),
+        "a2^2-a1^2": (lambda: a2c ** 2 - a1c ** 2, 0.0),
+    }
+    errs = {k: abs(float(np.sum(wt * f())) - v) for k, (f, v) in exact.items()}
+    # the step ~ (pi/2) eta / (3 + x) sets the bound
+    x = 0.4 * (3 * nat0 / (4 * math.pi)) ** (1 / 3)
+    h = (math.pi / 2) * etasca / (3 + x)
+    for k, e in errs.items():
+        assert e < 4 * math.pi * h * h, (k, e, h)
+    # the weight columns are the moments of the direction cosines
+    assert np.allclose(w[:, 1], wt * ct, atol=1e-7)
+    assert np.allclose(w[:, 4], wt * ct ** 2, atol=1e-7)
+    assert np.allclose(w[:, 2] ** 2 + w[:, 3] ** 2, (wt ** 2) * (1 - ct ** 2), atol=1e-7)
