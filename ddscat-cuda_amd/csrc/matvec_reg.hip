@@ -1709,10 +1709,19 @@ static void launch_sly_m(Context& c, float2* sl, int k0, int nb, hipStream_t st)
   hipLaunchKernelGGL((kr_pass3m<M, C, true>), dim3((m.gx + C - 1) / C, nb, 1), dim3(M * 16 * C), 0, st, sl, c.green,
                      m.twy, c.nx, c.ny, c.nz, m.gx, m.gy, (size_t)k0, BatchArgs{}, green_geo(c), P3Grid{});
 }
+// SLICES y pass by the register Stockham kernel (kr_pass3s along y)
+template <class PL, int C, int NBUF, int K, int WPE, int GS, bool RC>
+static void launch_sly_s(Context& c, float2* sl, int k0, int nb, hipStream_t st) {
+  const MatvecPlan& m = c.mv;
+  hipLaunchKernelGGL((kr_pass3s<PL, C, NBUF, K, WPE, true, GS, RC>), dim3((m.gx + C - 1) / C, nb, 1), dim3(PL::T * C),
+                     0, st, sl, c.green, m.twy, c.nx, c.ny, c.nz, m.gx, m.gy, (size_t)k0, BatchArgs{}, green_geo(c),
+                     P3Grid{});
+}
 using SlyLaunch = void (*)(Context&, float2*, int, int, hipStream_t);
 struct SlyShape {
   int L;
   SlyLaunch fn;
+  bool off32 = false;  // addresses the Green array with 32-bit byte offsets
 };
 // the default pass-3 shape of each register length
 static const SlyShape kSlyShapes[] = {
@@ -1722,13 +1731,17 @@ static const SlyShape kSlyShapes[] = {
     {120, &launch_sly<12, 10, 16>},  {128, &launch_sly<8, 16, 32>},   {144, &launch_sly<12, 12, 16>},
     {150, &launch_sly<15, 10, 16>},  {160, &launch_sly<16, 10, 16>},  {180, &launch_sly<15, 12, 16>},
     {192, &launch_sly<16, 12, 16>},  {240, &launch_sly<16, 15, 16>},  {256, &launch_sly<16, 16, 16>},
-    {512, &launch_sly_m<2, 16>},     {768, &launch_sly_m<3, 8>},
+    {512, &launch_sly_s<SkPlan<8, 8, 8, 8>, 8, 2, 1, 4, 4, true>, true},
+    {768, &launch_sly_s<SkPlan<12, 12, 4, 4, 4>, 8, 2, 1, 2, 4, false>, true},
 };
 
 int slices_y_green_reg(Context& c, float2* sl, int k0, int nb, hipStream_t st) {
   if (c.fft_generic) return 0;
   for (const SlyShape& r : kSlyShapes)
     if (r.L == c.mv.gy) {
+      // the slice buffers are small; a Green array of 4 GiB or more takes
+      // the LDS-staged kernel
+      if (r.off32 && green_points(c.nx, c.ny, c.nz, c.ipbc) * 6 * sizeof(float2) >= ((size_t)1 << 32)) return 0;
       r.fn(c, sl, k0, nb, st);
       return 1;
     }
