@@ -48,21 +48,31 @@ struct BatchState {
   double* packed = nullptr; // [B][3] status, resid, flags
   double* hpacked = nullptr;
   int* hflags = nullptr;    // pinned [B]
+  // iteration bookkeeping on the device (kb_step_end): [4][B] ints =
+  // iterations, check request, active flags saved at a pause, reliable-update
+  // MATVECs; then the pause flag.  hist: recursive residual of iteration i
+  // at row (i - 1) mod HIST_ROWS.  hctl / hhist: pinned copies.
+  int* ctl = nullptr;
+  double* hist = nullptr;
+  int* hctl = nullptr;
+  double* hhist = nullptr;
   int capB = 0;
 };
+constexpr int HIST_ROWS = 32;  // > RELIABLE_PERIOD: a chunk never wraps
 BatchState g_bs[MAX_TARGETS + 1];  // one per target slot
 BatchState& bstate(const Context& c) { return g_bs[c.slot]; }
 
 void bs_free(BatchState& S) {
   for (void** p : {(void**)&S.W, (void**)&S.a1, (void**)&S.a2, (void**)&S.scal, (void**)&S.part,
-                   (void**)&S.active, (void**)&S.sel, (void**)&S.m1, (void**)&S.m2, (void**)&S.packed}) {
+                   (void**)&S.active, (void**)&S.sel, (void**)&S.m1, (void**)&S.m2, (void**)&S.packed,
+                   (void**)&S.ctl, (void**)&S.hist}) {
     if (*p) hipFree(*p);
     *p = nullptr;
   }
-  if (S.hpacked) hipHostFree(S.hpacked);
-  if (S.hflags) hipHostFree(S.hflags);
-  S.hpacked = nullptr;
-  S.hflags = nullptr;
+  for (void** p : {(void**)&S.hpacked, (void**)&S.hflags, (void**)&S.hctl, (void**)&S.hhist}) {
+    if (*p) hipHostFree(*p);
+    *p = nullptr;
+  }
   S.cap_vec = S.cap_a1 = S.cap_a2 = 0;
   S.capB = 0;
 }
@@ -94,14 +104,14 @@ int bs_ensure(Context& c, BatchState& S, int B, int nvec) {
   }
   if (B > S.capB) {
     for (void** p : {(void**)&S.scal, (void**)&S.part, (void**)&S.active, (void**)&S.sel, (void**)&S.m1,
-                     (void**)&S.m2, (void**)&S.packed}) {
+                     (void**)&S.m2, (void**)&S.packed, (void**)&S.ctl, (void**)&S.hist}) {
       if (*p) hipFree(*p);
       *p = nullptr;
     }
-    if (S.hpacked) hipHostFree(S.hpacked);
-    if (S.hflags) hipHostFree(S.hflags);
-    S.hpacked = nullptr;
-    S.hflags = nullptr;
+    for (void** p : {(void**)&S.hpacked, (void**)&S.hflags, (void**)&S.hctl, (void**)&S.hhist}) {
+      if (*p) hipHostFree(*p);
+      *p = nullptr;
+    }
     S.capB = 0;
     DDH_TRY(hipMalloc((void**)&S.scal, (size_t)B * SCAL_COUNT * sizeof(double)), "batched scalars", 29);
     DDH_TRY(hipMalloc((void**)&S.part, (size_t)B * 2 * DOT_MAX * BMAX_BLOCKS * sizeof(double)), "batched partials",
@@ -113,6 +123,10 @@ int bs_ensure(Context& c, BatchState& S, int B, int nvec) {
     DDH_TRY(hipMalloc((void**)&S.packed, (size_t)B * 3 * sizeof(double)), "batched status", 29);
     DDH_TRY(hipHostMalloc((void**)&S.hpacked, (size_t)B * 3 * sizeof(double)), "pinned status", 29);
     DDH_TRY(hipHostMalloc((void**)&S.hflags, (size_t)B * sizeof(int)), "pinned flags", 29);
+    DDH_TRY(hipMalloc((void**)&S.ctl, ((size_t)4 * B + 1) * sizeof(int)), "batched iteration state", 29);
+    DDH_TRY(hipMalloc((void**)&S.hist, (size_t)HIST_ROWS * B * sizeof(double)), "batched residual history", 29);
+    DDH_TRY(hipHostMalloc((void**)&S.hctl, ((size_t)4 * B + 1) * sizeof(int)), "pinned iteration state", 29);
+    DDH_TRY(hipHostMalloc((void**)&S.hhist, (size_t)HIST_ROWS * B * sizeof(double)), "pinned residual history", 29);
     S.capB = B;
   }
   S.B = B;
@@ -512,6 +526,50 @@ __global__ void kb_pack(const double* __restrict__ scal, double* __restrict__ ou
   out[3 * r + 2] = scal[(size_t)r * SCAL_COUNT + SL_FLAGS];
 }
 
+// End of a batched iteration, on the device, so that several iterations run
+// between host synchronisations: for every RHS that was active, count the
+// iteration (and, P2, a reliable-update MATVEC when flag bit 0 is set), keep
+// its recursive residual in the history, retire it on a breakdown status,
+// and request a host check when the reliable-residual period is reached,
+// the residual is at or below tol, or maxit is reached.  A request pauses
+// the whole batch: the active flags are saved and cleared, so the rest of
+// the launched iterations exit on entry until the host has run the checks.
+// One workgroup.
+template <bool P2>
+__global__ void kb_step_end(const double* __restrict__ packed, int* __restrict__ active, int* __restrict__ ctl,
+                            double* __restrict__ hist, int B, double tol, int maxit, int period) {
+  __shared__ int pause;
+  if (threadIdx.x == 0) pause = 0;
+  __syncthreads();
+  int* it = ctl;
+  int* sel = ctl + B;
+  int* sav = ctl + 2 * B;
+  int* nxt = ctl + 3 * B;
+  for (int r = threadIdx.x; r < B; r += blockDim.x) {
+    if (!active[r]) continue;
+    const double st = packed[3 * r], rs = packed[3 * r + 1];
+    const int k = ++it[r];
+    if (P2) nxt[r] += ((int)packed[3 * r + 2]) & 1;
+    hist[(size_t)((k - 1) % HIST_ROWS) * B + r] = rs;
+    if (st != 0.0) {
+      active[r] = 0;
+      continue;
+    }
+    if (k % period == 0 || rs <= tol || k >= maxit) {
+      sel[r] = 1;
+      pause = 1;
+    }
+  }
+  __syncthreads();
+  if (pause) {
+    for (int r = threadIdx.x; r < B; r += blockDim.x) {
+      sav[r] = active[r];
+      active[r] = 0;
+    }
+    if (threadIdx.x == 0) ctl[4 * B] = 1;
+  }
+}
+
 }  // namespace
 
 // ------------------------------------- single-RHS QMRCCG and PBCGS2 steps --
@@ -681,6 +739,29 @@ struct Ops {
             "batched status D2H", 1);
     return 0;
   }
+  // device iteration bookkeeping after the pack of an iteration
+  int step_end(bool p2, double tol, int maxit) {
+    if (p2) hipLaunchKernelGGL(kb_step_end<true>, dim3(1), dim3(256), 0, st, S.packed, S.active, S.ctl, S.hist, S.B,
+                               tol, maxit, RELIABLE_PERIOD);
+    else hipLaunchKernelGGL(kb_step_end<false>, dim3(1), dim3(256), 0, st, S.packed, S.active, S.ctl, S.hist, S.B,
+                            tol, maxit, RELIABLE_PERIOD);
+    DDH_LAUNCH_OK("batched step end", 1);
+    return 0;
+  }
+  // check requests and the pause flag cleared before a chunk of iterations
+  int chunk_reset() {
+    DDH_TRY(hipMemsetAsync(S.ctl + S.B, 0, (size_t)S.B * sizeof(int), st), "batched iteration state", 1);
+    DDH_TRY(hipMemsetAsync(S.ctl + 4 * S.B, 0, sizeof(int), st), "batched iteration state", 1);
+    return 0;
+  }
+  // iteration state and residual history to the host, then a sync
+  int chunk_fetch() {
+    DDH_TRY(hipMemcpyAsync(S.hctl, S.ctl, ((size_t)4 * S.B + 1) * sizeof(int), hipMemcpyDeviceToHost, st),
+            "batched iteration state D2H", 1);
+    DDH_TRY(hipMemcpyAsync(S.hhist, S.hist, (size_t)HIST_ROWS * S.B * sizeof(double), hipMemcpyDeviceToHost, st),
+            "batched residual history D2H", 1);
+    return sync();
+  }
   int upload(int* dev, const std::vector<int>& h) {
     for (int r = 0; r < S.B; ++r) S.hflags[r] = h[r];
     DDH_TRY(hipMemcpyAsync(dev, S.hflags, (size_t)S.B * sizeof(int), hipMemcpyHostToDevice, st), "flags H2D", 1);
@@ -740,6 +821,25 @@ int run_step(Ops& o, Graph& gr, bool use_graph, F&& body) {
   }
   DDH_TRY(hipGraphLaunch(gr.e, o.st), "graph launch", 1);
   return 0;
+}
+
+// Iterations the next chunk runs before the host looks again: up to the
+// next reliable-residual check (every active RHS has the same iteration
+// count) and maxit, and no further than the first active RHS is expected to
+// reach tol at the residual decay of the last chunk (each iteration launched
+// after a pause costs only its early-exit launches, each host round trip a
+// synchronisation).  Grows from 2 while no estimate is available.
+int next_chunk(int itc, int maxit, int prev_k, double pred) {
+  int k = RELIABLE_PERIOD - itc % RELIABLE_PERIOD;
+  k = std::min(k, std::max(1, maxit - itc));
+  const int grow = std::min(RELIABLE_PERIOD, std::max(2, 2 * prev_k));
+  const int est = pred < 1e9 ? std::max(1, (int)std::floor(0.8 * pred)) : grow;
+  return std::max(1, std::min(k, std::min(grow, est)));
+}
+// iterations until RHS k reaches tol at the decay from r0 to r1 over n iterations
+double iters_to_tol(double r0, double r1, int n, double tol) {
+  if (n <= 0 || !(r0 > 0.0) || !(r1 > 0.0) || !(r1 < r0) || !(r1 > tol)) return 1e9;
+  return std::log(tol / r1) / (std::log(r1 / r0) / n);
 }
 
 // ----------------------------------------------------------- QMRCCG -------
@@ -846,27 +946,43 @@ int bsolve_qmrccg(Context& c, BatchState& S, const float2* b, float2* x, const f
                          S.part, o.nbk_vec);
     if (o.fin(SLOT_R2, 2, A, FIN_RESID)) return 59;
     DDH_LAUNCH_OK("QMRCCG batch step", 56);
-    return o.pack();
+    if (o.pack()) return 56;
+    return o.step_end(false, tol, maxit);
   };
+  // iterations run in chunks: kb_step_end counts them on the device and
+  // pauses the batch when a RHS needs the host (check, convergence, maxit)
+  DDH_TRY(hipMemsetAsync(S.ctl, 0, ((size_t)4 * B + 1) * sizeof(int), o.st), nm, 51);
+  int kchunk = 0;
   while (any_set(act)) {
-    const std::vector<int> prev = act;
-    if (int rc = run_step(o, gr, use_graph, body)) return rc;
-    if (o.sync()) return 59;
+    int itc = 0;
+    for (int k = 0; k < B; ++k)
+      if (act[k]) { itc = it[k]; break; }
+    kchunk = next_chunk(itc, maxit, kchunk, 1e9);
+    if (o.chunk_reset()) return 59;
+    for (int i = 0; i < kchunk; ++i)
+      if (int rc = run_step(o, gr, use_graph, body)) return rc;
+    if (o.chunk_fetch()) return 59;
     std::fill(sel.begin(), sel.end(), 0);
+    double pred = 1e9;
     for (int k = 0; k < B; ++k) {
       if (!act[k]) continue;
-      ++it[k];
-      nc[k] += 2;
+      const int n_it = S.hctl[k] - it[k];
+      nc[k] += 2 * n_it;
       const int status = (int)S.hpacked[3 * k];
+      for (int j = 0; j < (status ? n_it - 1 : n_it); ++j)
+        c.log.iter(k, S.hhist[(size_t)((it[k] + j) % HIST_ROWS) * B + k]);
+      it[k] = S.hctl[k];
       if (status) {
         rcode[k] = 58;
         act[k] = 0;
         continue;
       }
+      const double r0 = resid[k];
       resid[k] = S.hpacked[3 * k + 1];
-      c.log.iter(k, resid[k]);
-      if ((it[k] % RELIABLE_PERIOD) == 0 || resid[k] <= tol || it[k] >= maxit) sel[k] = 1;
+      sel[k] = S.hctl[B + k];
+      pred = std::min(pred, iters_to_tol(r0, resid[k], n_it, tol));
     }
+    kchunk = next_chunk(0, RELIABLE_PERIOD, kchunk, pred);
     if (any_set(sel)) {
       if (int rc = reliable(conv, rst, tres)) return rc;
       std::vector<int> restart(B, 0), replace(B, 0);
@@ -901,7 +1017,8 @@ int bsolve_qmrccg(Context& c, BatchState& S, const float2* b, float2* x, const f
         }
       }
     }
-    if (act != prev && o.upload(S.active, act)) return 59;
+    // a pause cleared the device flags
+    if (o.upload(S.active, act)) return 59;
   }
   DDH_TRY(hipMemcpyAsync(x, xs, vb, hipMemcpyDeviceToDevice, o.st), nm, 61);
   if (o.sync()) return 61;
@@ -999,32 +1116,49 @@ int bsolve_pbcgs2(Context& c, BatchState& S, const float2* b, float2* x, const f
     { LinComb cp; cp.lit(r0, 1.0); if (o.lin(bp, cp, S.m2)) return 104; }
     hipLaunchKernelGGL(kb_p2_reliable, sg, sb, 0, o.st, S.scal, S.m1, S.m2, B);
     DDH_LAUNCH_OK("PBCGS2 batch step", 103);
-    return o.pack();
+    if (o.pack()) return 103;
+    return o.step_end(true, tol, maxit);
   };
   Graph gr;
   std::vector<double> bn;
+  // iterations in chunks between host checks, as in the batched QMRCCG
+  DDH_TRY(hipMemsetAsync(S.ctl, 0, ((size_t)4 * B + 1) * sizeof(int), o.st), nm, 92);
+  std::vector<int> nxt(B, 0);
+  int kchunk = 0;
   while (any_set(act)) {
-    const std::vector<int> prev = act;
-    if (int rc = run_step(o, gr, use_graph, body)) return rc;
-    if (o.sync()) return 103;
+    int itc = 0;
+    for (int k = 0; k < B; ++k)
+      if (act[k]) { itc = it[k]; break; }
+    kchunk = next_chunk(itc, maxit, kchunk, 1e9);
+    if (o.chunk_reset()) return 103;
+    for (int i = 0; i < kchunk; ++i)
+      if (int rc = run_step(o, gr, use_graph, body)) return rc;
+    if (o.chunk_fetch()) return 103;
     std::fill(sel.begin(), sel.end(), 0);
+    double pred = 1e9;
     for (int k = 0; k < B; ++k) {
       if (!act[k]) continue;
-      ++it[k];
+      const int n_it = S.hctl[k] - it[k];
+      const int nx_now = S.hctl[3 * B + k];
+      nc[k] += 4 * n_it + (nx_now - nxt[k]);
+      nxt[k] = nx_now;
       const int status = (int)S.hpacked[3 * k];
-      const int flags = (int)S.hpacked[3 * k + 2];
-      nc[k] += 4 + ((flags & 1) ? 1 : 0);
+      for (int j = 0; j < (status ? n_it - 1 : n_it); ++j)
+        c.log.iter(k, S.hhist[(size_t)((it[k] + j) % HIST_ROWS) * B + k]);
+      it[k] = S.hctl[k];
       if (status) {
         rcode[k] = status;
         act[k] = 0;
         continue;
       }
+      const double r0 = resid[k];
       resid[k] = S.hpacked[3 * k + 1];
-      c.log.iter(k, resid[k]);
-      if ((it[k] % RELIABLE_PERIOD) == 0 || resid[k] <= tol || it[k] >= maxit) sel[k] = 1;
+      sel[k] = S.hctl[B + k];
+      pred = std::min(pred, iters_to_tol(r0, resid[k], n_it, tol));
     }
+    kchunk = next_chunk(0, RELIABLE_PERIOD, kchunk, pred);
     if (!any_set(sel)) {
-      if (act != prev && o.upload(S.active, act)) return 104;
+      if (o.upload(S.active, act)) return 104;
       continue;
     }
     // true residual |bp - A x| and gap |bp - A x - r0| (cu:543-556)
@@ -1062,7 +1196,8 @@ int bsolve_pbcgs2(Context& c, BatchState& S, const float2* b, float2* x, const f
       hipLaunchKernelGGL(kb_p2_restart, sg, sb, 0, o.st, S.scal, Sm, B);
       DDH_LAUNCH_OK("PBCGS2 batch restart", 104);
     }
-    if (act != prev && o.upload(S.active, act)) return 104;
+    // a pause cleared the device flags
+    if (o.upload(S.active, act)) return 104;
   }
   // ZBCG2 returns X + XP with one final true residual (cu:1258-1264)
   std::fill(act.begin(), act.end(), 1);
