@@ -123,7 +123,9 @@ def time_to_qext(lib, tg, kd, m=1.5 + 0.1j, method="qmrccg", tol=1e-5, maxit=100
         dt = time.perf_counter() - t0
         out[mode] = dict(seconds=round(dt, 4), rhs=per if B <= 2 else None,
                          mean_iterations=float(np.mean([p["iterations"] for p in per])),
-                         converged=all(p["resid"] <= 1.01 * tol and p["rc"] == 0 for p in per))
+                         converged=all(p["resid"] <= 1.01 * tol and p["rc"] == 0 for p in per),
+                         unconverged_rhs=[dict(rhs=r, iterations=p["iterations"], resid=p["resid"], rc=p["rc"])
+                                          for r, p in enumerate(per) if not (p["resid"] <= 1.01 * tol and p["rc"] == 0)])
         out[mode]["_per"] = per
     lib.set_precision(PREC_MIXED)
     xd.zero_()
@@ -137,7 +139,9 @@ def time_to_qext(lib, tg, kd, m=1.5 + 0.1j, method="qmrccg", tol=1e-5, maxit=100
                   qabs=q[r, 0] / geo) for r in range(B)]
     out["batched_mixed"] = dict(seconds=round(dt, 4), graph=graph, rc=rc, rhs=per_b if B <= 2 else None,
                                 mean_iterations=float(np.mean(it)),
-                                converged=bool(rc == 0 and np.all(te <= 1.01 * tol)))
+                                converged=bool(rc == 0 and np.all(te <= 1.01 * tol)),
+                                unconverged_rhs=[dict(rhs=r, iterations=int(it[r]), resid=float(te[r]))
+                                                 for r in range(B) if not te[r] <= 1.01 * tol])
     f, mx, bm = out["plain_fp32"], out["mixed"], out["batched_mixed"]
     qdiff = max(abs(a["qext"] - b["qext"]) / abs(b["qext"]) for a, b in zip(f.pop("_per"), per_b))
     mx.pop("_per")
