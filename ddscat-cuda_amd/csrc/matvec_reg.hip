@@ -399,8 +399,10 @@ static P3Grid p3_grid(int gx, int gy, int C, int nrhs) {
 }
 static unsigned p3_blocks(const P3Grid& g3) { return (unsigned)(((g3.units + 7) / 8) * 8 * g3.nrhs); }
 
-template <int R1, int R2, int C>
-__global__ __launch_bounds__(R1 * C) void kr_pass3(float2* __restrict__ a2, const float2* __restrict__ green,
+// PRE > 0: the Green loads of a thread's first PRE points are issued before
+// the third component's forward transform, so they arrive while it runs.
+template <int R1, int R2, int C, int PRE = 0, int WPE = 1>
+__global__ __launch_bounds__(R1 * C) __attribute__((amdgpu_waves_per_eu(WPE))) void kr_pass3(float2* __restrict__ a2, const float2* __restrict__ green,
                                                const float2* __restrict__ twg, int nx, int ny, int nz, int gx,
                                                int gy, size_t plane, BatchArgs ba, GreenGeo gg, P3Grid g3) {
   constexpr int L = R1 * R2, RT = R1 * C, P = C + 1, G = Reg2<R1, R2>::G, VN = Reg2<R1, R2>::VN;
@@ -445,7 +447,6 @@ __global__ __launch_bounds__(R1 * C) void kr_pass3(float2* __restrict__ a2, cons
   __syncthreads();
   fft2_fwd<R1, R2, +1>(v1, buf, twL, t, s, P);
   __syncthreads();
-  fft2_fwd<R1, R2, +1>(v2, buf, twL, t, s, P);
   // 3x3 symmetric Green multiply; octant parity reconstruction for IPBC=0 (cu:109-115), whole grid for IPBC=1.
   // Each octant entry serves up to eight (kx, ky, kz) mirrors; kz mirrors
   // are in this block and the kx / ky mirror blocks run on the same XCD at
@@ -475,11 +476,19 @@ __global__ __launch_bounds__(R1 * C) void kr_pass3(float2* __restrict__ a2, cons
     const int kr = min(kz, gg.mz - kz);
     return green_at_b(gb, pb48 + (unsigned)kr * st48);
   };
+  Green6 gpre[PRE > 0 ? PRE : 1];
+  if constexpr (PRE > 0) {
+    static_assert(G == 1 && PRE <= R1, "early Green loads: one k2 group per thread");
+#pragma unroll
+    for (int p = 0; p < PRE; ++p) gpre[p] = gload(0, p);
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  fft2_fwd<R1, R2, +1>(v2, buf, twL, t, s, P);
 #pragma unroll
   for (int i = 0; i < G; ++i) {
     if (!reg2_owns<R1, R2>(t, i)) continue;
 #pragma unroll
-    for (int k1 = 0; k1 < R1; ++k1) gmul(gload(i, k1), i, k1);
+    for (int k1 = 0; k1 < R1; ++k1) gmul((PRE > 0 && k1 < PRE) ? gpre[k1 < PRE ? k1 : 0] : gload(i, k1), i, k1);
   }
   __syncthreads();
   fft2_inv<R1, R2, -1>(v0, buf, twL, t, s, P);
@@ -1428,6 +1437,18 @@ static bool p3s_fits(const Context& c) {
   const size_t gbytes = green_points(c.nx, c.ny, c.nz, c.ipbc) * 6 * sizeof(float2);
   return comp < ((size_t)1 << 32) && gbytes < ((size_t)1 << 32);
 }
+// pass 3 by kr_pass3 with early Green loads; serves pass 3 only
+template <int R1, int R2, int C, int PRE, int WPE = 1>
+static void launch_p3pre(Context& c, int cj, int compact, const float2* x, float2* y, const float2* adia,
+                         const float2* aoff, hipStream_t st, const MvBatch& bt, int which) {
+  (void)cj; (void)compact; (void)x; (void)y; (void)adia; (void)aoff;
+  if (which != 3) return;
+  const MatvecPlan& m = c.mv;
+  const P3Grid g3 = p3_grid(m.gx, m.gy, C, bt.nrhs);
+  hipLaunchKernelGGL((kr_pass3<R1, R2, C, PRE, WPE>), dim3(p3_blocks(g3)), dim3(R1 * C), 0, st, bt.a2, c.green, m.twz,
+                     c.nx, c.ny, c.nz, m.gx, m.gy, m.p2, bt.k, green_geo(c), g3);
+}
+
 // pass 3 by kr_pass3s (register Stockham); serves pass 3 only
 template <class PL, int C, int NBUF, int K, int WPE, int GS = P3S_GREEN_GROUP, bool RC = false>
 static void launch_p3s(Context& c, int cj, int compact, const float2* x, float2* y, const float2* adia,
@@ -1533,6 +1554,10 @@ static const RegShape kPairShapes[] = {
     RegShape{768, 12, 12, 4, &launch_p3s<SkPlan<12, 12, 4, 4, 4>, 4, 1, 1, 3, 4, true>, 1 << 3},
     RegShape{256, 8, 8, 8, &launch_p3s<SkPlan<8, 8, 8, 4>, 8, 1, 1, 4, 4, true>, 1 << 3},
     RegShape{256, 8, 8, 16, &launch_p3s<SkPlan<8, 8, 8, 4>, 16, 1, 1, 4, 4, true>, 1 << 3},
+    // 256-point Green pass with the first two points' Green loads issued
+    // before the third forward transform (three or four points: 170-192
+    // VGPRs and 2 waves per SIMD, or spills at 3)
+    RegShape{256, 16, 2, 16, &launch_p3pre<16, 16, 16, 2>, 1 << 3},
 };
 #undef DDH_PAIR
 #undef DDH_SHAPE
