@@ -777,7 +777,8 @@ __global__ __launch_bounds__(M * 16 * C) void kr_pass3m(float2* __restrict__ a2,
 // uniform base per component plus a 32-bit byte offset per lane (one
 // register per address instead of two); the launcher checks that a
 // component of a2 and the Green array span less than 4 GiB.
-template <class PL, int C, int NBUF, int K, int WPE, bool SLY = false, int GS = P3S_GREEN_GROUP, bool RC = false>
+template <class PL, int C, int NBUF, int K, int WPE, bool SLY = false, int GS = P3S_GREEN_GROUP, bool RC = false,
+          int PRE = 0>
 __global__ __launch_bounds__(PL::T * C) __attribute__((amdgpu_waves_per_eu(WPE))) void kr_pass3s(
     float2* __restrict__ a2, const float2* __restrict__ green, const float2* __restrict__ twg, int nx, int ny, int nz,
     int gx, int gy, size_t plane, BatchArgs ba, GreenGeo gg, P3Grid g3) {
@@ -822,20 +823,36 @@ __global__ __launch_bounds__(PL::T * C) __attribute__((amdgpu_waves_per_eu(WPE))
   __syncthreads();
   // K = 3: the three components advance stage by stage together; K = 1: one after another
   SkLds<PL, C, NBUF, K, RC> lx{xb};
-#pragma unroll
-  for (int c = 0; c < 3; c += K) sk_forward<PL, C, NBUF, K, +1>(v[c], lx, twL, j, col);
   // 3x3 symmetric Green multiply at k = i + r L / R_last (octant point
-  // pb + step min(k, mf - k), 48 bytes each)
+  // pb + step min(k, mf - k), 48 bytes each); with PRE > 0 (components
+  // one after another) the first PRE points' entries are loaded before the
+  // last component's forward transform
   const char* const gb = reinterpret_cast<const char*>(green);
   const unsigned pb48 = (unsigned)q.pb * 48u, st48 = (unsigned)q.step * 48u;
+  auto gload = [&](int e) {
+    const int b = e / RL, r = e % RL, k = j + T * b + r * (L / RL);
+    return green_at_b(gb, pb48 + (unsigned)min(k, q.mf - k) * st48);
+  };
+  Green6 gpre[PRE > 0 ? PRE : 1];
+  static_assert(PRE == 0 || K == 1, "early Green loads with the components one after another");
+#pragma unroll
+  for (int c = 0; c < 3; c += K) {
+    if constexpr (PRE > 0) {
+      if (c == 2) {
+#pragma unroll
+        for (int p = 0; p < PRE; ++p) gpre[p] = gload(p);
+        __builtin_amdgcn_sched_barrier(0);
+      }
+    }
+    sk_forward<PL, C, NBUF, K, +1>(v[c], lx, twL, j, col);
+  }
 #pragma unroll
   for (int b = 0; b < V / RL; ++b)
 #pragma unroll
     for (int r = 0; r < RL; ++r) {
       const int e = b * RL + r, k = j + T * b + r * (L / RL);
-      const int kr = min(k, q.mf - k);
       const float sf = k <= q.tf ? 1.f : -1.f;
-      const Green6 G = green_at_b(gb, pb48 + (unsigned)kr * st48);
+      const Green6 G = e < PRE ? gpre[e < PRE ? e : 0] : gload(e);
       const float2 g12 = cscale(G.xy, SLY ? q.c12 * sf : q.c12);
       const float2 g13 = cscale(G.xz, SLY ? q.c13 : q.c13 * sf);
       const float2 g23 = cscale(G.yz, q.c23 * sf);
@@ -1450,14 +1467,14 @@ static void launch_p3pre(Context& c, int cj, int compact, const float2* x, float
 }
 
 // pass 3 by kr_pass3s (register Stockham); serves pass 3 only
-template <class PL, int C, int NBUF, int K, int WPE, int GS = P3S_GREEN_GROUP, bool RC = false>
+template <class PL, int C, int NBUF, int K, int WPE, int GS = P3S_GREEN_GROUP, bool RC = false, int PRE = 0>
 static void launch_p3s(Context& c, int cj, int compact, const float2* x, float2* y, const float2* adia,
                        const float2* aoff, hipStream_t st, const MvBatch& bt, int which) {
   (void)cj; (void)compact; (void)x; (void)y; (void)adia; (void)aoff;
   if (which != 3) return;
   const MatvecPlan& m = c.mv;
   const P3Grid g3 = p3_grid(m.gx, m.gy, C, bt.nrhs);
-  hipLaunchKernelGGL((kr_pass3s<PL, C, NBUF, K, WPE, false, GS, RC>), dim3(p3_blocks(g3)), dim3(PL::T * C), 0, st, bt.a2, c.green, m.twz,
+  hipLaunchKernelGGL((kr_pass3s<PL, C, NBUF, K, WPE, false, GS, RC, PRE>), dim3(p3_blocks(g3)), dim3(PL::T * C), 0, st, bt.a2, c.green, m.twz,
                      c.nx, c.ny, c.nz, m.gx, m.gy, m.p2, bt.k, green_geo(c), g3);
 }
 
@@ -1558,6 +1575,10 @@ static const RegShape kPairShapes[] = {
     // before the third forward transform (three or four points: 170-192
     // VGPRs and 2 waves per SIMD, or spills at 3)
     RegShape{256, 16, 2, 16, &launch_p3pre<16, 16, 16, 2>, 1 << 3},
+    // Stockham Green passes with the first two points' Green loads issued
+    // before the last component's forward transform
+    RegShape{512, 8, 8, 8, &launch_p3s<SkPlan<8, 8, 8, 8>, 8, 2, 1, 4, 4, true, 2>, 1 << 3},
+    RegShape{768, 12, 12, 8, &launch_p3s<SkPlan<12, 12, 4, 4, 4>, 8, 2, 1, 2, 4, false, 2>, 1 << 3},
 };
 #undef DDH_PAIR
 #undef DDH_SHAPE
