@@ -80,7 +80,10 @@ void bs_free(BatchState& S) {
 int bs_ensure(Context& c, BatchState& S, int B, int nvec) {
   const size_t n = (size_t)3 * c.nat0;
   const size_t need_vec = (size_t)nvec * B * n;
-  const size_t need_a1 = (size_t)B * matvec_a1_count(c), need_a2 = (size_t)B * matvec_a2_count(c);
+  // FFT work per RHS on FFT3D; SLICES runs the RHS through the context's slice buffers
+  const bool fft3d = c.backend == BACKEND_FFT3D;
+  const size_t need_a1 = fft3d ? (size_t)B * matvec_a1_count(c) : 0;
+  const size_t need_a2 = fft3d ? (size_t)B * matvec_a2_count(c) : 0;
   if (need_vec > S.cap_vec) {
     if (S.W) hipFree(S.W);
     S.W = nullptr;
@@ -644,6 +647,18 @@ int p21_comb(Context& c, float2* u0, float2* x, float2* r0, const float2* u1, co
   return 0;
 }
 
+int batch_matvec(Context& c, int cwhat, const float2* x, float2* y, const float2* adia, const float2* aoff,
+                 const MvBatch& bt, hipStream_t st) {
+  if (c.backend != BACKEND_SLICES) return matvec_launch(c, cwhat, x, y, adia, aoff, 1, st, nullptr, &bt);
+  for (int r = 0; r < bt.nrhs; ++r) {
+    const int* act = bt.k.active ? bt.k.active + r : nullptr;
+    if (int rc = slices_matvec(c, cwhat, x + r * bt.k.xs, y + r * bt.k.xs, adia + r * bt.k.os, aoff + r * bt.k.os, 1,
+                               st, act))
+      return rc;
+  }
+  return 0;
+}
+
 namespace {
 
 // ------------------------------------------------------------- host ops --
@@ -675,7 +690,7 @@ struct Ops {
     bt.k.active = mask;
     bt.a1 = S.a1;
     bt.a2 = S.a2;
-    return matvec_launch(c, 'N', x, y, adia, aoff, 1, st, nullptr, &bt);
+    return batch_matvec(c, 'N', x, y, adia, aoff, bt, st);
   }
   template <typename Acc>
   struct DotsRun {
@@ -1241,10 +1256,6 @@ extern "C" DDSCAT_HIP_API int ddscat_hip_apply_batch(int cwhat, int nrhs, const 
     set_error("apply_batch", "invalid argument");
     return 11;
   }
-  if (c.backend != BACKEND_FFT3D) {
-    set_error("apply_batch", "batched MATVEC runs on the FFT3D backend");
-    return 11;
-  }
   BatchState& S = bstate(c);
   if (int rc = bs_ensure(c, S, nrhs, 0)) return rc;
   MvBatch bt;
@@ -1256,8 +1267,7 @@ extern "C" DDSCAT_HIP_API int ddscat_hip_apply_batch(int cwhat, int nrhs, const 
   bt.a1 = S.a1;
   bt.a2 = S.a2;
   hipStream_t st = stream ? (hipStream_t)stream : c.stream;
-  return matvec_launch(c, cwhat, (const float2*)x, (float2*)y, (const float2*)adia, (const float2*)aoff, 1, st,
-                       nullptr, &bt);
+  return batch_matvec(c, cwhat, (const float2*)x, (float2*)y, (const float2*)adia, (const float2*)aoff, bt, st);
 }
 
 extern "C" DDSCAT_HIP_API int ddscat_hip_evalq_batch(int nrhs, const void* e, const void* p, const void* adia,
@@ -1321,8 +1331,8 @@ extern "C" DDSCAT_HIP_API int ddscat_hip_cabs_adjoint(int method, int nrhs, cons
     set_error("cabs_adjoint", "invalid argument");
     return 1;
   }
-  if (c.backend != BACKEND_FFT3D || (method != DDSCAT_HIP_QMRCCG && method != DDSCAT_HIP_PBCGS2)) {
-    set_error("cabs_adjoint", "adjoint solves run batched QMRCCG or PBCGS2 on the FFT3D backend");
+  if (method != DDSCAT_HIP_QMRCCG && method != DDSCAT_HIP_PBCGS2) {
+    set_error("cabs_adjoint", "adjoint solves run batched QMRCCG or PBCGS2");
     return 2;
   }
   BatchState& S = bstate(c);
@@ -1372,10 +1382,6 @@ extern "C" DDSCAT_HIP_API int ddscat_hip_solve_batch(int method, int nrhs, const
     set_error("solve_batch", "invalid argument");
     return 1;
   }
-  if (c.backend != BACKEND_FFT3D) {
-    set_error("solve_batch", "batched solves run on the FFT3D backend");
-    return 2;
-  }
   if (method != DDSCAT_HIP_QMRCCG && method != DDSCAT_HIP_PBCGS2) {
     set_error("solve_batch", "method not available in batched form");
     return 2;
@@ -1422,8 +1428,8 @@ extern "C" DDSCAT_HIP_API int ddscat_hip_solve_targets(int method, int ntargets,
     for (int u = 0; u < t; ++u)
       if (slots[u] == s) { set_error("solve_targets", "a target slot appears twice"); return 1; }
     Context& c = ctx_at(s);
-    if (!c.ready || c.backend != BACKEND_FFT3D) {
-      set_error("solve_targets", "target not prepared on the FFT3D backend");
+    if (!c.ready) {
+      set_error("solve_targets", "target not prepared");
       return 2;
     }
     first[t + 1] = first[t] + (size_t)nrhs[t];

@@ -1593,7 +1593,8 @@ static const RegShape kPairShapes[] = {
 template <int R1, int R2, int C, int DIR>
 __global__ __launch_bounds__(R1 * C) void kr_slx(float2* __restrict__ zw, float2* __restrict__ sl,
                                              const float2* __restrict__ twg, int nx, int ny, int gz, int k0,
-                                             int nrows) {
+                                             int nrows, const int* __restrict__ act) {
+  if (act && !*act) return;  // inactive right-hand side of a batch
   constexpr int L = R1 * R2, RT = R1 * C, P = row_stride<R1, R2>(), G = Reg2<R1, R2>::G, VN = Reg2<R1, R2>::VN;
   __shared__ float2 twL[L];
   __shared__ float buf[C * P];  // split exchange, row layout
@@ -1640,16 +1641,19 @@ __global__ __launch_bounds__(R1 * C) void kr_slx(float2* __restrict__ zw, float2
 }
 
 template <int R1, int R2, int C>
-static void launch_slx(Context& c, int dir, float2* zw, float2* sl, int k0, int nb, hipStream_t st) {
+static void launch_slx(Context& c, int dir, float2* zw, float2* sl, int k0, int nb, hipStream_t st,
+                       const int* act) {
   const MatvecPlan& m = c.mv;
   const int nrows = nb * 3 * c.ny;
   const dim3 grid((nrows + C - 1) / C);
   if (dir > 0)
-    hipLaunchKernelGGL((kr_slx<R1, R2, C, +1>), grid, dim3(R1 * C), 0, st, zw, sl, m.twx, c.nx, c.ny, m.gz, k0, nrows);
+    hipLaunchKernelGGL((kr_slx<R1, R2, C, +1>), grid, dim3(R1 * C), 0, st, zw, sl, m.twx, c.nx, c.ny, m.gz, k0, nrows,
+                       act);
   else
-    hipLaunchKernelGGL((kr_slx<R1, R2, C, -1>), grid, dim3(R1 * C), 0, st, zw, sl, m.twx, c.nx, c.ny, m.gz, k0, nrows);
+    hipLaunchKernelGGL((kr_slx<R1, R2, C, -1>), grid, dim3(R1 * C), 0, st, zw, sl, m.twx, c.nx, c.ny, m.gz, k0, nrows,
+                       act);
 }
-using SlxLaunch = void (*)(Context&, int, float2*, float2*, int, int, hipStream_t);
+using SlxLaunch = void (*)(Context&, int, float2*, float2*, int, int, hipStream_t, const int*);
 struct SlxShape {
   int L;
   SlxLaunch fn;
@@ -1665,11 +1669,11 @@ static const SlxShape kSlxShapes[] = {
     {512, &launch_slx<32, 16, 8>},   {768, &launch_slx<32, 24, 8>},
 };
 
-int slices_x_reg(Context& c, int dir, float2* zw, float2* sl, int k0, int nb, hipStream_t st) {
+int slices_x_reg(Context& c, int dir, float2* zw, float2* sl, int k0, int nb, hipStream_t st, const int* act) {
   if (c.fft_generic) return 0;
   for (const SlxShape& r : kSlxShapes)
     if (r.L == c.mv.gx) {
-      r.fn(c, dir, zw, sl, k0, nb, st);
+      r.fn(c, dir, zw, sl, k0, nb, st, act);
       return 1;
     }
   return 0;
@@ -1680,33 +1684,40 @@ int slices_x_reg(Context& c, int dir, float2* zw, float2* sl, int k0, int nb, hi
 // "row", columns at element stride NX NY).  dir = +1: in [3][NZ][NY][NX]
 // (the full-layout vector) -> out = zwork [3][2NZ][NY][NX]; dir = -1: zwork ->
 // out [3][NZ][NY][NX] (the NZ unpadded outputs).
+// one right-hand side (grid z = the 3 components, so RHS index 0) whose
+// active flag, when given, is *act
+static BatchArgs one_rhs(const int* act) {
+  BatchArgs ba;
+  ba.active = act;
+  return ba;
+}
 template <int R1, int R2, int C>
-static void launch_slz(Context& c, int dir, const float2* in, float2* out, hipStream_t st) {
+static void launch_slz(Context& c, int dir, const float2* in, float2* out, hipStream_t st, const int* act) {
   const MatvecPlan& m = c.mv;
   const int plane = c.nx * c.ny;
   const size_t full = (size_t)m.gz * plane, half = (size_t)c.nz * plane;
   const dim3 grid((plane + C - 1) / C, 1, 3);
   if (dir > 0)
     hipLaunchKernelGGL((kr_passy<R1, R2, C, +1>), grid, dim3(R1 * C), 0, st, in, out, m.twz, plane, c.nz, m.gz, 1, half,
-                       full, BatchArgs{});
+                       full, one_rhs(act));
   else
     hipLaunchKernelGGL((kr_passy<R1, R2, C, -1>), grid, dim3(R1 * C), 0, st, in, out, m.twz, plane, m.gz, c.nz, 1, full,
-                       half, BatchArgs{});
+                       half, one_rhs(act));
 }
 template <int M, int C>
-static void launch_slz_m(Context& c, int dir, const float2* in, float2* out, hipStream_t st) {
+static void launch_slz_m(Context& c, int dir, const float2* in, float2* out, hipStream_t st, const int* act) {
   const MatvecPlan& m = c.mv;
   const int plane = c.nx * c.ny;
   const size_t full = (size_t)m.gz * plane, half = (size_t)c.nz * plane;
   const dim3 grid((plane + C - 1) / C, 1, 3), block(M * 16 * C);
   if (dir > 0)
     hipLaunchKernelGGL((kr_passym<M, C, +1>), grid, block, 0, st, in, out, m.twz, plane, c.nz, m.gz, 1, half, full,
-                       BatchArgs{});
+                       one_rhs(act));
   else
     hipLaunchKernelGGL((kr_passym<M, C, -1>), grid, block, 0, st, in, out, m.twz, plane, m.gz, c.nz, 1, full, half,
-                       BatchArgs{});
+                       one_rhs(act));
 }
-using SlzLaunch = void (*)(Context&, int, const float2*, float2*, hipStream_t);
+using SlzLaunch = void (*)(Context&, int, const float2*, float2*, hipStream_t, const int*);
 struct SlzShape {
   int L;
   SlzLaunch fn;
@@ -1730,11 +1741,11 @@ int slices_z_reg_length(int L) {
   return 0;
 }
 
-int slices_z_reg(Context& c, int dir, const float2* in, float2* out, hipStream_t st) {
+int slices_z_reg(Context& c, int dir, const float2* in, float2* out, hipStream_t st, const int* act) {
   if (c.fft_generic) return 0;
   for (const SlzShape& r : kSlzShapes)
     if (r.L == c.mv.gz) {
-      r.fn(c, dir, in, out, st);
+      r.fn(c, dir, in, out, st, act);
       return 1;
     }
   return 0;
@@ -1744,26 +1755,26 @@ int slices_z_reg(Context& c, int dir, const float2* in, float2* out, hipStream_t
 // y-FFT, Green multiply at kz and inverse y-FFT of nb slice planes starting
 // at kz = k0, with the register kernels of pass 3 (column setup p3_col<true>)
 template <int R1, int R2, int C>
-static void launch_sly(Context& c, float2* sl, int k0, int nb, hipStream_t st) {
+static void launch_sly(Context& c, float2* sl, int k0, int nb, hipStream_t st, const int* act) {
   const MatvecPlan& m = c.mv;
   hipLaunchKernelGGL((kr_sly<R1, R2, C>), dim3((m.gx + C - 1) / C, nb, 1), dim3(R1 * C), 0, st, sl, c.green, m.twy,
-                     c.nx, c.ny, c.nz, m.gx, m.gy, (size_t)k0, BatchArgs{}, green_geo(c));
+                     c.nx, c.ny, c.nz, m.gx, m.gy, (size_t)k0, one_rhs(act), green_geo(c));
 }
 template <int M, int C>
-static void launch_sly_m(Context& c, float2* sl, int k0, int nb, hipStream_t st) {
+static void launch_sly_m(Context& c, float2* sl, int k0, int nb, hipStream_t st, const int* act) {
   const MatvecPlan& m = c.mv;
   hipLaunchKernelGGL((kr_pass3m<M, C, true>), dim3((m.gx + C - 1) / C, nb, 1), dim3(M * 16 * C), 0, st, sl, c.green,
-                     m.twy, c.nx, c.ny, c.nz, m.gx, m.gy, (size_t)k0, BatchArgs{}, green_geo(c), P3Grid{});
+                     m.twy, c.nx, c.ny, c.nz, m.gx, m.gy, (size_t)k0, one_rhs(act), green_geo(c), P3Grid{});
 }
 // SLICES y pass by the register Stockham kernel (kr_pass3s along y)
 template <class PL, int C, int NBUF, int K, int WPE, int GS, bool RC>
-static void launch_sly_s(Context& c, float2* sl, int k0, int nb, hipStream_t st) {
+static void launch_sly_s(Context& c, float2* sl, int k0, int nb, hipStream_t st, const int* act) {
   const MatvecPlan& m = c.mv;
   hipLaunchKernelGGL((kr_pass3s<PL, C, NBUF, K, WPE, true, GS, RC>), dim3((m.gx + C - 1) / C, nb, 1), dim3(PL::T * C),
-                     0, st, sl, c.green, m.twy, c.nx, c.ny, c.nz, m.gx, m.gy, (size_t)k0, BatchArgs{}, green_geo(c),
+                     0, st, sl, c.green, m.twy, c.nx, c.ny, c.nz, m.gx, m.gy, (size_t)k0, one_rhs(act), green_geo(c),
                      P3Grid{});
 }
-using SlyLaunch = void (*)(Context&, float2*, int, int, hipStream_t);
+using SlyLaunch = void (*)(Context&, float2*, int, int, hipStream_t, const int*);
 struct SlyShape {
   int L;
   SlyLaunch fn;
@@ -1781,14 +1792,14 @@ static const SlyShape kSlyShapes[] = {
     {768, &launch_sly_s<SkPlan<12, 12, 4, 4, 4>, 8, 2, 1, 2, 4, false>, true},
 };
 
-int slices_y_green_reg(Context& c, float2* sl, int k0, int nb, hipStream_t st) {
+int slices_y_green_reg(Context& c, float2* sl, int k0, int nb, hipStream_t st, const int* act) {
   if (c.fft_generic) return 0;
   for (const SlyShape& r : kSlyShapes)
     if (r.L == c.mv.gy) {
       // the slice buffers are small; a Green array of 4 GiB or more takes
       // the LDS-staged kernel
       if (r.off32 && green_points(c.nx, c.ny, c.nz, c.ipbc) * 6 * sizeof(float2) >= ((size_t)1 << 32)) return 0;
-      r.fn(c, sl, k0, nb, st);
+      r.fn(c, sl, k0, nb, st, act);
       return 1;
     }
   return 0;

@@ -36,7 +36,8 @@ static SliceState g_sl[MAX_TARGETS + 1];  // one per target slot
 __global__ __launch_bounds__(SL_THREADS) void k_sl_zfwd(const float2* __restrict__ x, const int* __restrict__ qmap,
                                                        float2* __restrict__ zw, const float2* __restrict__ twg,
                                                        FftPlan p, int nx, int ny, int nz, int nat, int nat0,
-                                                       int compact, int cj, int cz_log2) {
+                                                       int compact, int cj, int cz_log2, const int* __restrict__ act) {
+  if (act && !*act) return;  // inactive right-hand side of a batch
   extern __shared__ float2 sm[];
   const int L = p.L;
   const int cz = 1 << cz_log2;
@@ -76,7 +77,8 @@ __global__ __launch_bounds__(SL_THREADS) void k_sl_zfwd(const float2* __restrict
 // rows r = (b, c, iy) of the batch; slice[(b*3 + c)*ny + iy][0..2nx)
 __global__ __launch_bounds__(SL_THREADS) void k_sl_xfwd(const float2* __restrict__ zw, float2* __restrict__ sl,
                                                        const float2* __restrict__ twg, FftPlan p, int nx, int ny,
-                                                       int gz, int k0, int nb, int rx) {
+                                                       int gz, int k0, int nb, int rx, const int* __restrict__ act) {
+  if (act && !*act) return;  // inactive right-hand side of a batch
   extern __shared__ float2 sm[];
   const int L = p.L;
   float2* tw = sm;
@@ -107,7 +109,9 @@ __global__ __launch_bounds__(SL_THREADS) void k_sl_xfwd(const float2* __restrict
 // ------------------------------ y-FFT, Green multiply at kz, inverse y-FFT (in slice)
 __global__ __launch_bounds__(SL_THREADS) void k_sl_y_green(float2* __restrict__ sl, const float2* __restrict__ green,
                                                           const float2* __restrict__ twg, FftPlan p, int nx, int ny,
-                                                          int nz, int k0, int cy_log2, GreenGeo gg) {
+                                                          int nz, int k0, int cy_log2, GreenGeo gg,
+                                                          const int* __restrict__ act) {
+  if (act && !*act) return;  // inactive right-hand side of a batch
   extern __shared__ float2 sm[];
   const int L = p.L;  // 2 ny
   const int gx = 2 * nx;
@@ -176,7 +180,8 @@ __global__ __launch_bounds__(SL_THREADS) void k_sl_y_green(float2* __restrict__ 
 // ------------------------------------------ x-FFT inverse (slice -> zwork plane)
 __global__ __launch_bounds__(SL_THREADS) void k_sl_xinv(const float2* __restrict__ sl, float2* __restrict__ zw,
                                                        const float2* __restrict__ twg, FftPlan p, int nx, int ny,
-                                                       int gz, int k0, int nb, int rx) {
+                                                       int gz, int k0, int nb, int rx, const int* __restrict__ act) {
+  if (act && !*act) return;  // inactive right-hand side of a batch
   extern __shared__ float2 sm[];
   const int L = p.L;
   float2* tw = sm;
@@ -205,7 +210,8 @@ __global__ __launch_bounds__(SL_THREADS) void k_sl_zinv_finish(
     const float2* __restrict__ zw, const float2* __restrict__ twg, FftPlan p, const float2* __restrict__ x,
     float2* __restrict__ y, const float2* __restrict__ adia, const float2* __restrict__ aoff,
     const int16_t* __restrict__ iocc, const int* __restrict__ qmap, int nx, int ny, int nz, int nat, int nat0,
-    int compact, int cj, float inv, int cz_log2) {
+    int compact, int cj, float inv, int cz_log2, const int* __restrict__ act) {
+  if (act && !*act) return;  // inactive right-hand side of a batch
   extern __shared__ float2 sm[];
   const int L = p.L;  // 2 nz
   const int cz = 1 << cz_log2;
@@ -270,7 +276,8 @@ __global__ __launch_bounds__(SL_THREADS) void k_sl_zinv_finish(
 // x (compact via the site map, or full) into zin [3][NAT], conj for 'C'
 __global__ __launch_bounds__(SL_THREADS) void k_sl_gather(const float2* __restrict__ x, const int* __restrict__ qmap,
                                                          float2* __restrict__ zin, int nat, int nat0, int compact,
-                                                         int cj) {
+                                                         int cj, const int* __restrict__ act) {
+  if (act && !*act) return;  // inactive right-hand side of a batch
   const int c = blockIdx.y;
   const size_t stride = (size_t)gridDim.x * SL_THREADS;
   for (size_t j = (size_t)blockIdx.x * SL_THREADS + threadIdx.x; j < (size_t)nat; j += stride) {
@@ -293,7 +300,8 @@ __global__ __launch_bounds__(SL_THREADS) void k_sl_finish(const float2* __restri
                                                          const float2* __restrict__ aoff,
                                                          const int16_t* __restrict__ iocc,
                                                          const int* __restrict__ qmap, int nat, int nat0, int compact,
-                                                         int cj, float inv) {
+                                                         int cj, float inv, const int* __restrict__ act) {
+  if (act && !*act) return;  // inactive right-hand side of a batch
   const size_t gs = (size_t)gridDim.x * SL_THREADS;
   for (size_t j = (size_t)blockIdx.x * SL_THREADS + threadIdx.x; j < (size_t)nat; j += gs) {
     size_t o = j, sd = (size_t)nat;
@@ -408,7 +416,7 @@ void slices_release(Context& c) {
 }
 
 int slices_matvec(Context& c, int cwhat, const float2* x, float2* y, const float2* adia, const float2* aoff,
-                  int layout, hipStream_t st) {
+                  int layout, hipStream_t st, const int* act) {
   SliceState& s = g_sl[c.slot];
   const MatvecPlan& m = c.mv;
   if (!s.zwork) { set_error("SLICES MATVEC", "backend not prepared"); return 20; }
@@ -421,15 +429,15 @@ int slices_matvec(Context& c, int cwhat, const float2* x, float2* y, const float
     const float2* zsrc = x;
     if (compact || cj) {
       hipLaunchKernelGGL(k_sl_gather, dim3(ewg, 3), dim3(SL_THREADS), 0, st, x, c.qmap, s.zin, c.nat, c.nat0,
-                         compact, cj);
+                         compact, cj, act);
       zsrc = s.zin;
     }
-    slices_z_reg(c, +1, zsrc, s.zwork, st);
+    slices_z_reg(c, +1, zsrc, s.zwork, st, act);
   } else {
     dim3 grid((c.nx + s.cz - 1) / s.cz, c.ny, 3);
     const size_t lds = (size_t)(m.gz + s.cz * m.gz) * sizeof(float2);
     hipLaunchKernelGGL(k_sl_zfwd, grid, dim3(SL_THREADS), lds, st, x, c.qmap, s.zwork, m.twz, m.pz, c.nx, c.ny,
-                       c.nz, c.nat, c.nat0, compact, cj, lg2(s.cz));
+                       c.nz, c.nat, c.nat0, compact, cj, lg2(s.cz), act);
   }
   DDH_LAUNCH_OK("SLICES z FFT", 21);
   DDH_TRY(hipEventRecord(s.ev_in, st), "SLICES record", 25);
@@ -440,25 +448,25 @@ int slices_matvec(Context& c, int cwhat, const float2* x, float2* y, const float
     hipStream_t ss = s.side[k & 1];
     float2* sl = s.slice[k & 1];
     const int nrows = nb * 3 * c.ny;
-    if (!slices_x_reg(c, +1, s.zwork, sl, k0, nb, ss)) {
+    if (!slices_x_reg(c, +1, s.zwork, sl, k0, nb, ss, act)) {
       dim3 grid((nrows + s.rx_f - 1) / s.rx_f);
       const size_t lds = (size_t)(m.gx + s.rx_f * m.gx) * sizeof(float2);
       hipLaunchKernelGGL(k_sl_xfwd, grid, dim3(SL_THREADS), lds, ss, s.zwork, sl, m.twx, m.px, c.nx, c.ny, m.gz, k0,
-                         nb, s.rx_f);
+                         nb, s.rx_f, act);
     }
     DDH_LAUNCH_OK("SLICES x FFT", 23);
-    if (!slices_y_green_reg(c, sl, k0, nb, ss)) {
+    if (!slices_y_green_reg(c, sl, k0, nb, ss, act)) {
       dim3 grid((m.gx + s.cy - 1) / s.cy, nb);
       const size_t lds = (size_t)(m.gy + 3 * s.cy * m.gy) * sizeof(float2);
       hipLaunchKernelGGL(k_sl_y_green, grid, dim3(SL_THREADS), lds, ss, sl, c.green, m.twy, m.py, c.nx, c.ny, c.nz,
-                         k0, lg2(s.cy), green_geo(c));
+                         k0, lg2(s.cy), green_geo(c), act);
     }
     DDH_LAUNCH_OK("SLICES y FFT + Green", 24);
-    if (!slices_x_reg(c, -1, s.zwork, sl, k0, nb, ss)) {
+    if (!slices_x_reg(c, -1, s.zwork, sl, k0, nb, ss, act)) {
       dim3 grid((nrows + s.rx_i - 1) / s.rx_i);
       const size_t lds = (size_t)(m.gx + s.rx_i * m.gx) * sizeof(float2);
       hipLaunchKernelGGL(k_sl_xinv, grid, dim3(SL_THREADS), lds, ss, sl, s.zwork, m.twx, m.px, c.nx, c.ny, m.gz, k0,
-                         nb, s.rx_i);
+                         nb, s.rx_i, act);
     }
     DDH_LAUNCH_OK("SLICES x inverse FFT", 26);
   }
@@ -467,15 +475,15 @@ int slices_matvec(Context& c, int cwhat, const float2* x, float2* y, const float
     DDH_TRY(hipStreamWaitEvent(st, s.ev_done[i], 0), "SLICES wait", 25);
   }
   if (regz) {
-    slices_z_reg(c, -1, s.zwork, s.zin, st);
+    slices_z_reg(c, -1, s.zwork, s.zin, st, act);
     hipLaunchKernelGGL(k_sl_finish, dim3(ewg), dim3(SL_THREADS), 0, st, s.zin, x, y, adia, aoff,
-                       compact ? nullptr : c.iocc, c.qmap, c.nat, c.nat0, compact, cj, 1.f / (float)m.ngrid);
+                       compact ? nullptr : c.iocc, c.qmap, c.nat, c.nat0, compact, cj, 1.f / (float)m.ngrid, act);
   } else {
     dim3 grid((c.nx + s.cz - 1) / s.cz, c.ny);
     const size_t lds = (size_t)(m.gz + 3 * s.cz * m.gz) * sizeof(float2);
     hipLaunchKernelGGL(k_sl_zinv_finish, grid, dim3(SL_THREADS), lds, st, s.zwork, m.twz, m.pz, x, y, adia, aoff,
                        compact ? nullptr : c.iocc, c.qmap, c.nx, c.ny, c.nz, c.nat, c.nat0, compact, cj,
-                       1.f / (float)m.ngrid, lg2(s.cz));
+                       1.f / (float)m.ngrid, lg2(s.cz), act);
   }
   DDH_LAUNCH_OK("SLICES z inverse FFT + finish", 29);
   c.matvec_count++;

@@ -76,8 +76,15 @@ int scatter_full(Context& c, const float2* compact, float2* full, hipStream_t st
 // SLICES backend (low-memory MATVEC)
 int slices_setup(Context& c);
 void slices_release(Context& c);
+// act: the right-hand side's device-side active flag when the MATVEC is one
+// of a batch (every kernel exits at once when it reads 0), else nullptr
 int slices_matvec(Context& c, int cwhat, const float2* x, float2* y, const float2* adia,
-                  const float2* aoff, int layout, hipStream_t st);
+                  const float2* aoff, int layout, hipStream_t st, const int* act = nullptr);
+// batched MATVEC on either backend: FFT3D runs the B right-hand sides in each
+// launch (matvec_launch), SLICES runs them one after another through the
+// context's slice buffers, each honouring its active flag
+int batch_matvec(Context& c, int cwhat, const float2* x, float2* y, const float2* adia, const float2* aoff,
+                 const MvBatch& bt, hipStream_t st);
 
 int finish_prepare(Context& c, const int16_t* iocc, int code_alloc, int code_occ);
 // frees the batched-solver buffers (bsolve.hip)

@@ -142,16 +142,17 @@ int reg_candidates(int L, int which);
 int reg_pass(Context& c, int which, int cand, int cj, int compact, const float2* x, float2* y, const float2* adia,
              const float2* aoff, hipStream_t st, const MvBatch& bt);
 // SLICES y-FFT + Green + inverse y-FFT of nb slice planes from kz = k0 with
-// the register kernels when 2NY has a register shape; 1 = launched
-int slices_y_green_reg(Context& c, float2* sl, int k0, int nb, hipStream_t st);
+// the register kernels when 2NY has a register shape; 1 = launched.  act (all
+// SLICES launchers): the right-hand side's active flag in a batch, or nullptr.
+int slices_y_green_reg(Context& c, float2* sl, int k0, int nb, hipStream_t st, const int* act);
 // SLICES x-FFT (dir +1: zwork rows -> slice) or inverse x-FFT (dir -1:
 // slice -> zwork rows) of nb planes from kz = k0 with the register kernels
 // when 2NX has a register shape; 1 = launched
-int slices_x_reg(Context& c, int dir, float2* zw, float2* sl, int k0, int nb, hipStream_t st);
+int slices_x_reg(Context& c, int dir, float2* zw, float2* sl, int k0, int nb, hipStream_t st, const int* act);
 // SLICES z-FFT (dir +1: [3][NZ][NY][NX] -> zwork) or inverse z-FFT (dir -1:
 // zwork -> [3][NZ][NY][NX]) with the register column kernels when 2NZ has a
 // register shape; 1 = launched.  slices_z_reg_length: whether L has one.
-int slices_z_reg(Context& c, int dir, const float2* in, float2* out, hipStream_t st);
+int slices_z_reg(Context& c, int dir, const float2* in, float2* out, hipStream_t st, const int* act);
 int slices_z_reg_length(int L);
 // times every FFT kernel candidate of each pass and stores the fastest in c.mv.kind
 int matvec_autotune(Context& c);

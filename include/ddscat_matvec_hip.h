@@ -168,11 +168,13 @@ DDSCAT_HIP_API int ddscat_hip_solve_device(int method, const void *b, void *x, c
 
 /* nrhs MATVECs in one launch sequence on compact DEVICE vectors x, y
  * [nrhs][3][NAT0]; adia, aoff per RHS (op_per_rhs = 1) or shared (0).
- * Stream-ordered on `stream` (NULL = library stream), FFT3D backend. */
+ * Stream-ordered on `stream` (NULL = library stream).  FFT3D runs the nrhs
+ * right-hand sides in each kernel launch; SLICES runs them one after another
+ * through its slice buffers. */
 DDSCAT_HIP_API int ddscat_hip_apply_batch(int cwhat, int nrhs, const void *x, void *y, const void *adia,
                                           const void *aoff, int op_per_rhs, void *stream);
 
-/* Batched solve of nrhs right-hand sides of the prepared target (FFT3D
+/* Batched solve of nrhs right-hand sides of the prepared target (either
  * backend; method DDSCAT_HIP_QMRCCG or DDSCAT_HIP_PBCGS2).  b, x (initial guess in, solution out):
  * compact DEVICE vectors [nrhs][3][NAT0]; adia, aoff: [nrhs][3][NAT0] when
  * op_per_rhs = 1 (per-RHS LDR operators), one shared [3][NAT0] when 0.
@@ -183,7 +185,7 @@ DDSCAT_HIP_API int ddscat_hip_apply_batch(int cwhat, int nrhs, const void *x, vo
  * in chunks of up to 20 between host synchronisations, with the per-RHS
  * iteration counts and stopping tests kept on the device.  Returns 0, or
  * the breakdown code of the first RHS that broke down (the others are
- * still solved); 1 invalid argument, 2 method/backend not batched, 29 memory. */
+ * still solved); 1 invalid argument, 2 method not batched, 29 memory. */
 DDSCAT_HIP_API int ddscat_hip_solve_batch(int method, int nrhs, const void *b, void *x, const void *adia,
                                           const void *aoff, int op_per_rhs, int maxit, float tol, int *itno,
                                           float *tole, int *ncompte, int use_graph);
@@ -206,7 +208,7 @@ DDSCAT_HIP_API int ddscat_hip_select_target(int slot);
  * ddscat_hip_solve_batch on its own stream, driven from its own host thread,
  * so the targets' kernels share the GPU; the call returns when all are done.
  * itno, tole, ncompte: host arrays of sum(nrhs) entries, target after
- * target.  Every slot must be prepared on the FFT3D backend and appear once.
+ * target.  Every slot must be prepared (either backend) and appear once.
  * Returns 0 or the first target's nonzero code (1 invalid argument, 2
  * method/backend, 29 memory, else the solver's). */
 DDSCAT_HIP_API int ddscat_hip_solve_targets(int method, int ntargets, const int *slots, const int *nrhs,
@@ -230,7 +232,7 @@ DDSCAT_HIP_API int ddscat_hip_evalq_batch(int nrhs, const void *e, const void *p
  * DEVICE arrays.  One batched adjoint solve A mu = W conj(p) (method
  * DDSCAT_HIP_QMRCCG or DDSCAT_HIP_PBCGS2, tolerance tol) with the forward
  * MATVEC (A is complex symmetric); itno / tole [nrhs] report it.  Codes: 1
- * invalid argument, 2 method/backend, 3 launch, 29 memory, else the solver's. */
+ * invalid argument, 2 method, 3 launch, 29 memory, else the solver's. */
 DDSCAT_HIP_API int ddscat_hip_cabs_adjoint(int method, int nrhs, const void *p, const void *adia,
                                            const void *aoff, int op_per_rhs, const float *ak, float e02,
                                            int maxit, float tol, void *g_adia, void *g_aoff, int *itno,
