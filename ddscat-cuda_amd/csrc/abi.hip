@@ -257,6 +257,19 @@ extern "C" DDSCAT_HIP_API int ddscat_hip_force_restart(void) {
   return 0;
 }
 
+extern "C" DDSCAT_HIP_API int ddscat_hip_poison_workspace(void) {
+  Context& c = ctx();
+  if (!c.ready) { set_error("poison_workspace", "not prepared"); return 10; }
+  if (int rc = ensure_solver_workspace(c, 14)) return rc;
+  if (ensure_stream(c)) return 12;
+  // all-ones bytes: every float of the workspace reads as NaN
+  DDH_TRY(hipMemsetAsync(c.wrk, 0xFF, (size_t)c.wrk_vectors * 3 * c.nat0 * sizeof(float2), c.stream),
+          "poison_workspace", 12);
+  if (int rc = batch_poison(c)) return rc;
+  DDH_TRY(hipStreamSynchronize(c.stream), "poison_workspace", 12);
+  return 0;
+}
+
 extern "C" DDSCAT_HIP_API void* ddscat_hip_stream(void) {
   Context& c = ctx();
   if (ensure_stream(c)) return nullptr;
@@ -289,7 +302,6 @@ extern "C" DDSCAT_HIP_API int ddscat_hip_nearfield(const void* p_compact, const 
   Context& c = ctx();
   if (!c.ready) { set_error("nearfield", "not prepared"); return 10; }
   if (!p_compact || !out_full) { set_error("nearfield", "invalid argument"); return 11; }
-  if (c.backend != BACKEND_FFT3D) { set_error("nearfield", "near field runs on the FFT3D backend"); return 11; }
   clear_error();
   hipStream_t st = c.stream;
   const size_t n = (size_t)3 * c.nat;
@@ -298,7 +310,8 @@ extern "C" DDSCAT_HIP_API int ddscat_hip_nearfield(const void* p_compact, const 
   if (scatter_full(c, (const float2*)p_compact, c.xfull, st)) return 12;
   int16_t* keep = c.iocc;
   c.iocc = nullptr;  // full-layout finish without the occupancy mask
-  const int rc = matvec_launch(c, 'N', c.xfull, c.yfull, c.adia, c.aoff, 0, st);
+  const int rc = (c.backend == BACKEND_SLICES) ? slices_matvec(c, 'N', c.xfull, c.yfull, c.adia, c.aoff, 0, st)
+                                               : matvec_launch(c, 'N', c.xfull, c.yfull, c.adia, c.aoff, 0, st);
   c.iocc = keep;
   if (rc) return rc;
   hipLaunchKernelGGL(k_nearfield_finish, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, c.yfull,

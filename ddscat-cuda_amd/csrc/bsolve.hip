@@ -875,7 +875,7 @@ int bsolve_qmrccg(Context& c, BatchState& S, const float2* b, float2* x, const f
   if (o.upload(S.active, act)) return 51;
   const int* A = S.active;
   const int* Sm = S.sel;
-  LinComb z; z.lit(r, 0.0);
+  LinComb z;  // no terms: exact zeros, whatever the target held
   // |b|
   { DotSet ds; ds.add(b, b, DOT_N2); if (o.dots(ds, SL_D, A, FIN_BNORM)) return 52; }
   auto restart_from = [&](const float2* rs, const int* m) -> int {
@@ -1071,7 +1071,7 @@ int bsolve_pbcgs2(Context& c, BatchState& S, const float2* b, float2* x, const f
   if (o.upload(S.active, act)) return 92;
   const int* A = S.active;
   const int* Sm = S.sel;
-  LinComb z; z.lit(r0, 0.0);
+  LinComb z;  // no terms: exact zeros, whatever the target held
   // r0 = b - A xp for nonzero initial guesses, b otherwise (ZBCG2 NONZERO_X)
   { DotSet ds; ds.add(xp, xp, DOT_N2); if (o.dots(ds, SL_D, A, FIN_NONE)) return 92; }
   if (o.fetch_slots(SL_D, 1, slots)) return 92;
@@ -1242,6 +1242,13 @@ int bsolve_pbcgs2(Context& c, BatchState& S, const float2* b, float2* x, const f
 }  // namespace
 
 void batch_release(Context& c) { bs_free(bstate(c)); }
+
+int batch_poison(Context& c) {
+  BatchState& S = bstate(c);
+  if (S.W && S.cap_vec)
+    DDH_TRY(hipMemsetAsync(S.W, 0xFF, S.cap_vec * sizeof(float2), c.stream), "poison_workspace", 12);
+  return 0;
+}
 
 }  // namespace ddhip
 

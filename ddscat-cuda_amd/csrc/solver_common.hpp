@@ -89,6 +89,8 @@ int batch_matvec(Context& c, int cwhat, const float2* x, float2* y, const float2
 int finish_prepare(Context& c, const int16_t* iocc, int code_alloc, int code_occ);
 // frees the batched-solver buffers (bsolve.hip)
 void batch_release(Context& c);
+// fills the batched-solver vectors, when allocated, with NaN bit patterns (test hook)
+int batch_poison(Context& c);
 int ensure_stream(Context& c);
 
 // MATVEC on the compact solver vectors with the context's operators

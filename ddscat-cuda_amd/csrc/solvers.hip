@@ -153,7 +153,7 @@ int solve_pbcgs2(Context& c, const SolverArgs& a) {
     if (lincomb(c, r0, l, n, st)) return 92;
     LinComb cp; cp.lit(r0, 1.0);
     if (lincomb(c, rr, cp, n, st) || lincomb(c, bp, cp, n, st)) return 92;
-    LinComb z; z.lit(r0, 0.0);
+    LinComb z;  // no terms: exact zeros, whatever the target held
     for (float2* v : {r1, r2, u0, u1, u2, x})
       if (lincomb(c, v, z, n, st)) return 92;
   }
@@ -217,7 +217,7 @@ int solve_pbcgs2(Context& c, const SolverArgs& a) {
         LinComb lr; lr.lit(bp, 1.0).lit(c.sy, -1.0);
         LinComb lx; lx.lit(x, 1.0).lit(xp, 1.0);
         LinComb cp; cp.lit(r0, 1.0);
-        LinComb z; z.lit(r0, 0.0);
+        LinComb z;  // no terms: exact zeros, whatever the target held
         if (lincomb(c, r0, lr, n, st) || lincomb(c, xp, lx, n, st) || lincomb(c, x, z, n, st) ||
             lincomb(c, bp, cp, n, st) || lincomb(c, rr, cp, n, st))
           return 104;
@@ -234,7 +234,7 @@ int solve_pbcgs2(Context& c, const SolverArgs& a) {
       if (lincomb(c, r0, lr, n, st)) return 104;
       if (xpdt) {
         LinComb lx; lx.lit(x, 1.0).lit(xp, 1.0);
-        LinComb z; z.lit(r0, 0.0);
+        LinComb z;  // no terms: exact zeros, whatever the target held
         LinComb cp; cp.lit(r0, 1.0);
         if (lincomb(c, xp, lx, n, st) || lincomb(c, x, z, n, st) || lincomb(c, bp, cp, n, st)) return 104;
       }
@@ -272,7 +272,7 @@ int solve_qmrccg(Context& c, const SolverArgs& a) {
   if (bnorm < 0) return 52;
   if (bnorm == 0.0) { set_error(nm, "zero RHS norm"); return 52; }
   int nc = 0;
-  LinComb z; z.lit(r, 0.0);
+  LinComb z;  // no terms: exact zeros, whatever the target held
   auto restart_from = [&](const float2* rsrc) -> int {
     // r = rsrc; p = Ap = q = d = s = A w~ = 0; v~ = w~ = r
     LinComb cp; cp.lit(rsrc, 1.0);
@@ -508,7 +508,7 @@ int solve_gpbicg(Context& c, const SolverArgs& a) {
   if (bnorm < 0) return 32;
   if (bnorm == 0.0) { set_error(nm, "zero RHS norm"); return 32; }
   int nc = 0;
-  LinComb z0; z0.lit(r0, 0.0);
+  LinComb z0;  // no terms: exact zeros, whatever the target held
   auto restart_from = [&](const float2* src) -> int {
     LinComb cp; cp.lit(src, 1.0);
     if (lincomb(c, r0, cp, n, st) || lincomb(c, r, cp, n, st)) return 37;
@@ -649,7 +649,7 @@ int solve_pbcgst(Context& c, const SolverArgs& a) {
   int total_it = 0, nc = 0;
   double resid = 1.0;
   bool converged = false;
-  LinComb z; z.lit(r, 0.0);
+  LinComb z;  // no terms: exact zeros, whatever the target held
   auto true_residual = [&]() -> int {
     if (int rc = run_mv(c, 'N', xs, c.sy, nc)) return rc;
     LinComb l; l.lit(c.sb, 1.0).lit(c.sy, -1.0);

@@ -50,7 +50,8 @@ struct Ops {
   int affine(float2* d, const float2* a, zc z) { return lc(d, {{a, 1.0}, {d, z}}); }  // d = a + z d
   int scale(float2* d, const float2* a, zc z) { return lc(d, {{a, z}}); }
   int copy(float2* d, const float2* a) { return lc(d, {{a, 1.0}}); }
-  int zero(float2* d) { return lc(d, {{d, 0.0}}); }
+  // no terms: exact zeros (0 * d would keep a NaN or Inf of uninitialised memory)
+  int zero(float2* d) { return lincomb(c, d, LinComb{}, n, st); }
 };
 
 // Gaussian elimination with partial pivoting on a[n][ld] (row major), b[n]

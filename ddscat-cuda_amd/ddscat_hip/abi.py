@@ -34,6 +34,7 @@ EXTENSION_SYMBOLS = [
     "ddscat_hip_nearfield", "ddscat_hip_set_pass_kernel", "ddscat_hip_scat_matrix", "ddscat_hip_force_restart",
     "ddscat_hip_solve_log", "ddscat_hip_set_polarizability", "ddscat_hip_scat_f32_noalloc",
     "ddscat_hip_torque", "ddscat_hip_set_reliable_mode", "ddscat_hip_select_target", "ddscat_hip_solve_targets",
+    "ddscat_hip_poison_workspace",
 ]
 # method ids of ddscat_hip_solve_device
 SOLVER_IDS = {"pbcgs2": 0, "qmrccg": 1, "petrkp": 2, "gpbicg": 3, "pbcgst": 4, "bicgstab4": 5,
@@ -153,6 +154,8 @@ class Library:
         L.ddscat_hip_scat_matrix.restype = i
         L.ddscat_hip_force_restart.argtypes = []
         L.ddscat_hip_force_restart.restype = i
+        L.ddscat_hip_poison_workspace.argtypes = []
+        L.ddscat_hip_poison_workspace.restype = i
         L.ddscat_hip_solve_log.argtypes = [i, vp, vp, i, vp, i, vp]
         L.ddscat_hip_solve_log.restype = i
         L.ddscat_hip_set_polarizability.argtypes = [i]
@@ -398,6 +401,11 @@ class Library:
     def force_restart(self):
         """The next reliable-residual check of a single-RHS solve restarts (test hook)."""
         return self._check("force_restart", self.lib.ddscat_hip_force_restart())
+
+    def poison_workspace(self):
+        """Fill the solver workspaces (single-RHS, and batched when allocated)
+        with NaN bit patterns (test hook)."""
+        return self._check("poison_workspace", self.lib.ddscat_hip_poison_workspace())
 
     def solve_log(self, rhs=0) -> dict:
         """Convergence record of RHS `rhs` of the last solve: restarts, the

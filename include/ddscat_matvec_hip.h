@@ -302,8 +302,8 @@ DDSCAT_HIP_API int ddscat_hip_setup_rhs(int nrhs, const float *ak, const float *
  * out_full [3][NAT] (DEVICE) = einc_full + E_sca, or E_sca alone when
  * einc_full is NULL, where E_sca,j = sum_k G_jk P_k is radiated by the compact
  * polarization p_compact [3][NAT0] (DEVICE); computed with the MATVEC's FFT
- * convolution and no occupancy mask.  FFT3D backend; blocks until done.
- * Codes 10 not prepared, 11 argument / backend, 12-13 launch or copy. */
+ * convolution and no occupancy mask (either backend); blocks until done.
+ * Codes 10 not prepared, 11 argument, 12-13 launch or copy. */
 DDSCAT_HIP_API int ddscat_hip_nearfield(const void *p_compact, const void *einc_full, void *out_full);
 
 /* Amplitude and Mueller matrices for npair incident-polarization pairs (the
@@ -333,6 +333,13 @@ DDSCAT_HIP_API int ddscat_hip_scat_matrix(int npair, const void *p, int ndir, co
  * 1e-3 restart threshold) and restarts from the true residual.  The restart
  * count of the last single-RHS solve is ddscat_hip_info field 23. */
 DDSCAT_HIP_API int ddscat_hip_force_restart(void);
+
+/* Test hook for uninitialised memory: fills the selected slot's single-RHS
+ * solver workspace (sized for the largest solver) and, when a batched solve
+ * has allocated them, its batched solver vectors with NaN bit patterns, as
+ * memory that held other data can; every solver must write a work vector
+ * before it reads it.  Codes 10 not prepared, 12 launch, 29 memory. */
+DDSCAT_HIP_API int ddscat_hip_poison_workspace(void);
 
 /* Convergence record of the last solve: RHS `rhs` of the last
  * ddscat_hip_solve_batch / ddscat_hip_cabs_adjoint, or rhs 0 of the last

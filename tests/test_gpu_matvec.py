@@ -257,15 +257,28 @@ def test_general_register_shapes_match_oracle(lib, dims):
     assert err_full < MATVEC_TOL and err_c < MATVEC_TOL
 
 
-def test_nearfield_matches_oracle_on_all_sites(lib):
+@pytest.mark.parametrize("backend", ["fft3d", "slices"])
+def test_nearfield_matches_oracle_on_all_sites(lib, backend):
     """E_inc + E_sca on every lattice site, empty sites included, equals the
     oracle's convolution of P with the Green tensor (FFT MATVEC with the local
-    terms off and no occupancy mask, sign-flipped)."""
+    terms off and no occupancy mask, sign-flipped), on either backend."""
     import torch
+    from ddscat_hip import BACKEND_FFT3D, BACKEND_SLICES
     nx, ny, nz = 12, 10, 14
     rng = np.random.default_rng(33)
     iocc, x, _, _ = random_case(rng, nx, ny, nz, fill=0.5)
     akd = 0.31
+    lib.release()
+    lib.set_backend(BACKEND_SLICES if backend == "slices" else BACKEND_FFT3D)
+    try:
+        _nearfield_case(lib, nx, ny, nz, iocc, x, akd, rng)
+    finally:
+        lib.release()
+        lib.set_backend(BACKEND_FFT3D)
+
+
+def _nearfield_case(lib, nx, ny, nz, iocc, x, akd, rng):
+    import torch
     lib.prepare_green(nx, ny, nz, iocc, (akd, 0.0, 0.0))
     occ = np.flatnonzero(iocc)
     p_c = np.ascontiguousarray(x.reshape(3, -1)[:, occ].reshape(-1))

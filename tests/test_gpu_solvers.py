@@ -63,6 +63,26 @@ def test_solver_matches_dense_solution(lib, small, name):
     assert np.all(x.reshape(3, -1)[:, tg.iocc == 0] == 0)
 
 
+@pytest.mark.parametrize("name", ALL)
+def test_solver_ignores_uninitialised_workspace(lib, small, name):
+    """Work vectors that held other data (here NaN bit patterns in the whole
+    workspace) must not leak into a solve: every solver writes a vector,
+    zeros included, before it reads it.  Regression: clearing a vector as
+    0 * itself kept NaN and broke BiCGStab(4)'s MR polynomial."""
+    P = small
+    tg = P["tg"]
+    lib.prepare_green(tg.nx, tg.ny, tg.nz, tg.iocc, P["khat"] * P["kd"])
+    lib.poison_workspace()
+    tol = 1e-5
+    x, it, tole, nc, rc = lib.solve(name, P["b"], P["adia"], P["aoff"], maxit=300, tol=tol)
+    xo = x.astype(complex)[P["occ3"]]
+    bo = P["b"].astype(complex)[P["occ3"]]
+    true_res = np.linalg.norm(P["Ao"] @ xo - bo) / np.linalg.norm(bo)
+    print(f"{name} on a NaN-filled workspace: it={it} tole={tole:.2e} true={true_res:.2e}")
+    assert rc == 0 and np.all(np.isfinite(x))
+    assert true_res <= 2 * tol
+
+
 @pytest.mark.parametrize("name", ["pbcgs2", "qmrccg", "petrkp"])
 def test_iteration_count_band_vs_oracle(lib, small, name):
     """Same algorithm: device and FP64 oracle iteration counts agree loosely."""
