@@ -464,10 +464,13 @@ def large_target(lib, edge=384, kd=0.25, m=1.7 + 0.05j, steps=10, warmup=2, seed
     return out
 
 
-def large_sphere_solve(lib, edge=384, kd=0.25, m=1.7 + 0.05j, tol=1e-5, maxit=1500, qmr_budget=300):
+def large_sphere_solve(lib, edge=384, kd=0.25, m=1.7 + 0.05j, tol=1e-5, maxit=1500, qmr_budget=300,
+                       backend="fft3d", far_field=True):
     """The large single-target solve: sphere D=edge (384: 29.6 M dipoles,
     padded 768^3 x 3), m = 1.7+0.05i, kd = 0.25, incidence +x with E || y and
-    E || z, inputs built on the device (ddscat_hip_setup_rhs), FFT3D.
+    E || z, inputs built on the device (ddscat_hip_setup_rhs), on the FFT3D or
+    SLICES backend (device_gb: device memory in use at the end of the solve,
+    the library's and the leg's own arrays).
       pbcgs2: batched mixed PBCGS2 to tol, then the batched EVALQ: time to
               Q_ext; both polarizations' Q_abs must agree (the lattice's
               cubic symmetry);
@@ -477,13 +480,17 @@ def large_sphere_solve(lib, edge=384, kd=0.25, m=1.7 + 0.05j, tol=1e-5, maxit=15
               qmr_budget iterations: the residual it reaches."""
     import numpy as np
     import torch
-    from ddscat_hip import targets as T, scattering as DS, PREC_MIXED
+    from ddscat_hip import targets as T, scattering as DS, PREC_MIXED, BACKEND_FFT3D, BACKEND_SLICES
     tg = T.sphere(edge)
     khat = np.array([1.0, 0.0, 0.0])
     e0s = np.array([[0, 1, 0], [0, 0, 1]], np.complex64)
     aks = np.tile((khat * kd).astype(np.float32), (2, 1))
     x0 = tg.centroid_x0()
     lib.set_precision(PREC_MIXED)
+    lib.release()
+    lib.set_backend(BACKEND_SLICES if backend == "slices" else BACKEND_FFT3D)
+    torch.cuda.synchronize()
+    free0 = torch.cuda.mem_get_info()[0]
     t0 = time.perf_counter()
     lib.prepare_green(tg.nx, tg.ny, tg.nz, tg.iocc, khat * kd)
     prep = time.perf_counter() - t0
@@ -494,7 +501,8 @@ def large_sphere_solve(lib, edge=384, kd=0.25, m=1.7 + 0.05j, tol=1e-5, maxit=15
     lib.setup_rhs(aks, e0s, x0, eps.data_ptr(), b.data_ptr(), ad.data_ptr(), ao.data_ptr())
     geo = math.pi * tg.a_eff() ** 2
     out = dict(workload=f"sphere D={edge} ({tg.nat0} dipoles, padded {2 * edge}^3 x 3), m=1.7+0.05i, kd={kd} "
-                        f"(x={kd * tg.a_eff():.1f}), +x incidence, E||y and E||z, FFT3D", prepare_seconds=round(prep, 3))
+                        f"(x={kd * tg.a_eff():.1f}), +x incidence, E||y and E||z, {backend.upper()}",
+               prepare_seconds=round(prep, 3))
     x = torch.zeros_like(b)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -503,9 +511,17 @@ def large_sphere_solve(lib, edge=384, kd=0.25, m=1.7 + 0.05j, tol=1e-5, maxit=15
     q = lib.evalq_batch(2, b.data_ptr(), x.data_ptr(), ad.data_ptr(), ao.data_ptr(), aks, check=False)
     dt = time.perf_counter() - t0
     qabs, qext = q[:, 0] / geo, q[:, 1] / geo
+    torch.cuda.synchronize()
+    out["device_gb"] = round((free0 - torch.cuda.mem_get_info()[0]) / 1e9, 2)
     out["pbcgs2"] = dict(seconds_to_qext=round(dt, 3), iterations=it.tolist(), resid=te.tolist(), rc=int(rc),
                          converged=bool(rc == 0 and np.all(te <= 1.01 * tol)), qext=qext.tolist(), qabs=qabs.tolist(),
                          qabs_rel_diff_between_polarizations=float(abs(qabs[0] - qabs[1]) / abs(qabs[0])))
+    if not far_field:
+        del b, ad, ao, x, eps
+        torch.cuda.empty_cache()
+        lib.release()
+        lib.set_backend(BACKEND_FFT3D)
+        return out
     theta = np.radians(np.arange(181.0))
     phis = np.radians([0.0, 90.0, 180.0, 270.0])
     th, ph = np.repeat(theta, 4), np.tile(phis, 181)
@@ -526,6 +542,7 @@ def large_sphere_solve(lib, edge=384, kd=0.25, m=1.7 + 0.05j, tol=1e-5, maxit=15
     del b, ad, ao, x, eps
     torch.cuda.empty_cache()
     lib.release()
+    lib.set_backend(BACKEND_FFT3D)
     return out
 
 
@@ -697,6 +714,8 @@ def main():
                     help="time to Q_ext at 2 and 16 right-hand sides on the 128^3 and 256^3 blocks, then exit")
     ap.add_argument("--large-only", action="store_true",
                     help="run only the 384^3 sphere MATVEC leg (FFT3D and SLICES) and print its JSON")
+    ap.add_argument("--large-solve-study", action="store_true",
+                    help="the 384^3 sphere's batched PBCGS2 to 1e-5 on FFT3D and on SLICES (time, memory), then exit")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build measures one GPU only (single-GPU box); --gpus must be 1", file=sys.stderr)
@@ -709,6 +728,13 @@ def main():
     lib = Library()
     if args.large_only:
         print(json.dumps(dict(large_target=large_target(lib, steps=args.steps, warmup=args.warmup))), flush=True)
+        return
+    if args.large_solve_study:
+        res = {}
+        for be in ("fft3d", "slices"):
+            res[be] = large_sphere_solve(lib, qmr_budget=0, backend=be, far_field=False)
+            print(json.dumps({be: res[be]}), flush=True)
+        print(json.dumps(dict(large_solve_study=res)), flush=True)
         return
     if args.batch_study:
         from ddscat_hip import targets as T
