@@ -1165,7 +1165,8 @@ __device__ __forceinline__ void p5_load(float2* v, const float2* __restrict__ sr
       v[i * R1 + k1] = reg2_owns<R1, R2>(t, i) ? src[(t + R1 * i) + R2 * k1] : make_float2(0.f, 0.f);
 }
 
-template <int R1, int R2, int C>
+// FG: outputs per finish load group
+template <int R1, int R2, int C, int FG = 4>
 __global__ __launch_bounds__(R1 * C) void kr_pass5(const float2* __restrict__ a1, const float2* __restrict__ twg,
                                                const float2* __restrict__ x, float2* __restrict__ y,
                                                const float2* __restrict__ adia, const float2* __restrict__ aoff,
@@ -1197,8 +1198,15 @@ __global__ __launch_bounds__(R1 * C) void kr_pass5(const float2* __restrict__ a1
   fft2_inv<R1, R2, -1, 1>(v1, buf, twL, t, s, P);
   __syncthreads();
   fft2_inv<R1, R2, -1, 1>(v2, buf, twL, t, s, P);
+  // the transforms' kept outputs are materialised here: otherwise their last
+  // butterflies sink into the finish, interleave with its loads and hold
+  // both live at once (248 instead of 140 VGPRs at 16 x 16, 2 waves per SIMD)
+#pragma unroll
+  for (int n2 = 0; n2 < (R2 + 1) / 2; ++n2)
+    asm volatile("" : "+v"(v0[n2].x), "+v"(v0[n2].y), "+v"(v1[n2].x), "+v"(v1[n2].y), "+v"(v2[n2].x),
+                 "+v"(v2[n2].y));
   if (!live) return;
-  p5_finish<R1, R2, 4>(v0, v1, v2, t, g, x, y, adia, aoff, iocc, sm, nx, nat, nat0, compact, cj, inv);
+  p5_finish<R1, R2, FG>(v0, v1, v2, t, g, x, y, adia, aoff, iocc, sm, nx, nat, nat0, compact, cj, inv);
 }
 
 // ------------------------------------------------ pass 5, L = M * 256
@@ -1385,6 +1393,19 @@ static void launch_ypair(Context& c, int cj, int compact, const float2* x, float
   const MatvecPlan& m = c.mv;
   hipLaunchKernelGGL((kr_pass3y<R1, R2, C>), dim3((m.gx + C - 1) / C, c.ny, (unsigned)bt.nrhs), dim3(2 * R1 * C), 0,
                      st, bt.a2, c.green, m.twz, c.nx, c.ny, c.nz, m.gx, m.gy, m.p2, bt.k, green_geo(c));
+}
+
+// pass 5 by kr_pass5 with finish load groups of FG outputs; serves pass 5 only
+template <int R1, int R2, int C, int FG>
+static void launch_p5f(Context& c, int cj, int compact, const float2* x, float2* y, const float2* adia,
+                       const float2* aoff, hipStream_t st, const MvBatch& bt, int which) {
+  if (which != 5) return;
+  const MatvecPlan& m = c.mv;
+  const int nrows = c.ny * c.nz;
+  SiteMap sm{c.rowptr, c.rowlo, c.qmap, c.rows_contiguous};
+  hipLaunchKernelGGL((kr_pass5<R1, R2, C, FG>), dim3((nrows + C - 1) / C, (unsigned)bt.nrhs), dim3(R1 * C), 0,
+                     st, bt.a1, m.twx, x, y, adia, aoff, compact ? nullptr : c.iocc, sm, c.nx, nrows, c.nat, c.nat0,
+                     compact, cj, 1.f / (float)m.ngrid, bt.k);
 }
 
 // single-component passes only (1, 2, 4): shapes whose per-thread arrays are
@@ -1575,6 +1596,10 @@ static const RegShape kPairShapes[] = {
     // before the third forward transform (three or four points: 170-192
     // VGPRs and 2 waves per SIMD, or spills at 3)
     RegShape{256, 16, 2, 16, &launch_p3pre<16, 16, 16, 2>, 1 << 3},
+    // pass 5 with finish load groups of 2 and 3 outputs (123 and 125 VGPRs,
+    // 4 waves per SIMD)
+    RegShape{256, 16, 16, 16, &launch_p5f<16, 16, 16, 2>, 1 << 5},
+    RegShape{192, 16, 12, 16, &launch_p5f<16, 12, 16, 3>, 1 << 5},
     // Stockham Green passes with the first two points' Green loads issued
     // before the last component's forward transform
     RegShape{512, 8, 8, 8, &launch_p3s<SkPlan<8, 8, 8, 8>, 8, 2, 1, 4, 4, true, 2>, 1 << 3},
