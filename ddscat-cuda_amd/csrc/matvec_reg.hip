@@ -1596,6 +1596,11 @@ static const RegShape kPairShapes[] = {
     // before the third forward transform (three or four points: 170-192
     // VGPRs and 2 waves per SIMD, or spills at 3)
     RegShape{256, 16, 2, 16, &launch_p3pre<16, 16, 16, 2>, 1 << 3},
+    // and at 144-192 points (180 and 160 points: 6-7 % faster)
+    RegShape{192, 16, 2, 16, &launch_p3pre<16, 12, 16, 2>, 1 << 3},
+    RegShape{180, 15, 2, 16, &launch_p3pre<15, 12, 16, 2>, 1 << 3},
+    RegShape{160, 16, 2, 16, &launch_p3pre<16, 10, 16, 2>, 1 << 3},
+    RegShape{144, 12, 2, 16, &launch_p3pre<12, 12, 16, 2>, 1 << 3},
     // pass 5 with finish load groups of 2 and 3 outputs (123 and 125 VGPRs,
     // 4 waves per SIMD)
     RegShape{256, 16, 16, 16, &launch_p5f<16, 16, 16, 2>, 1 << 5},
