@@ -107,9 +107,10 @@ def test_direct_sum_matches_oracle_fft_matvec(d):
 
 
 @pytest.mark.gpu
-def test_block256_matvec_matches_direct_sums(lib):
+@pytest.mark.parametrize("backend", ["fft3d", "slices"])
+def test_block256_matvec_matches_direct_sums(lib, backend):
     tg = T.block(256)
-    check(lib, tg, 0.1259, "fft3d", 256, "block 256^3")
+    check(lib, tg, 0.1259, backend, 256, "block 256^3")
 
 
 @pytest.mark.gpu
