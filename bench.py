@@ -196,13 +196,15 @@ def batched_matvec_scaling(lib, tg, bufs, sizes=(1, 2, 4, 8, 16), reps=10):
     return out
 
 
-def batched_small_target(lib, n_orient=16, method="pbcgs2", tol=1e-5, maxit=300, seed=20261202):
+def batched_small_target(lib, n_orient=16, method="pbcgs2", tol=1e-5, maxit=300, seed=20261202, reps=5):
     """Sphere of diameter 32 dipoles (17,256 dipoles), m = 1.33+0.01i, x = 2,
     n_orient incidence directions x 2 polarizations, PBCGS2 at tol 1e-5:
     the small-target workload where single-RHS solves are launch-bound.
     plain_fp32: one single-RHS device solve per RHS (PREC_FP32);
     batched_mixed: all RHS in one batched solve (PREC_MIXED, graph replay).
-    Each is timed from the first solve call to all Q_ext on the host."""
+    Each is timed from the first solve call to all Q_ext on the host, reps
+    times from a zero initial guess (a few ms each); the medians are
+    reported, with every repetition."""
     import numpy as np
     import torch
     from ddscat_hip import PREC_MIXED, PREC_FP32
@@ -235,35 +237,40 @@ def batched_small_target(lib, n_orient=16, method="pbcgs2", tol=1e-5, maxit=300,
                      row(aod, 0).data_ptr(), maxit=2, tol=tol, check=False)
     lib.solve_batch(method, B, bd.data_ptr(), xd.data_ptr(), add.data_ptr(), aod.data_ptr(), maxit=2, tol=tol,
                     check=False)
-    lib.set_precision(PREC_FP32)
-    xd.zero_()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    q_single, it_single = [], []
-    for r in range(B):
-        it, tole, nc, rc = lib.solve_device(method, row(bd, r).data_ptr(), row(xd, r).data_ptr(),
-                                            row(add, r).data_ptr(), row(aod, r).data_ptr(), maxit=maxit, tol=tol,
-                                            check=False)
-        cabs, cext, _, _ = lib.evalq(aks[r], check=False)
-        q_single.append(cext / geo)
-        it_single.append(it)
-    t_single = time.perf_counter() - t0
-    lib.set_precision(PREC_MIXED)
-    xd.zero_()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    it, te, nc, rc = lib.solve_batch(method, B, bd.data_ptr(), xd.data_ptr(), add.data_ptr(), aod.data_ptr(),
-                                     maxit=maxit, tol=tol, check=False)
-    q = lib.evalq_batch(B, bd.data_ptr(), xd.data_ptr(), add.data_ptr(), aod.data_ptr(), aks, check=False)
-    t_batch = time.perf_counter() - t0
+    ts_single, ts_batch = [], []
+    for _ in range(reps):
+        lib.set_precision(PREC_FP32)
+        xd.zero_()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        q_single, it_single = [], []
+        for r in range(B):
+            it, tole, nc, rc = lib.solve_device(method, row(bd, r).data_ptr(), row(xd, r).data_ptr(),
+                                                row(add, r).data_ptr(), row(aod, r).data_ptr(), maxit=maxit,
+                                                tol=tol, check=False)
+            cabs, cext, _, _ = lib.evalq(aks[r], check=False)
+            q_single.append(cext / geo)
+            it_single.append(it)
+        ts_single.append(time.perf_counter() - t0)
+        lib.set_precision(PREC_MIXED)
+        xd.zero_()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        it, te, nc, rc = lib.solve_batch(method, B, bd.data_ptr(), xd.data_ptr(), add.data_ptr(), aod.data_ptr(),
+                                         maxit=maxit, tol=tol, check=False)
+        q = lib.evalq_batch(B, bd.data_ptr(), xd.data_ptr(), add.data_ptr(), aod.data_ptr(), aks, check=False)
+        ts_batch.append(time.perf_counter() - t0)
+    t_single, t_batch = float(np.median(ts_single)), float(np.median(ts_batch))
     qb = q[:, 1] / geo
     qs = np.asarray(q_single)
     return dict(workload=f"sphere D=32 ({tg.nat0} dipoles), m=1.33+0.01i, x=2, {n_orient} orientations x 2 "
                          f"polarizations = {B} RHS, {method.upper()} tol {tol}",
                 plain_fp32=dict(seconds=round(t_single, 4), rhs_per_s=round(B / t_single, 2),
-                                mean_iterations=float(np.mean(it_single))),
+                                mean_iterations=float(np.mean(it_single)),
+                                seconds_each=[round(v, 4) for v in ts_single]),
                 batched_mixed=dict(seconds=round(t_batch, 4), rhs_per_s=round(B / t_batch, 2),
-                                   mean_iterations=float(np.mean(it)), rc=int(rc)),
+                                   mean_iterations=float(np.mean(it)), rc=int(rc),
+                                   seconds_each=[round(v, 4) for v in ts_batch]),
                 speedup_batched_mixed_vs_plain_fp32=round(t_single / t_batch, 3),
                 qext_max_rel_diff=float(np.max(np.abs(qb - qs) / np.abs(qs))),
                 qext_mean=float(np.mean(qb)))

@@ -817,6 +817,12 @@ struct Graph {
   }
 };
 
+// Iterations run eagerly before the iteration graph is captured: a solve
+// that ends within them (a few iterations on a small target) does not pay the
+// capture and instantiation (about 0.3 ms); graph replay and eager launches
+// give the same results bit for bit.
+constexpr int GRAPH_FROM_ITER = 4;
+
 // runs `body` once, either directly or through a captured graph replay
 template <typename F>
 int run_step(Ops& o, Graph& gr, bool use_graph, F&& body) {
@@ -975,7 +981,7 @@ int bsolve_qmrccg(Context& c, BatchState& S, const float2* b, float2* x, const f
     kchunk = next_chunk(itc, maxit, kchunk, 1e9);
     if (o.chunk_reset()) return 59;
     for (int i = 0; i < kchunk; ++i)
-      if (int rc = run_step(o, gr, use_graph, body)) return rc;
+      if (int rc = run_step(o, gr, use_graph && itc >= GRAPH_FROM_ITER, body)) return rc;
     if (o.chunk_fetch()) return 59;
     std::fill(sel.begin(), sel.end(), 0);
     double pred = 1e9;
@@ -1147,7 +1153,7 @@ int bsolve_pbcgs2(Context& c, BatchState& S, const float2* b, float2* x, const f
     kchunk = next_chunk(itc, maxit, kchunk, 1e9);
     if (o.chunk_reset()) return 103;
     for (int i = 0; i < kchunk; ++i)
-      if (int rc = run_step(o, gr, use_graph, body)) return rc;
+      if (int rc = run_step(o, gr, use_graph && itc >= GRAPH_FROM_ITER, body)) return rc;
     if (o.chunk_fetch()) return 103;
     std::fill(sel.begin(), sel.end(), 0);
     double pred = 1e9;

@@ -181,7 +181,8 @@ DDSCAT_HIP_API int ddscat_hip_apply_batch(int cwhat, int nrhs, const void *x, vo
  * itno, tole, ncompte: host arrays [nrhs] (per-RHS iterations, achieved
  * residual, MATVEC count).  Every RHS follows the single-RHS algorithm and
  * reliable-residual policy; finished RHS stop while the rest continue.
- * use_graph = 1 replays one captured HIP graph per iteration; iterations run
+ * use_graph = 1 replays one captured HIP graph per iteration from the fifth
+ * iteration on (the first four run as direct launches); iterations run
  * in chunks of up to 20 between host synchronisations, with the per-RHS
  * iteration counts and stopping tests kept on the device.  Returns 0, or
  * the breakdown code of the first RHS that broke down (the others are
