@@ -108,8 +108,24 @@ __global__ __launch_bounds__(R1 * C) void kr_pass1(const float2* __restrict__ x,
 }
 
 // ------------------------------------------------------------ passes 2, 4
-// in [3][nz][nin][gx] -> out [3][nz][nout][gx] along the middle axis
-template <int R1, int R2, int C, int DIR>
+// element j of component c of a MATVEC input as BatchArgs' gather fields
+// name it (the SLICES z-forward pass reads x itself, compact or conjugated)
+__device__ __forceinline__ float2 gathered(const BatchArgs& ba, int c, size_t j) {
+  float2 v = make_float2(0.f, 0.f);
+  if (ba.gmap) {
+    const int q = ba.gmap[j];
+    if (q >= 0) v = ba.gsrc[(size_t)c * ba.gcs + q];
+  } else {
+    v = ba.gsrc[(size_t)c * ba.gcs + j];
+  }
+  if (ba.gconj) v.y = -v.y;
+  return v;
+}
+
+// in [3][nz][nin][gx] -> out [3][nz][nout][gx] along the middle axis.
+// GA (DIR = +1, nz = 1: the SLICES z pass): input element n of column kx is
+// gathered(ba, c, n gx + kx) instead of in[c ins + n gx + kx].
+template <int R1, int R2, int C, int DIR, bool GA = false>
 __global__ __launch_bounds__(R1 * C) void kr_passy(const float2* __restrict__ in, float2* __restrict__ out,
                                                const float2* __restrict__ twg, int gx, int nin, int nout, int nz,
                                                size_t ins, size_t outs, BatchArgs ba) {
@@ -131,7 +147,10 @@ __global__ __launch_bounds__(R1 * C) void kr_passy(const float2* __restrict__ in
 #pragma unroll
     for (int n2 = 0; n2 < R2; ++n2) {
       const int n = t + R1 * n2;
-      v[n2] = (live && lower_half<R2>(n2, n, nin)) ? src[(size_t)n * gx] : make_float2(0.f, 0.f);
+      if constexpr (GA)
+        v[n2] = (live && lower_half<R2>(n2, n, nin)) ? gathered(ba, c, (size_t)n * gx + kx) : make_float2(0.f, 0.f);
+      else
+        v[n2] = (live && lower_half<R2>(n2, n, nin)) ? src[(size_t)n * gx] : make_float2(0.f, 0.f);
     }
     __syncthreads();
     fft2_fwd<R1, R2, +1>(v, buf, twL, t, s, P);
@@ -168,7 +187,8 @@ __global__ __launch_bounds__(R1 * C) void kr_passy(const float2* __restrict__ in
 // 768-point column tile of C = 8 needs about 36 KB of LDS instead of the
 // 61 KB of the two-level 32 x 24 shape.  DIR = +1: nin (<= L/2, pruned)
 // inputs -> L spectral outputs; DIR = -1: L spectral inputs -> nout (<= L/2).
-template <int M, int C, int DIR>
+// GA as in kr_passy.
+template <int M, int C, int DIR, bool GA = false>
 __global__ __launch_bounds__(M * 16 * C) void kr_passym(const float2* __restrict__ in, float2* __restrict__ out,
                                                     const float2* __restrict__ twg, int gx, int nin, int nout, int nz,
                                                     size_t ins, size_t outs, BatchArgs ba) {
@@ -228,7 +248,10 @@ __global__ __launch_bounds__(M * 16 * C) void kr_passym(const float2* __restrict
 #pragma unroll
     for (int n2 = 0; n2 < R; ++n2) {
       const int n = M * (t + R * n2) + m;
-      v[n2] = (live && n2 < R / 2 && n < nin) ? src[(size_t)n * gx] : make_float2(0.f, 0.f);
+      if constexpr (GA)
+        v[n2] = (live && n2 < R / 2 && n < nin) ? gathered(ba, c, (size_t)n * gx + kx) : make_float2(0.f, 0.f);
+      else
+        v[n2] = (live && n2 < R / 2 && n < nin) ? src[(size_t)n * gx] : make_float2(0.f, 0.f);
     }
     __syncthreads();
     fft2_fwd_s<R, R, +1>(v, xf, twS, t, sv, PS);  // v[i] = Y_m[t + 16 i]
@@ -1721,33 +1744,41 @@ static BatchArgs one_rhs(const int* act) {
   ba.active = act;
   return ba;
 }
+// ba: one_rhs(act), with the gather fields set when the forward pass reads
+// the MATVEC's x itself (compact or conjugated)
 template <int R1, int R2, int C>
-static void launch_slz(Context& c, int dir, const float2* in, float2* out, hipStream_t st, const int* act) {
+static void launch_slz(Context& c, int dir, const float2* in, float2* out, hipStream_t st, const BatchArgs& ba) {
   const MatvecPlan& m = c.mv;
   const int plane = c.nx * c.ny;
   const size_t full = (size_t)m.gz * plane, half = (size_t)c.nz * plane;
   const dim3 grid((plane + C - 1) / C, 1, 3);
-  if (dir > 0)
+  if (dir > 0 && ba.gsrc)
+    hipLaunchKernelGGL((kr_passy<R1, R2, C, +1, true>), grid, dim3(R1 * C), 0, st, in, out, m.twz, plane, c.nz, m.gz,
+                       1, half, full, ba);
+  else if (dir > 0)
     hipLaunchKernelGGL((kr_passy<R1, R2, C, +1>), grid, dim3(R1 * C), 0, st, in, out, m.twz, plane, c.nz, m.gz, 1, half,
-                       full, one_rhs(act));
+                       full, ba);
   else
     hipLaunchKernelGGL((kr_passy<R1, R2, C, -1>), grid, dim3(R1 * C), 0, st, in, out, m.twz, plane, m.gz, c.nz, 1, full,
-                       half, one_rhs(act));
+                       half, ba);
 }
 template <int M, int C>
-static void launch_slz_m(Context& c, int dir, const float2* in, float2* out, hipStream_t st, const int* act) {
+static void launch_slz_m(Context& c, int dir, const float2* in, float2* out, hipStream_t st, const BatchArgs& ba) {
   const MatvecPlan& m = c.mv;
   const int plane = c.nx * c.ny;
   const size_t full = (size_t)m.gz * plane, half = (size_t)c.nz * plane;
   const dim3 grid((plane + C - 1) / C, 1, 3), block(M * 16 * C);
-  if (dir > 0)
+  if (dir > 0 && ba.gsrc)
+    hipLaunchKernelGGL((kr_passym<M, C, +1, true>), grid, block, 0, st, in, out, m.twz, plane, c.nz, m.gz, 1, half,
+                       full, ba);
+  else if (dir > 0)
     hipLaunchKernelGGL((kr_passym<M, C, +1>), grid, block, 0, st, in, out, m.twz, plane, c.nz, m.gz, 1, half, full,
-                       one_rhs(act));
+                       ba);
   else
     hipLaunchKernelGGL((kr_passym<M, C, -1>), grid, block, 0, st, in, out, m.twz, plane, m.gz, c.nz, 1, full, half,
-                       one_rhs(act));
+                       ba);
 }
-using SlzLaunch = void (*)(Context&, int, const float2*, float2*, hipStream_t, const int*);
+using SlzLaunch = void (*)(Context&, int, const float2*, float2*, hipStream_t, const BatchArgs&);
 struct SlzShape {
   int L;
   SlzLaunch fn;
@@ -1771,11 +1802,19 @@ int slices_z_reg_length(int L) {
   return 0;
 }
 
-int slices_z_reg(Context& c, int dir, const float2* in, float2* out, hipStream_t st, const int* act) {
+int slices_z_reg(Context& c, int dir, const float2* in, float2* out, hipStream_t st, const int* act, bool gather,
+                 int compact, int cj) {
   if (c.fft_generic) return 0;
+  BatchArgs ba = one_rhs(act);
+  if (gather && dir > 0) {
+    ba.gsrc = in;
+    ba.gmap = compact ? c.qmap : nullptr;
+    ba.gcs = compact ? (size_t)c.nat0 : (size_t)c.nat;
+    ba.gconj = cj;
+  }
   for (const SlzShape& r : kSlzShapes)
     if (r.L == c.mv.gz) {
-      r.fn(c, dir, in, out, st, act);
+      r.fn(c, dir, in, out, st, ba);
       return 1;
     }
   return 0;

@@ -23,7 +23,7 @@ static constexpr int SL_PERW = 16;  // single-component kernels
 
 struct SliceState {
   float2* zwork = nullptr;
-  float2* zin = nullptr;  // [3][NZ][NY][NX] (full layout): z-pass input / inverse output (register z path)
+  float2* zin = nullptr;  // [3][NZ][NY][NX] (full layout): inverse z-FFT output (register z path)
   float2* slice[2] = {nullptr, nullptr};
   hipStream_t side[2] = {nullptr, nullptr};
   hipEvent_t ev_in = nullptr, ev_done[2] = {nullptr, nullptr};
@@ -272,27 +272,7 @@ __global__ __launch_bounds__(SL_THREADS) void k_sl_zinv_finish(
   }
 }
 
-// ------------------------------------ register z path: gather and finish
-// x (compact via the site map, or full) into zin [3][NAT], conj for 'C'
-__global__ __launch_bounds__(SL_THREADS) void k_sl_gather(const float2* __restrict__ x, const int* __restrict__ qmap,
-                                                         float2* __restrict__ zin, int nat, int nat0, int compact,
-                                                         int cj, const int* __restrict__ act) {
-  if (act && !*act) return;  // inactive right-hand side of a batch
-  const int c = blockIdx.y;
-  const size_t stride = (size_t)gridDim.x * SL_THREADS;
-  for (size_t j = (size_t)blockIdx.x * SL_THREADS + threadIdx.x; j < (size_t)nat; j += stride) {
-    float2 v;
-    if (compact) {
-      const int q = qmap[j];
-      v = q >= 0 ? x[(size_t)c * nat0 + q] : make_float2(0.f, 0.f);
-    } else {
-      v = x[(size_t)c * nat + j];
-    }
-    if (cj) v.y = -v.y;
-    zin[(size_t)c * nat + j] = v;
-  }
-}
-
+// ------------------------------------------- register z path: finish
 // finish on the inverse-transformed w [3][NAT] (cu:116-118): y = adia x - w/G
 // + offdiag(aoff) x at every occupied site, IOCC mask, conj for 'C'
 __global__ __launch_bounds__(SL_THREADS) void k_sl_finish(const float2* __restrict__ w, const float2* __restrict__ x,
@@ -428,13 +408,8 @@ int slices_matvec(Context& c, int cwhat, const float2* x, float2* y, const float
   const bool regz = s.zin && !c.fft_generic;
   const int ewg = (int)std::min<size_t>(8192, ((size_t)c.nat + SL_THREADS * 4 - 1) / (SL_THREADS * 4));
   if (regz) {
-    const float2* zsrc = x;
-    if (compact || cj) {
-      hipLaunchKernelGGL(k_sl_gather, dim3(ewg, 3), dim3(SL_THREADS), 0, st, x, c.qmap, s.zin, c.nat, c.nat0,
-                         compact, cj, act);
-      zsrc = s.zin;
-    }
-    slices_z_reg(c, +1, zsrc, s.zwork, st, act);
+    // a compact or conjugated x is gathered inside the z-FFT's loads
+    slices_z_reg(c, +1, x, s.zwork, st, act, compact || cj, compact, cj);
   } else {
     dim3 grid((c.nx + s.cz - 1) / s.cz, c.ny, 3);
     const size_t lds = (size_t)(m.gz + s.cz * m.gz) * sizeof(float2);

@@ -117,6 +117,13 @@ void release_solver_workspace(Context& c);
 struct BatchArgs {
   size_t xs = 0, os = 0, a1s = 0, a2s = 0;
   const int* active = nullptr;
+  // SLICES z-forward input read straight from the MATVEC's x: element j of
+  // component c is gsrc[c gcs + gmap[j]] (compact; 0 where gmap[j] < 0) or
+  // gsrc[c gcs + j] (gmap == nullptr), conjugated when gconj
+  const float2* gsrc = nullptr;
+  const int* gmap = nullptr;
+  size_t gcs = 0;
+  int gconj = 0;
 };
 struct MvBatch {
   int nrhs = 1;
@@ -152,7 +159,10 @@ int slices_x_reg(Context& c, int dir, float2* zw, float2* sl, int k0, int nb, hi
 // SLICES z-FFT (dir +1: [3][NZ][NY][NX] -> zwork) or inverse z-FFT (dir -1:
 // zwork -> [3][NZ][NY][NX]) with the register column kernels when 2NZ has a
 // register shape; 1 = launched.  slices_z_reg_length: whether L has one.
-int slices_z_reg(Context& c, int dir, const float2* in, float2* out, hipStream_t st, const int* act);
+// gather (dir +1 only): `in` is the MATVEC's x, compact through the site map
+// when `compact`, conjugated when `cj`, read inside the z-FFT's loads.
+int slices_z_reg(Context& c, int dir, const float2* in, float2* out, hipStream_t st, const int* act,
+                 bool gather = false, int compact = 0, int cj = 0);
 int slices_z_reg_length(int L);
 // times every FFT kernel candidate of each pass and stores the fastest in c.mv.kind
 int matvec_autotune(Context& c);
