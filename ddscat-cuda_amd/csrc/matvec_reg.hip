@@ -109,15 +109,22 @@ __global__ __launch_bounds__(R1 * C) void kr_pass1(const float2* __restrict__ x,
 
 // ------------------------------------------------------------ passes 2, 4
 // element j of component c of a MATVEC input as BatchArgs' gather fields
-// name it (the SLICES z-forward pass reads x itself, compact or conjugated)
-__device__ __forceinline__ float2 gathered(const BatchArgs& ba, int c, size_t j) {
-  float2 v = make_float2(0.f, 0.f);
-  if (ba.gmap) {
+// name it, or 0 when !take (the SLICES z-forward pass reads x itself, compact
+// or conjugated).  MAP: compact input through the site map.  Branch-free: a
+// lane that takes nothing loads site 0 and discards it, so a thread's
+// site-map loads, then its x loads, issue back to back.
+template <bool MAP>
+__device__ __forceinline__ float2 gathered(const BatchArgs& ba, int c, size_t j, bool take) {
+  j = take ? j : 0;
+  float2 v;
+  if constexpr (MAP) {
     const int q = ba.gmap[j];
-    if (q >= 0) v = ba.gsrc[(size_t)c * ba.gcs + q];
+    take = take && q >= 0;
+    v = ba.gsrc[(size_t)c * ba.gcs + (q >= 0 ? q : 0)];
   } else {
     v = ba.gsrc[(size_t)c * ba.gcs + j];
   }
+  if (!take) v = make_float2(0.f, 0.f);
   if (ba.gconj) v.y = -v.y;
   return v;
 }
@@ -144,14 +151,21 @@ __global__ __launch_bounds__(R1 * C) void kr_passy(const float2* __restrict__ in
   float2* dst = out + ((size_t)c * nz + iz) * outs + kx;
   float2 v[VN];
   if (DIR > 0) {
+    auto load = [&](auto mapc) {
 #pragma unroll
-    for (int n2 = 0; n2 < R2; ++n2) {
-      const int n = t + R1 * n2;
-      if constexpr (GA)
-        v[n2] = (live && lower_half<R2>(n2, n, nin)) ? gathered(ba, c, (size_t)n * gx + kx) : make_float2(0.f, 0.f);
-      else
-        v[n2] = (live && lower_half<R2>(n2, n, nin)) ? src[(size_t)n * gx] : make_float2(0.f, 0.f);
-    }
+      for (int n2 = 0; n2 < R2; ++n2) {
+        const int n = t + R1 * n2;
+        const bool take = live && lower_half<R2>(n2, n, nin);
+        if constexpr (GA)
+          v[n2] = gathered<decltype(mapc)::value>(ba, c, (size_t)n * gx + kx, take);
+        else
+          v[n2] = take ? src[(size_t)n * gx] : make_float2(0.f, 0.f);
+      }
+    };
+    if (GA && ba.gmap)
+      load(std::true_type{});
+    else
+      load(std::false_type{});
     __syncthreads();
     fft2_fwd<R1, R2, +1>(v, buf, twL, t, s, P);
     if (live) {
@@ -245,14 +259,21 @@ __global__ __launch_bounds__(M * 16 * C) void kr_passym(const float2* __restrict
     }
   };
   if (DIR > 0) {
+    auto load = [&](auto mapc) {
 #pragma unroll
-    for (int n2 = 0; n2 < R; ++n2) {
-      const int n = M * (t + R * n2) + m;
-      if constexpr (GA)
-        v[n2] = (live && n2 < R / 2 && n < nin) ? gathered(ba, c, (size_t)n * gx + kx) : make_float2(0.f, 0.f);
-      else
-        v[n2] = (live && n2 < R / 2 && n < nin) ? src[(size_t)n * gx] : make_float2(0.f, 0.f);
-    }
+      for (int n2 = 0; n2 < R; ++n2) {
+        const int n = M * (t + R * n2) + m;
+        const bool take = live && n2 < R / 2 && n < nin;
+        if constexpr (GA)
+          v[n2] = gathered<decltype(mapc)::value>(ba, c, (size_t)n * gx + kx, take);
+        else
+          v[n2] = take ? src[(size_t)n * gx] : make_float2(0.f, 0.f);
+      }
+    };
+    if (GA && ba.gmap)
+      load(std::true_type{});
+    else
+      load(std::false_type{});
     __syncthreads();
     fft2_fwd_s<R, R, +1>(v, xf, twS, t, sv, PS);  // v[i] = Y_m[t + 16 i]
 #pragma unroll
