@@ -800,7 +800,9 @@ def main():
         dom_bytes, dom_ms, dom_name = pb[dom], passes[dom], per_pass[dom]["kernel"]
     else:
         # SLICES interleaves its kernels over two streams: whole-MATVEC figure
+        # (no per-pass counter summary: traffic is null)
         per_pass = []
+        dom = None
         dom_bytes, dom_ms, dom_name = total_bytes, passes[5], "slices_matvec (all kernels)"
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
     roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
